@@ -1,0 +1,152 @@
+"""Initial fields of the benchmark and parity cases, written as FEN user
+programs write them (interior values, then update_ghost_nodes).
+
+The perturbations come from a counter-based SplitMix64 generator so that the
+oracle and the GPU receive bit-identical inputs on any machine:
+value(component c, Fortran linear index q) = mix(seed + (c * 2**40 + q + 1) * GOLDEN).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ._lib import BC_PERIODIC, BC_WALL, P, U, V, W
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+SEED = 20261202
+
+
+def splitmix64(counter):
+    z = counter.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(component, n, seed=SEED):
+    """n uniforms in [0,1) for one component stream (53-bit mantissa)."""
+    q = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        ctr = np.uint64(seed) + (np.uint64(component) * np.uint64(1 << 40) + q + np.uint64(1)) * GOLDEN
+    return (splitmix64(ctr) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def gaussian(component, n, seed=SEED):
+    """Box-Muller normals from two uniform streams (2c, 2c+1)."""
+    u1 = uniform01(2 * component + 100, n, seed)
+    u2 = uniform01(2 * component + 101, n, seed)
+    u1 = np.where(u1 <= 0.0, 1e-300, u1)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+class Case:
+    """Grid, physics and initial interior fields of one run."""
+
+    def __init__(self, name, dim, n, L, bc, density, viscosity, g, dt, dt_o, steps, fields, cfl=1.0):
+        self.name, self.dim, self.n, self.L, self.bc = name, dim, n, L, bc
+        self.density, self.viscosity, self.g = density, viscosity, g
+        self.dt, self.dt_o, self.steps, self.fields, self.cfl = dt, dt_o, steps, fields, cfl
+
+    def apply(self, solver, upload):
+        """Configure a solver (oracle or GPU) and write the initial fields.
+
+        `upload(field, interior_array)` writes interior values; ghosts are then
+        filled with update_ghost_nodes as FEN's init_fields routines do.
+        """
+        solver.set_params(density=self.density, viscosity=self.viscosity, g=self.g, cfl=self.cfl)
+        solver.set_dt_o(self.dt_o)
+        for fid, arr in self.fields.items():
+            upload(fid, arr)
+        for fid in self.fields:
+            solver.update_ghost_nodes(fid)
+
+
+def _coords(n, delta):
+    return np.arange(1, n + 1, dtype=np.float64)
+
+
+def tgv2d(N=128, nu=0.01, dt=1e-3, steps=1000):
+    """2-D Taylor-Green vortex, doubly periodic [0,2pi]^2.
+
+    u = sin x cos y at (i d, (j-1/2) d), v = -cos x sin y at ((i-1/2) d, j d),
+    p = -(cos 2x + cos 2y)/4 at centres (the stated initial data of the
+    2-D benchmark case; its pressure sign is washed out by the first projection).
+    """
+    L = 2.0 * math.pi
+    d = L / float(np.float32(N))
+    i = np.arange(1, N + 1, dtype=np.float64)
+    I, J = np.meshgrid(i, i, indexing="ij")
+    u = (np.sin(I * d) * np.cos((J - 0.5) * d))[:, :, None]
+    v = (-np.cos((I - 0.5) * d) * np.sin(J * d))[:, :, None]
+    p = (-0.25 * (np.cos(2.0 * (I - 0.5) * d) + np.cos(2.0 * (J - 0.5) * d)))[:, :, None]
+    return Case("tgv2d", 2, (N, N, 1), (L, L, L / N), [BC_PERIODIC] * 6, 1.0, nu, (0.0, 0.0, 0.0), dt, dt, steps,
+                {U: u, V: v, P: p})
+
+
+def tgv3d(N=256, Re=1600.0, steps=200, eps=1e-3, cfl=0.5, Uref=2.0, seed=SEED):
+    """3-D Taylor-Green vortex, triply periodic [0,2pi]^3, plus a seeded
+    uniform perturbation eps*U(-1,1) on u, v, w.
+
+    dt follows set_timestep with CFL = 0.5 and U = 2 (dt = delta/4 here).
+    Initial pressure p = (cos 2x + cos 2y)(cos 2z + 2)/16 at centres.
+    """
+    L = 2.0 * math.pi
+    d = L / float(np.float32(N))
+    nu = 1.0 / Re
+    c = (np.arange(1, N + 1, dtype=np.float64) - 0.5) * d
+    f = np.arange(1, N + 1, dtype=np.float64) * d
+    n3 = N ** 3
+
+    def grid(a, b, cc):
+        return np.meshgrid(a, b, cc, indexing="ij")
+
+    X, Y, Z = grid(f, c, c)
+    u = np.sin(X) * np.cos(Y) * np.cos(Z)
+    X, Y, Z = grid(c, f, c)
+    v = -np.cos(X) * np.sin(Y) * np.cos(Z)
+    w = np.zeros((N, N, N))
+    for comp, arr in enumerate((u, v, w)):
+        arr += eps * (2.0 * uniform01(comp, n3, seed).reshape((N, N, N), order="F") - 1.0)
+    X, Y, Z = grid(c, c, c)
+    p = (np.cos(2.0 * X) + np.cos(2.0 * Y)) * (np.cos(2.0 * Z) + 2.0) / 16.0
+    dt_conv = cfl * d / Uref
+    dt_visc = (1.0 / 6.0) * d * d * 1.0 / nu
+    dt = min(dt_conv, dt_visc)
+    return Case("tgv3d", 3, (N, N, N), (L, L, L), [BC_PERIODIC] * 6, 1.0, nu, (0.0, 0.0, 0.0), dt, dt, steps,
+                {U: np.asfortranarray(u), V: np.asfortranarray(v), W: np.asfortranarray(w), P: np.asfortranarray(p)},
+                cfl=cfl)
+
+
+def channel(nx=512, ny=512, nz=256, Ub=15.7, noise=0.05, nu=1.0 / 180.0, steps=100, cfl=0.5, seed=SEED):
+    """Plane channel in FEN axis order: x, y periodic, walls at z (ppn).
+
+    delta = 1/128 at the default size (Lz = 2, h = 1), constant pressure-gradient
+    forcing g = (1, 0, 0) (u_tau = 1).  Initial u = 1.5 Ub (1 - (z-1)^2) plus
+    Gaussian noise of `noise*Ub` on u, v, w.  dt from set_timestep with
+    CFL = 0.5 and U = the centreline velocity 1.5 Ub.
+    """
+    d = 2.0 / nz
+    Lx, Ly, Lz = nx * d, ny * d, 2.0
+    zc = (np.arange(1, nz + 1, dtype=np.float64) - 0.5) * d
+    n3 = nx * ny * nz
+    shp = (nx, ny, nz)
+    u = np.broadcast_to(1.5 * Ub * (1.0 - (zc - 1.0) ** 2), shp).copy(order="F")
+    v = np.zeros(shp, order="F")
+    w = np.zeros(shp, order="F")
+    for comp, arr in enumerate((u, v, w)):
+        arr += noise * Ub * gaussian(comp, n3, seed).reshape(shp, order="F")
+    w[:, :, -1] = 0.0   # the top face of the last cell is the wall
+    p = np.zeros(shp, order="F")
+    dt_conv = cfl * d / (1.5 * Ub)
+    dt_visc = (1.0 / 6.0) * d * d / nu
+    dt = min(dt_conv, dt_visc)
+    bc = [BC_PERIODIC, BC_PERIODIC, BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL]
+    return Case("channel", 3, shp, (Lx, Ly, Lz), bc, 1.0, nu, (1.0, 0.0, 0.0), dt, dt, steps,
+                {U: u, V: v, W: w, P: p}, cfl=cfl)
+
+
+def interior(shape_ghosted):
+    return tuple(slice(1, s - 1) for s in shape_ghosted)

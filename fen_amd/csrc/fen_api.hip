@@ -1,0 +1,751 @@
+// fen_api.hip -- the extern "C" boundary of libfen (declared in include/fen.h).
+//
+// Orchestrates one navier_stokes_solver step (src/navier_stokes.f90:50-136)
+// as a fixed sequence of kernels on the context's stream:
+//   K1 momentum (u -> u*, dv_o)        ghost(u*)
+//   K2 div(u*)*rho/dt + x r2c          K3..K5 y / z passes (ppp, ppn, pp)
+//   K6 x c2r -> phi (+ mean, pp/ppn)   ghost(phi)
+//   K7 projection + p update           ghost(u, p)
+//   K8 checks
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fen_internal.hpp"
+
+using fen::Box;
+using fen::GhostDesc;
+
+namespace fen {
+
+void prof_begin(fen_ctx *c, int k)
+{
+    if (!c->prof_on) return;
+    ProfSlot &s = c->prof[k];
+    if (s.used + 2 > s.ev.size()) {
+        hipEvent_t e0, e1;
+        FEN_HIP_CHECK(hipEventCreate(&e0));
+        FEN_HIP_CHECK(hipEventCreate(&e1));
+        s.ev.push_back(e0);
+        s.ev.push_back(e1);
+    }
+    FEN_HIP_CHECK(hipEventRecord(s.ev[s.used], c->stream));
+}
+
+void prof_end(fen_ctx *c, int k, double bytes)
+{
+    if (!c->prof_on) return;
+    ProfSlot &s = c->prof[k];
+    FEN_HIP_CHECK(hipEventRecord(s.ev[s.used + 1], c->stream));
+    s.used += 2;
+    s.bytes = bytes;
+    s.launches++;
+}
+}  // namespace fen
+
+namespace {
+
+const char *kKernelNames[fen::K_COUNT] = {
+    "momentum", "ghost", "div_r2c", "y_fwd", "z_solve", "thomas", "y_inv", "y_solve", "c2r", "project",
+    "checks", "reduce"};
+
+int fail(fen_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+bool pow2(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+
+int field_slot(int field)   // index into bc_type / d_plane
+{
+    return field >= FEN_P && field <= FEN_W ? field : -1;
+}
+
+char field_loc(int field)
+{
+    switch (field) {
+    case FEN_U: return 'x';
+    case FEN_V: return 'y';
+    case FEN_W: return 'z';
+    default: return 'c';
+    }
+}
+
+// BC mapping of allocate_navier_stokes_fields (src/navier_stokes.f90:779-1017).
+int map_bc(int field, int face, int pc, bool *ok)
+{
+    const bool vel = field == FEN_U || field == FEN_V || field == FEN_W;
+    *ok = true;
+    switch (pc) {
+    case FEN_BC_PERIODIC: return 0;
+    case FEN_BC_WALL: return vel ? 1 : 2;
+    case FEN_BC_INFLOW:
+        if (face == FEN_BACK) { *ok = false; return 0; }
+        if (!vel) return 2;
+        if (face == FEN_LEFT || face == FEN_RIGHT) return field == FEN_U ? 1 : 2;
+        return 1;
+    case FEN_BC_OUTFLOW:
+        if (face == FEN_FRONT) { *ok = false; return 0; }
+        return vel ? 2 : 1;
+    }
+    *ok = false;
+    return 0;
+}
+
+// Modified wavenumbers (src/poisson.f90:141,628,646,664), glibc cos as gfortran's dcos.
+std::vector<double> modified_wavenumbers(int n, double delta, double factor)
+{
+    std::vector<double> m(n);
+    const double pi = std::acos(-1.0);
+    for (int i = 1; i <= n; i++)
+        m[i - 1] = 2.0 * (std::cos(factor * pi * ((double)i - 1.0) / (double)(float)n) - 1.0) / (delta * delta);
+    return m;
+}
+
+std::vector<double2> twiddles(int n)
+{
+    std::vector<double2> w(n);
+    const long double tpi = 2.0L * std::acos(-1.0L);
+    for (int m = 0; m < n; m++) {
+        const long double a = tpi * (long double)m / (long double)n;
+        w[m] = double2{(double)std::cos(a), (double)(-std::sin(a))};
+    }
+    return w;
+}
+
+template <typename T> T *dev_alloc(size_t n)
+{
+    void *p = nullptr;
+    FEN_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+    FEN_HIP_CHECK(hipMemset(p, 0, n * sizeof(T)));
+    return static_cast<T *>(p);
+}
+
+template <typename T> T *dev_copy(const std::vector<T> &h)
+{
+    T *d = dev_alloc<T>(h.size());
+    FEN_HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+size_t plane_count(const fen_ctx *c, int face)
+{
+    const size_t gx = c->nx + 2, gy = c->ny + 2, gz = c->nz + 2;
+    if (face < 2) return gy * gz;
+    if (face < 4) return gx * gz;
+    return gx * gy;
+}
+
+GhostDesc make_desc(fen_ctx *c, int field, double *arr)
+{
+    GhostDesc d{};
+    d.f = arr;
+    const int s = field_slot(field);
+    for (int f = 0; f < 6; f++) {
+        d.type[f] = c->bc_type[s][f];
+        d.plane[f] = c->d_plane[s][f];
+    }
+    d.loc = field_loc(field);
+    return d;
+}
+
+double *field_ptr(fen_ctx *c, int field)
+{
+    switch (field) {
+    case FEN_P: return c->d_p;
+    case FEN_PHI: return c->d_phi;
+    case FEN_U: return c->d_u[0];
+    case FEN_V: return c->d_u[1];
+    case FEN_W: return c->d_u[2];
+    case FEN_DVO_X: case FEN_DVO_Y: case FEN_DVO_Z: return c->d_dvo[field - FEN_DVO_X];
+    case FEN_S_X: case FEN_S_Y: case FEN_S_Z: return c->d_S[field - FEN_S_X];
+    }
+    return nullptr;
+}
+
+void free_all(fen_ctx *c)
+{
+    auto f = [](void *p) { if (p) (void)hipFree(p); };
+    for (int d = 0; d < 3; d++) { f(c->d_u[d]); f(c->d_us[d]); f(c->d_dvo[d]); f(c->d_S[d]); }
+    f(c->d_p); f(c->d_phi); f(c->d_spec); f(c->d_c1);
+    f(c->d_mwx); f(c->d_mwy); f(c->d_mwz); f(c->d_ta); f(c->d_tb); f(c->d_tc);
+    f(c->d_twx); f(c->d_twy); f(c->d_twz); f(c->d_part); f(c->d_scal);
+    for (int s = 0; s < 5; s++) for (int p = 0; p < 6; p++) f(c->d_plane[s][p]);
+    if (c->h_scal) (void)hipHostFree(c->h_scal);
+    for (auto &s : c->prof) for (auto e : s.ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+void sync(fen_ctx *c) { FEN_HIP_CHECK(hipStreamSynchronize(c->stream)); }
+
+// Fortran Ew.d edit descriptor with d = 6 or 8 (0.ddddddE+xx), right-justified.
+std::string fortran_e(double x, int w, int d)
+{
+    char buf[64];
+    if (std::isnan(x)) { snprintf(buf, sizeof buf, "%*s", w, "NaN"); return buf; }
+    if (std::isinf(x)) { snprintf(buf, sizeof buf, "%*s", w, x > 0 ? "Infinity" : "-Infinity"); return buf; }
+    std::string body;
+    if (x == 0.0) {
+        body = std::string(std::signbit(x) ? "-" : "") + "0." + std::string(d, '0') + "E+00";
+    } else {
+        char tmp[64];
+        snprintf(tmp, sizeof tmp, "%.*E", d - 1, std::fabs(x));   // d.ddddE+xx
+        std::string s(tmp);
+        const size_t epos = s.find('E');
+        const int e = std::atoi(s.c_str() + epos + 1) + 1;
+        std::string digits = s.substr(0, 1) + s.substr(2, epos - 2);
+        char ex[16];
+        if (std::abs(e) <= 99) snprintf(ex, sizeof ex, "E%c%02d", e < 0 ? '-' : '+', std::abs(e));
+        else snprintf(ex, sizeof ex, "%c%03d", e < 0 ? '-' : '+', std::abs(e));
+        body = std::string(x < 0 ? "-" : "") + "0." + digits + ex;
+    }
+    snprintf(buf, sizeof buf, "%*s", w, body.c_str());
+    return buf;
+}
+
+// One time step; see the kernel sequence at the top of this file.
+void step_fp64(fen_ctx *c, double dt)
+{
+    using namespace fen;
+    const bool d3 = c->dim == 3;
+    const double A = 1.0 + 0.5 * dt / c->dt_o;          // src/navier_stokes.f90:157-158
+    const double B = -(0.5 * dt / c->dt_o);
+    c->last_dt = dt;
+
+    launch_momentum(c, dt, A, B);
+    {
+        GhostDesc g[3] = {make_desc(c, FEN_U, c->d_us[0]), make_desc(c, FEN_V, c->d_us[1]),
+                          make_desc(c, FEN_W, c->d_us[2])};
+        launch_ghost_desc(c, g, d3 ? 3 : 2);
+    }
+    const double *vs[3] = {c->d_us[0], c->d_us[1], c->d_us[2]};
+    const double post_x = c->solver == FEN_SOLVER_PPN ? 1.0 / (double)c->nx : 1.0;
+    launch_div_r2c_impl(c, vs, nullptr, 0, 0, 0, c->density, dt, post_x);
+    switch (c->solver) {
+    case FEN_SOLVER_PPP:
+        launch_y_fft(c, -1, 1.0);
+        launch_z_solve(c, 1.0 / (double)(float)((long)c->nx * c->ny * c->nz));
+        launch_y_fft(c, +1, 1.0);
+        break;
+    case FEN_SOLVER_PPN:
+        launch_y_fft(c, -1, 1.0 / (double)c->ny);
+        launch_thomas(c);
+        launch_y_fft(c, +1, 1.0);
+        break;
+    case FEN_SOLVER_PP:
+        launch_y_solve_2d(c, 1.0 / (double)(float)((long)c->nx * c->ny));
+        break;
+    default: throw std::runtime_error("solver not available");
+    }
+    const bool mean = c->solver == FEN_SOLVER_PP || c->solver == FEN_SOLVER_PPN;
+    launch_c2r(c, c->d_phi, 1, c->box.gx, c->box.gxy, mean);
+    c->shift_active = mean;
+    if (mean) {
+        bool dirichlet = false;
+        for (int f = 0; f < 6; f++) dirichlet = dirichlet || c->bc_type[FEN_PHI][f] == 1;
+        if (dirichlet) {   // the shift does not commute with 2*bc - f: apply it first
+            launch_apply_shift(c, c->d_phi, c->n_ghost);
+            c->shift_active = false;
+        }
+    }
+    {
+        GhostDesc g = make_desc(c, FEN_PHI, c->d_phi);
+        launch_ghost_desc(c, &g, 1);
+    }
+    launch_project(c, dt);
+    {
+        GhostDesc g[4] = {make_desc(c, FEN_U, c->d_u[0]), make_desc(c, FEN_V, c->d_u[1]),
+                          d3 ? make_desc(c, FEN_W, c->d_u[2]) : make_desc(c, FEN_P, c->d_p),
+                          make_desc(c, FEN_P, c->d_p)};
+        launch_ghost_desc(c, g, d3 ? 4 : 3);
+    }
+    const double *vn[3] = {c->d_u[0], c->d_u[1], c->d_u[2]};
+    launch_checks(c, vn, c->d_scal);
+    c->checks_valid = true;
+}
+
+}  // namespace
+
+#define FEN_TRY(ctx, ...)                                              \
+    try {                                                              \
+        __VA_ARGS__;                                                   \
+    } catch (const std::exception &e_) {                               \
+        return fail(ctx, FEN_ERR_HIP, e_.what());                      \
+    }
+
+extern "C" {
+
+int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double ly, double lz, const double *x0,
+               const int *bc, int precision)
+{
+    if (!out) return FEN_ERR_ARG;
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return FEN_ERR_ARG;
+    if (!bc) return FEN_ERR_ARG;
+    if (dim == 2) nz = 1;
+    if (precision != FEN_FP64) return FEN_ERR_UNSUPPORTED;
+    fen_ctx *c = new fen_ctx();
+    c->dim = dim; c->nx = nx; c->ny = ny; c->nz = nz;
+    c->lx = lx; c->ly = ly; c->lz = lz;
+    for (int d = 0; d < 3; d++) c->x0[d] = x0 ? x0[d] : 0.0;
+    for (int s = 0; s < 6; s++) c->bc_phys[s] = (dim == 3 || s < 4) ? bc[s] : FEN_BC_PERIODIC;
+    c->precision = precision;
+    c->delta = lx / (double)(float)nx;                                   // src/grid.f90:140
+    c->periodic[0] = c->bc_phys[0] == FEN_BC_PERIODIC && c->bc_phys[1] == FEN_BC_PERIODIC;
+    c->periodic[1] = c->bc_phys[2] == FEN_BC_PERIODIC && c->bc_phys[3] == FEN_BC_PERIODIC;
+    c->periodic[2] = dim == 3 ? (c->bc_phys[4] == FEN_BC_PERIODIC && c->bc_phys[5] == FEN_BC_PERIODIC) : 1;
+    const int *pb = c->periodic;
+    if (dim == 3) {
+        if (pb[0] && pb[1] && pb[2]) c->solver = FEN_SOLVER_PPP;
+        else if (pb[0] && pb[1] && !pb[2]) c->solver = FEN_SOLVER_PPN;
+        else { delete c; return FEN_ERR_UNSUPPORTED; }
+    } else {
+        if (pb[0] && pb[1]) c->solver = FEN_SOLVER_PP;
+        else { delete c; return FEN_ERR_UNSUPPORTED; }
+    }
+    if (!pow2(nx) || !pow2(ny) || (c->solver == FEN_SOLVER_PPP && !pow2(nz)) ||
+        (dim == 3 && nz < 2)) {
+        delete c;
+        return FEN_ERR_UNSUPPORTED;
+    }
+    for (int s = 0; s < 5; s++)
+        for (int f = 0; f < 6; f++) {
+            bool ok = true;
+            c->bc_type[s][f] = (dim == 3 || f < 4) ? map_bc(s, f, c->bc_phys[f], &ok) : 0;
+            if (!ok) { delete c; return FEN_ERR_UNSUPPORTED; }
+        }
+    Box &b = c->box;
+    b.nx = nx; b.ny = ny; b.nz = nz; b.dim = dim;
+    b.gx = nx + 2;
+    b.gxy = (long)(nx + 2) * (ny + 2);
+    c->n_ghost = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    c->n_int = (size_t)nx * ny * nz;
+    c->hx = nx / 2 + 1;
+    try {
+        FEN_HIP_CHECK(hipStreamCreate(&c->stream));
+        for (int d = 0; d < 3; d++) {
+            c->d_u[d] = dev_alloc<double>(c->n_ghost);
+            c->d_us[d] = dev_alloc<double>(c->n_ghost);
+            c->d_dvo[d] = dev_alloc<double>(c->n_int);
+        }
+        c->d_p = dev_alloc<double>(c->n_ghost);
+        c->d_phi = dev_alloc<double>(c->n_ghost);
+        for (int s = 0; s < 5; s++)
+            for (int f = 0; f < 6; f++) c->d_plane[s][f] = dev_alloc<double>(plane_count(c, f));
+        const size_t nspec = (size_t)c->hx * ny * nz;
+        c->d_spec = dev_alloc<double2>(nspec);
+        if (c->solver == FEN_SOLVER_PPN) c->d_c1 = dev_alloc<double>(nspec);
+        c->d_mwx = dev_copy(modified_wavenumbers(nx, c->delta, 2.0));
+        c->d_mwy = dev_copy(modified_wavenumbers(ny, c->delta, 2.0));
+        if (c->solver == FEN_SOLVER_PPP) c->d_mwz = dev_copy(modified_wavenumbers(nz, c->delta, 2.0));
+        if (c->solver == FEN_SOLVER_PPN) {
+            // src/poisson.f90:741-757
+            const double d2 = c->delta * c->delta;
+            std::vector<double> a(nz, 1.0 / d2), bb(nz, -(2.0 / d2)), cc(nz, 1.0 / d2);
+            bb[0] = bb[0] + a[0];
+            if (c->bc_phys[4] == FEN_BC_INFLOW && c->bc_phys[5] == FEN_BC_OUTFLOW) bb[nz - 1] = bb[nz - 1] - cc[nz - 1];
+            else bb[nz - 1] = bb[nz - 1] + cc[nz - 1];
+            a[0] = 0.0;
+            cc[nz - 1] = 0.0;
+            c->d_ta = dev_copy(a); c->d_tb = dev_copy(bb); c->d_tc = dev_copy(cc);
+        }
+        c->d_twx = dev_copy(twiddles(nx));
+        c->d_twy = dev_copy(twiddles(ny));
+        if (c->solver == FEN_SOLVER_PPP) c->d_twz = dev_copy(twiddles(nz));
+        c->n_part = std::max<size_t>(3 * 1024, (size_t)ny * nz / 2 + 1);
+        c->d_part = dev_alloc<double>(c->n_part);
+        c->d_scal = dev_alloc<double>(8);
+        c->d_shift = c->d_scal + 3;
+        FEN_HIP_CHECK(hipHostMalloc((void **)&c->h_scal, 8 * sizeof(double)));
+        FEN_HIP_CHECK(hipDeviceSynchronize());
+    } catch (const std::exception &) {
+        free_all(c);
+        delete c;
+        return FEN_ERR_HIP;
+    }
+    *out = c;
+    return FEN_OK;
+}
+
+int fen_destroy(fen_ctx *c)
+{
+    if (!c) return FEN_ERR_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return FEN_OK;
+}
+
+const char *fen_last_error(const fen_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int fen_set_params(fen_ctx *c, double density, double viscosity, const double *g, double cfl, int constant_cfl)
+{
+    if (!c) return FEN_ERR_ARG;
+    if (!(density > 0.0) || !(viscosity >= 0.0)) return fail(c, FEN_ERR_ARG, "density must be > 0, viscosity >= 0");
+    c->density = density;
+    c->viscosity = viscosity;
+    for (int d = 0; d < 3; d++) c->g[d] = g ? g[d] : 0.0;
+    c->cfl = cfl;
+    c->constant_cfl = constant_cfl;
+    return FEN_OK;
+}
+
+int fen_set_timestep(fen_ctx *c, double U, double *dt)
+{
+    if (!c || !dt) return FEN_ERR_ARG;
+    // src/navier_stokes.f90:641-664 (single-phase branch)
+    const double d = c->delta;
+    c->dt_conv = c->cfl * d / U;
+    c->dt_visc = 0.125 * d * d * c->density / c->viscosity;
+    if (c->dim == 3) c->dt_visc = (1.0 / 6.0) * d * d * c->density / c->viscosity;
+    *dt = c->dt_conv < c->dt_visc ? c->dt_conv : c->dt_visc;
+    c->dt_o = *dt;
+    c->have_dt_o = c->have_dt_visc = true;
+    return FEN_OK;
+}
+
+int fen_set_dt_o(fen_ctx *c, double dt_o)
+{
+    if (!c) return FEN_ERR_ARG;
+    c->dt_o = dt_o;
+    c->have_dt_o = true;
+    return FEN_OK;
+}
+
+int fen_get_scalars(fen_ctx *c, double *maxdiv, double *maxcfl, double *dt_o, double *dt_visc, double *dt_conv)
+{
+    if (!c) return FEN_ERR_ARG;
+    FEN_TRY(c, {
+        if (dt_o) *dt_o = c->dt_o;
+        if (dt_visc) *dt_visc = c->dt_visc;
+        if (dt_conv) *dt_conv = c->dt_conv;
+        if (maxdiv || maxcfl) {
+            if (!c->checks_valid) return fail(c, FEN_ERR_STATE, "no step has run since the last upload");
+            FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal, c->d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            sync(c);
+            if (maxdiv) *maxdiv = c->h_scal[0];
+            if (maxcfl) *maxcfl = c->last_dt * c->h_scal[1] / c->delta;    // src/navier_stokes.f90:617
+            if (c->h_scal[2] != 0.0) return fail(c, FEN_ERR_NONFINITE, "non-finite velocity or divergence");
+        }
+    });
+    return FEN_OK;
+}
+
+int fen_field_count(const fen_ctx *c, int field, size_t *count)
+{
+    if (!c || !count) return FEN_ERR_ARG;
+    if (field < FEN_P || field > FEN_S_Z) return FEN_ERR_ARG;
+    *count = field <= FEN_W ? c->n_ghost : c->n_int;
+    return FEN_OK;
+}
+
+int fen_upload(fen_ctx *c, int field, const double *host, size_t count)
+{
+    if (!c || !host) return FEN_ERR_ARG;
+    size_t n = 0;
+    if (fen_field_count(c, field, &n) != FEN_OK || n != count)
+        return fail(c, FEN_ERR_ARG, "field id or element count does not match the grid");
+    if (field == FEN_W && c->dim == 2) return fail(c, FEN_ERR_ARG, "no w component in 2-D");
+    FEN_TRY(c, {
+        if (field >= FEN_S_X && !c->d_S[0]) {
+            for (int d = 0; d < 3; d++) c->d_S[d] = dev_alloc<double>(c->n_int);
+        }
+        if (field >= FEN_S_X) c->has_source = true;
+        double *dst = field_ptr(c, field);
+        FEN_HIP_CHECK(hipMemcpyAsync(dst, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        sync(c);
+        if (field == FEN_PHI) c->shift_active = false;
+        if (field >= FEN_U && field <= FEN_W) c->checks_valid = false;
+    });
+    return FEN_OK;
+}
+
+int fen_download(fen_ctx *c, int field, double *host, size_t count)
+{
+    if (!c || !host) return FEN_ERR_ARG;
+    size_t n = 0;
+    if (fen_field_count(c, field, &n) != FEN_OK || n != count)
+        return fail(c, FEN_ERR_ARG, "field id or element count does not match the grid");
+    FEN_TRY(c, {
+        const double *src = field_ptr(c, field);
+        if (!src) {   // source never set: zero, as allocated by the reference
+            std::memset(host, 0, n * sizeof(double));
+            return FEN_OK;
+        }
+        FEN_HIP_CHECK(hipMemcpyAsync(host, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (field == FEN_PHI && c->shift_active)
+            FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal + 3, c->d_shift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        if (field == FEN_PHI && c->shift_active) {
+            const double s = c->h_scal[3];
+            for (size_t q = 0; q < n; q++) host[q] = host[q] - s;
+        }
+    });
+    return FEN_OK;
+}
+
+int fen_set_bc_plane(fen_ctx *c, int field, int face, const double *plane, size_t count)
+{
+    if (!c || !plane) return FEN_ERR_ARG;
+    const int s = field_slot(field);
+    if (s < 0 || face < 0 || face > 5) return fail(c, FEN_ERR_ARG, "bad field or face");
+    if (count != plane_count(c, face)) return fail(c, FEN_ERR_ARG, "plane size does not match the face");
+    FEN_TRY(c, {
+        FEN_HIP_CHECK(hipMemcpyAsync(c->d_plane[s][face], plane, count * sizeof(double), hipMemcpyHostToDevice,
+                                     c->stream));
+        sync(c);
+    });
+    return FEN_OK;
+}
+
+int fen_set_bc_type(fen_ctx *c, int field, int face, int type)
+{
+    if (!c) return FEN_ERR_ARG;
+    const int s = field_slot(field);
+    if (s < 0 || face < 0 || face > 5 || type < 0 || type > 2) return fail(c, FEN_ERR_ARG, "bad field, face or type");
+    c->bc_type[s][face] = type;
+    return FEN_OK;
+}
+
+int fen_get_bc_type(const fen_ctx *c, int field, int face, int *type)
+{
+    if (!c || !type) return FEN_ERR_ARG;
+    const int s = field_slot(field);
+    if (s < 0 || face < 0 || face > 5) return FEN_ERR_ARG;
+    *type = c->bc_type[s][face];
+    return FEN_OK;
+}
+
+int fen_update_ghost_nodes(fen_ctx *c, int field)
+{
+    if (!c) return FEN_ERR_ARG;
+    if (field_slot(field) < 0) return fail(c, FEN_ERR_ARG, "field has no ghost nodes");
+    if (field == FEN_W && c->dim == 2) return fail(c, FEN_ERR_ARG, "no w component in 2-D");
+    FEN_TRY(c, {
+        if (field == FEN_PHI && c->shift_active) {
+            fen::launch_apply_shift(c, c->d_phi, c->n_ghost);
+            c->shift_active = false;
+        }
+        GhostDesc g = make_desc(c, field, field_ptr(c, field));
+        fen::launch_ghost_desc(c, &g, 1);
+        FEN_HIP_CHECK(hipGetLastError());
+    });
+    return FEN_OK;
+}
+
+int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl)
+{
+    (void)step;
+    if (!c || !dt) return FEN_ERR_ARG;
+    if (!c->have_dt_o) return fail(c, FEN_ERR_STATE, "dt_o is undefined: call fen_set_timestep or fen_set_dt_o");
+    FEN_TRY(c, {
+        if (c->constant_cfl) {
+            // update_timestep (src/navier_stokes.f90:670-730)
+            if (!c->have_dt_visc) return fail(c, FEN_ERR_STATE, "constant CFL needs fen_set_timestep first");
+            if (!c->checks_valid) {
+                const double *vn[3] = {c->d_u[0], c->d_u[1], c->d_u[2]};
+                fen::launch_checks(c, vn, c->d_scal);
+            }
+            FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal, c->d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            sync(c);
+            c->dt_o = *dt;
+            const double mv = c->h_scal[1];
+            c->dt_conv = mv > 0.0 ? c->cfl * c->delta / mv : 1.0;
+            *dt = c->dt_conv < c->dt_visc ? c->dt_conv : c->dt_visc;
+            if (*dt > 1.1 * c->dt_o) *dt = 1.1 * c->dt_o;
+        }
+        step_fp64(c, *dt);
+        FEN_HIP_CHECK(hipGetLastError());
+    });
+    if (maxdiv || maxcfl) return fen_get_scalars(c, maxdiv, maxcfl, nullptr, nullptr, nullptr);
+    return FEN_OK;
+}
+
+int fen_ns_run(fen_ctx *c, int step0, int nsteps, double *dt)
+{
+    if (!c || !dt || nsteps < 0) return FEN_ERR_ARG;
+    for (int s = 0; s < nsteps; s++) {
+        const int rc = fen_ns_step(c, step0 + 1 + s, dt, nullptr, nullptr);
+        if (rc != FEN_OK) return rc;
+    }
+    return FEN_OK;
+}
+
+int fen_poisson_solve(fen_ctx *c, double *f, int gl)
+{
+    if (!c || !f || (gl != 0 && gl != 1)) return FEN_ERR_ARG;
+    const size_t n = gl ? c->n_ghost : c->n_int;
+    const long sx = gl ? c->nx + 2 : c->nx;
+    const long sy = gl ? (long)(c->nx + 2) * (c->ny + 2) : (long)c->nx * c->ny;
+    FEN_TRY(c, {
+        double *buf = c->d_us[0];   // scratch between steps
+        FEN_HIP_CHECK(hipMemcpyAsync(buf, f, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        const double post_x = c->solver == FEN_SOLVER_PPN ? 1.0 / (double)c->nx : 1.0;
+        fen::launch_div_r2c_impl(c, nullptr, buf, sx, sy, gl, 1.0, 1.0, post_x);
+        switch (c->solver) {
+        case FEN_SOLVER_PPP:
+            fen::launch_y_fft(c, -1, 1.0);
+            fen::launch_z_solve(c, 1.0 / (double)(float)((long)c->nx * c->ny * c->nz));
+            fen::launch_y_fft(c, +1, 1.0);
+            break;
+        case FEN_SOLVER_PPN:
+            fen::launch_y_fft(c, -1, 1.0 / (double)c->ny);
+            fen::launch_thomas(c);
+            fen::launch_y_fft(c, +1, 1.0);
+            break;
+        case FEN_SOLVER_PP:
+            fen::launch_y_solve_2d(c, 1.0 / (double)(float)((long)c->nx * c->ny));
+            break;
+        default: return fail(c, FEN_ERR_UNSUPPORTED, "solver not available");
+        }
+        const bool mean = c->solver != FEN_SOLVER_PPP;
+        fen::launch_c2r(c, buf, gl, sx, sy, mean);
+        std::vector<double> host(n);
+        FEN_HIP_CHECK(hipMemcpyAsync(host.data(), buf, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (mean)
+            FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal + 3, c->d_shift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        if (c->solver == FEN_SOLVER_PPN && gl) {
+            // phi%f = 0 over the whole array before the c2r (src/poisson.f90:1151)
+            const size_t gx = c->nx + 2, gy = c->ny + 2;
+            for (size_t q = 0; q < n; q++) {
+                const size_t i = q % gx, j = (q / gx) % gy, k = q / (gx * gy);
+                const bool interior = i >= 1 && i <= (size_t)c->nx && j >= 1 && j <= (size_t)c->ny && k >= 1 &&
+                                      k <= (size_t)c->nz;
+                if (!interior) host[q] = 0.0;
+            }
+        }
+        if (mean) {
+            const double s = c->h_scal[3];
+            for (size_t q = 0; q < n; q++) host[q] = host[q] - s;   // whole array (:503, :1171)
+        }
+        std::memcpy(f, host.data(), n * sizeof(double));
+    });
+    return FEN_OK;
+}
+
+int fen_add_advection(fen_ctx *c, double *rx, double *ry, double *rz)
+{
+    if (!c || !rx || !ry || (c->dim == 3 && !rz)) return FEN_ERR_ARG;
+    FEN_TRY(c, {
+        double *r[3] = {c->d_us[0], c->d_us[1], c->d_us[2]};
+        double *h[3] = {rx, ry, rz};
+        const int nd = c->dim == 3 ? 3 : 2;
+        for (int d = 0; d < nd; d++)
+            FEN_HIP_CHECK(hipMemcpyAsync(r[d], h[d], c->n_int * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        fen::launch_advection_only(c, r[0], r[1], r[2]);
+        for (int d = 0; d < nd; d++)
+            FEN_HIP_CHECK(hipMemcpyAsync(h[d], r[d], c->n_int * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+    });
+    return FEN_OK;
+}
+
+int fen_format_status(fen_ctx *c, int step, double time, double dt, char *buf, size_t len)
+{
+    if (!c || !buf || len == 0) return FEN_ERR_ARG;
+    double md = 0.0, mc = 0.0;
+    int rc = fen_get_scalars(c, &md, &mc, nullptr, nullptr, nullptr);
+    if (rc != FEN_OK && rc != FEN_ERR_NONFINITE) return rc;
+    // (A6,I7,1x,A6,E13.6,1x,A4,E13.6,1x,A8,E13.6,1x,A9,1x,E13.6), src/navier_stokes.f90:746
+    std::string s = "step: ";
+    char ib[32];
+    snprintf(ib, sizeof ib, "%7d", step);
+    s += ib;
+    s += " time: " + fortran_e(time, 13, 6);
+    s += " dt: " + fortran_e(dt, 13, 6);
+    s += " maxdiv: " + fortran_e(md, 13, 6);
+    s += "  maxCFL:  " + fortran_e(mc, 13, 6);
+    snprintf(buf, len, "%s", s.c_str());
+    return rc;
+}
+
+int fen_write_grid_json(const fen_ctx *c, const char *path)
+{
+    if (!c || !path) return FEN_ERR_ARG;
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) return FEN_ERR_ARG;
+    // src/grid.f90:246-258
+    std::fprintf(fp, "{\n");
+    std::fprintf(fp, "    \"Grid\": {\n");
+    std::fprintf(fp, "        \"Nx\":  %7d,\n", c->nx);
+    std::fprintf(fp, "        \"Ny\":  %7d,\n", c->ny);
+    std::fprintf(fp, "        \"Nz\":  %7d,\n", c->nz);
+    std::fprintf(fp, "        \"origin\": [ %s,%s,%s],\n", fortran_e(c->x0[0], 16, 8).c_str(),
+                 fortran_e(c->x0[1], 16, 8).c_str(), fortran_e(c->x0[2], 16, 8).c_str());
+    std::fprintf(fp, "        \"Lx\":  %s,\n", fortran_e(c->lx, 16, 8).c_str());
+    std::fprintf(fp, "        \"Ly\":  %s,\n", fortran_e(c->ly, 16, 8).c_str());
+    std::fprintf(fp, "        \"Lz\":  %s\n", fortran_e(c->lz, 16, 8).c_str());
+    std::fprintf(fp, "      }\n");
+    std::fprintf(fp, "}\n");
+    std::fclose(fp);
+    return FEN_OK;
+}
+
+int fen_solver_kind(const fen_ctx *c, int *kind)
+{
+    if (!c || !kind) return FEN_ERR_ARG;
+    *kind = c->solver;
+    return FEN_OK;
+}
+
+int fen_grid_delta(const fen_ctx *c, double *delta)
+{
+    if (!c || !delta) return FEN_ERR_ARG;
+    *delta = c->delta;
+    return FEN_OK;
+}
+
+int fen_synchronize(fen_ctx *c)
+{
+    if (!c) return FEN_ERR_ARG;
+    FEN_TRY(c, sync(c));
+    return FEN_OK;
+}
+
+int fen_profile_enable(fen_ctx *c, int enable)
+{
+    if (!c) return FEN_ERR_ARG;
+    c->prof_on = enable != 0;
+    return FEN_OK;
+}
+
+int fen_profile_reset(fen_ctx *c)
+{
+    if (!c) return FEN_ERR_ARG;
+    FEN_TRY(c, {
+        sync(c);
+        for (auto &s : c->prof) { s.used = 0; s.ms = 0.0; s.launches = 0; }
+    });
+    return FEN_OK;
+}
+
+int fen_profile_count(void) { return fen::K_COUNT; }
+
+const char *fen_profile_name(int k) { return k >= 0 && k < fen::K_COUNT ? kKernelNames[k] : ""; }
+
+int fen_profile_read(fen_ctx *c, int k, double *total_ms, long long *launches, double *bytes)
+{
+    if (!c || k < 0 || k >= fen::K_COUNT) return FEN_ERR_ARG;
+    FEN_TRY(c, {
+        sync(c);
+        fen::ProfSlot &s = c->prof[k];
+        double ms = 0.0;
+        for (size_t p = 0; p + 1 < s.used; p += 2) {
+            float e = 0.0f;
+            FEN_HIP_CHECK(hipEventElapsedTime(&e, s.ev[p], s.ev[p + 1]));
+            ms += e;
+        }
+        if (total_ms) *total_ms = ms;
+        if (launches) *launches = s.launches;
+        if (bytes) *bytes = s.bytes;
+    });
+    return FEN_OK;
+}
+
+}  // extern "C"

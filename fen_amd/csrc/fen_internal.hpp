@@ -1,0 +1,156 @@
+// fen_internal.hpp -- shared definitions of libfen's host code and kernels.
+//
+// Data layout in HBM (all FP64 unless stated):
+//  * ghosted fields (p, phi, u, v, w and the predicted velocity u*, v*, w*):
+//    FEN's own Fortran layout, (nx+2)*(ny+2)*(nz+2), i fastest, so host
+//    uploads/downloads are plain copies.  In 2-D the k extent is 0:2 and only
+//    the k = 1 plane is live.
+//  * interior-only fields (dv_o, S): nx*ny*nz, i fastest.
+//  * half spectrum of the Poisson solve: complex (double2), laid out
+//    [kx][k][j] with j fastest, kx in 0..nx/2.  Every pass of the solver
+//    then reads and writes runs of >= 128 contiguous bytes: x-lines are
+//    transformed in LDS and scattered as j-runs, y-lines are contiguous,
+//    z-lines are strided by ny and are read as tiles of adjacent j.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fen.h"
+
+namespace fen {
+
+// Grid extents as seen by kernels.
+struct Box {
+    int nx, ny, nz, dim;
+    long gx, gxy;   // ghosted strides in j and k
+    __host__ __device__ long g(int i, int j, int k) const { return i + gx * j + gxy * k; }
+    __host__ __device__ long q(int i, int j, int k) const {
+        return (i - 1) + (long)nx * ((j - 1) + (long)ny * (k - 1));
+    }
+};
+
+// Ghost-fill description of one field (scalar%bc, src/scalar.f90:23-47).
+struct GhostDesc {
+    double *f;
+    const double *plane[6];
+    int type[6];
+    int loc;   // 'c', 'x', 'y', 'z'
+};
+
+// Kernels whose launches are timed by the profiler.
+enum KernelId {
+    K_MOMENTUM = 0,   // advection + diffusion + grad p + AB2 + update + dv_o
+    K_GHOST,          // ghost-node fill (any field set)
+    K_DIV_R2C,        // divergence * rho/dt fused with the x r2c transform
+    K_Y_FWD,          // y forward transform of the half spectrum
+    K_Z_SOLVE,        // z forward + eigenvalue divide + z inverse (ppp)
+    K_THOMAS,         // batched tridiagonal solve along z (ppn)
+    K_Y_INV,          // y inverse transform
+    K_Y_SOLVE,        // 2-D: y forward + divide + y inverse (pp)
+    K_C2R,            // x c2r -> phi (+ partial sums for the mean)
+    K_PROJECT,        // projection u = u* - dt grad(phi)/rho, p = p + phi
+    K_CHECKS,         // max div(u), max(|u|+|v|+|w|)
+    K_REDUCE,         // final small reductions
+    K_COUNT
+};
+
+struct ProfSlot {
+    std::vector<hipEvent_t> ev;   // begin/end pairs
+    size_t used = 0;
+    double ms = 0.0;
+    long long launches = 0;
+    double bytes = 0.0;
+};
+
+}  // namespace fen
+
+struct fen_ctx {
+    int dim = 3, nx = 0, ny = 0, nz = 0;
+    double lx = 0, ly = 0, lz = 0, x0[3] = {0, 0, 0}, delta = 0;
+    int bc_phys[6] = {0, 0, 0, 0, 0, 0};
+    int periodic[3] = {1, 1, 1};
+    int solver = FEN_SOLVER_PPP;
+    int precision = FEN_FP64;
+
+    // navier_stokes_mod scalars (src/navier_stokes.f90:17-45)
+    double density = 1.0, viscosity = 1.0, g[3] = {0, 0, 0}, cfl = 1.0;
+    int constant_cfl = 0;
+    double dt_o = 0.0, dt_visc = 0.0, dt_conv = 0.0;
+    bool have_dt_o = false, have_dt_visc = false;
+
+    fen::Box box{};
+    size_t n_ghost = 0, n_int = 0;
+
+    // bc types and device bc planes of p, phi, u, v, w
+    int bc_type[5][6] = {};
+    double *d_plane[5][6] = {};
+
+    // resident fields
+    double *d_u[3] = {nullptr, nullptr, nullptr};    // velocity (state)
+    double *d_us[3] = {nullptr, nullptr, nullptr};   // predicted velocity u*
+    double *d_p = nullptr, *d_phi = nullptr;
+    double *d_dvo[3] = {nullptr, nullptr, nullptr};
+    double *d_S[3] = {nullptr, nullptr, nullptr};
+    bool has_source = false;
+
+    // Poisson solver state
+    int hx = 0;                  // nx/2 + 1
+    double2 *d_spec = nullptr;   // [kx][k][j]
+    double *d_c1 = nullptr;      // Thomas scratch (ppn), [kx][k][j]
+    double *d_mwx = nullptr, *d_mwy = nullptr, *d_mwz = nullptr;
+    double *d_ta = nullptr, *d_tb = nullptr, *d_tc = nullptr;
+    double2 *d_twx = nullptr, *d_twy = nullptr, *d_twz = nullptr;   // W_n^m, m < n
+    double phi_shift = 0.0;      // mean/N removed lazily (pp, ppn)
+    double *d_shift = nullptr;   // device copy of the subtracted mean
+    bool shift_active = false;
+
+    // reductions
+    double *d_part = nullptr;    // per-block partials
+    size_t n_part = 0;
+    double *d_scal = nullptr;    // [0] maxdiv [1] max|u|+|v|+|w| [2] nonfinite flag [3] mean shift
+    double *h_scal = nullptr;    // pinned mirror
+    double last_dt = 0.0;        // dt of the last step (for maxCFL)
+    bool checks_valid = false;
+
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    bool prof_on = false;
+    fen::ProfSlot prof[fen::K_COUNT];
+};
+
+namespace fen {
+
+// Launchers (defined in the kernel translation units).  All enqueue on ctx->stream.
+void launch_ghost_desc(fen_ctx *c, const GhostDesc *descs, int n);
+void launch_momentum(fen_ctx *c, double dt, double A, double B);
+// vel == nullptr: the right-hand side is read from direct_src (strides sx, sy, ghost level gl)
+void launch_div_r2c_impl(fen_ctx *c, const double *const *vel, const double *direct_src, long sx, long sy, int gl,
+                         double rho, double dt, double post);
+void launch_y_fft(fen_ctx *c, int sign, double post_scale);
+void launch_y_solve_2d(fen_ctx *c, double inv_n);
+void launch_z_solve(fen_ctx *c, double inv_n);
+void launch_thomas(fen_ctx *c);
+void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want_sum);
+void launch_project(fen_ctx *c, double dt);
+void launch_checks(fen_ctx *c, const double *const *vel, double *out2);
+void launch_apply_shift(fen_ctx *c, double *f, size_t n);
+void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz);
+
+// Profiling helpers.
+void prof_begin(fen_ctx *c, int k);
+void prof_end(fen_ctx *c, int k, double bytes);
+
+}  // namespace fen
+
+#define FEN_HIP_CHECK(expr)                                                        \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + \
+                                                       hipGetErrorString(e_));     \
+    } while (0)

@@ -1,0 +1,140 @@
+// fft_device.hpp -- LDS-resident FFT building blocks for gfx950.
+//
+// A line of N complex points (N a power of two, 8 <= N <= 1024) is
+// transformed by T = N/8 threads with the Stockham autosort formulation:
+// each stage loads radix-R groups x[j + r*N/R] into registers, applies the
+// twiddles W_{Ns*R}^{r*(j mod Ns)}, does an R-point DFT in registers and
+// stores to (j div Ns)*Ns*R + (j mod Ns) + r*Ns.  Stages use radix 8 and one
+// final radix-4 or radix-2 stage when log2(N) is not a multiple of 3.
+// Output is in natural order, unnormalised; SIGN = -1 is FFTW_FORWARD
+// (exp(-2 pi i jk/N)) and SIGN = +1 is FFTW_BACKWARD.
+//
+// Twiddles come from a table W_N^m = exp(-2 pi i m/N), m < N, computed once
+// on the host in long double and rounded to the element type.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fen {
+
+template <typename T> struct Cplx;
+template <> struct Cplx<double> { using type = double2; };
+template <> struct Cplx<float> { using type = float2; };
+
+template <typename C> __device__ __forceinline__ C c_add(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C c_sub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C c_mul(C a, C b)
+{
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename C> __device__ __forceinline__ C c_conj(C a) { return C{a.x, -a.y}; }
+
+// a * (-i) for SIGN = -1, a * (+i) for SIGN = +1
+template <int SIGN, typename C> __device__ __forceinline__ C c_rot(C a)
+{
+    if constexpr (SIGN < 0) return C{a.y, -a.x};
+    else return C{-a.y, a.x};
+}
+
+// a * W_8^m (m odd) with W_8 = exp(SIGN * 2 pi i / 8)
+template <int SIGN, int M, typename C> __device__ __forceinline__ C c_w8(C a)
+{
+    using R = decltype(a.x);
+    const R h = (R)0.70710678118654752440084436210484903928;
+    constexpr int m = ((M % 8) + 8) % 8;
+    // forward: W^1 = h(1 - i), W^3 = h(-1 - i); inverse: conjugates
+    if constexpr ((m == 1 && SIGN < 0) || (m == 7 && SIGN > 0)) return C{h * (a.x + a.y), h * (a.y - a.x)};
+    else if constexpr ((m == 3 && SIGN < 0) || (m == 5 && SIGN > 0)) return C{h * (a.y - a.x), -(h * (a.x + a.y))};
+    else if constexpr ((m == 5 && SIGN < 0) || (m == 3 && SIGN > 0)) return C{-(h * (a.x + a.y)), h * (a.x - a.y)};
+    else return C{h * (a.x - a.y), h * (a.x + a.y)};   // (m==7 fwd) or (m==1 inv)
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3)
+{
+    C t0 = c_add(a0, a2), t1 = c_sub(a0, a2);
+    C t2 = c_add(a1, a3), t3 = c_rot<SIGN>(c_sub(a1, a3));
+    a0 = c_add(t0, t2); a2 = c_sub(t0, t2);
+    a1 = c_add(t1, t3); a3 = c_sub(t1, t3);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft8(C *v)
+{
+    C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<SIGN>(e0, e1, e2, e3);
+    dft4<SIGN>(o0, o1, o2, o3);
+    o1 = c_w8<SIGN, 1>(o1);
+    o2 = c_rot<SIGN>(o2);
+    o3 = c_w8<SIGN, 3>(o3);
+    v[0] = c_add(e0, o0); v[4] = c_sub(e0, o0);
+    v[1] = c_add(e1, o1); v[5] = c_sub(e1, o1);
+    v[2] = c_add(e2, o2); v[6] = c_sub(e2, o2);
+    v[3] = c_add(e3, o3); v[7] = c_sub(e3, o3);
+}
+
+template <int R, int SIGN, typename C> __device__ __forceinline__ void dft_any(C *v)
+{
+    if constexpr (R == 8) dft8<SIGN>(v);
+    else if constexpr (R == 4) dft4<SIGN>(v[0], v[1], v[2], v[3]);
+    else { C a = v[0], b = v[1]; v[0] = c_add(a, b); v[1] = c_sub(a, b); }
+}
+
+// LDS index with one pad element every 16 to break power-of-two strides.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+template <int N> constexpr int lds_line() { return N + N / 16; }
+
+// One Stockham stage and the recursion over the remaining ones.  `line`
+// points at this thread's line in LDS (padded indexing), t in [0, N/8).
+// All threads of the block must call this (it contains barriers).
+template <int N, int NS, int SIGN, typename C>
+__device__ __forceinline__ void fft_stages(C *line, int t, const C *tw)
+{
+    if constexpr (NS < N) {
+        constexpr int REM = N / NS;
+        constexpr int R = REM >= 8 ? 8 : REM;
+        constexpr int T = N / 8;
+        constexpr int NB = 8 / R;
+        C v[8];
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+            const int j = t + m * T;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[m * R + r] = line[lds_pad(j + r * (N / R))];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+            const int j = t + m * T;
+            const int k = j & (NS - 1);
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    C w = tw[r * k * (N / (NS * R))];
+                    if constexpr (SIGN > 0) w.y = -w.y;
+                    v[m * R + r] = c_mul(v[m * R + r], w);
+                }
+            }
+            dft_any<R, SIGN>(&v[m * R]);
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) line[lds_pad(base + r * NS)] = v[m * R + r];
+        }
+        __syncthreads();
+        fft_stages<N, NS * R, SIGN>(line, t, tw);
+    }
+}
+
+template <int N, int SIGN, typename C>
+__device__ __forceinline__ void fft_lds(C *line, int t, const C *tw)
+{
+    fft_stages<N, 1, SIGN>(line, t, tw);
+}
+
+// Copies the twiddle table W_N^m (m < N) into LDS.
+template <int N, typename C>
+__device__ __forceinline__ void load_twiddles(C *tw_lds, const C *tw_glob)
+{
+    for (int m = threadIdx.x; m < N; m += blockDim.x) tw_lds[m] = tw_glob[m];
+}
+
+}  // namespace fen

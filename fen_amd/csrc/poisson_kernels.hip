@@ -1,0 +1,492 @@
+// poisson_kernels.hip -- FP64 fast direct Poisson solver passes.
+//
+// Reference: poisson_solver_ppp (src/poisson.f90:941-1034),
+// poisson_solver_ppn (:1038-1173), poisson_solver_pp (:416-505) and the
+// right-hand side phi = div(u*)*rho/dt (src/navier_stokes.f90:111-121).
+// Half-spectrum layout: spec[(kx*nz + (k-1))*ny + (j-1)], kx in 0..nx/2.
+#include <stdexcept>
+
+#include "fen_internal.hpp"
+#include "fft_device.hpp"
+
+namespace fen {
+
+constexpr int kBlock = 256;
+
+template <int N> constexpr int threads_per_line() { return N / 8; }
+template <int N> constexpr int lines_per_block() { return kBlock / (N / 8); }
+
+// ---------------------------------------------------------------------------
+// K2: phi = (div(u*) * rho) / dt on TJ consecutive (j,k) rows, x r2c of row
+// pairs packed as one complex line, spectrum scattered as j-runs.
+// ---------------------------------------------------------------------------
+struct DivR2CArgs {
+    const double *u, *v, *w;
+    double2 *spec;
+    const double2 *tw;
+    Box b;
+    double id, rho, dt, post;
+    long nrows, kxs;
+    int direct;   // 1: `u` holds phi itself (interior values, strides sx/sy, ghost gl)
+    long sx, sy;
+    int gl;
+};
+
+template <int D3, int N>
+__global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs a)
+{
+    constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), TJ = 2 * L, LL = lds_line<N>();
+    __shared__ double2 buf[L * LL];
+    __shared__ double2 tw[N];
+    load_twiddles<N>(tw, a.tw);
+    const long row0 = (long)blockIdx.x * TJ;
+    const int ny = a.b.ny;
+    for (int e = threadIdx.x; e < TJ * N; e += kBlock) {
+        const int row = e / N, i = e % N + 1;
+        const long r = row0 + row;
+        double val = 0.0;
+        if (r < a.nrows) {
+            const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
+            if (a.direct) {
+                val = a.u[(i - 1 + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
+            } else {
+                const long g = a.b.g(i, j, k);
+                double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
+                if constexpr (D3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
+                val = d * a.rho / a.dt;
+            }
+        }
+        double2 *line = buf + (row >> 1) * LL;
+        if (row & 1) line[lds_pad(i - 1)].y = val;
+        else line[lds_pad(i - 1)].x = val;
+    }
+    __syncthreads();
+    fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
+    for (int e = threadIdx.x; e < TJ * (N / 2 + 1); e += kBlock) {
+        const int row = e % TJ, kx = e / TJ;
+        const long r = row0 + row;
+        if (r >= a.nrows) continue;
+        const double2 *line = buf + (row >> 1) * LL;
+        const double2 zk = line[lds_pad(kx)], zn = line[lds_pad((N - kx) & (N - 1))];
+        double2 X;
+        if ((row & 1) == 0) X = double2{(zk.x + zn.x) * 0.5, (zk.y - zn.y) * 0.5};
+        else X = double2{(zk.y + zn.y) * 0.5, (zn.x - zk.x) * 0.5};
+        X.x = X.x * a.post;
+        X.y = X.y * a.post;
+        a.spec[kx * a.kxs + r] = X;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K6: x c2r of row pairs -> phi rows (+ per-block sums for the mean).
+// ---------------------------------------------------------------------------
+struct C2RArgs {
+    const double2 *spec;
+    const double2 *tw;
+    double *out;
+    long nrows, kxs, sx, sy;
+    int ny, gl, want_sum;
+    double *part;
+};
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs a)
+{
+    constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), TJ = 2 * L, LL = lds_line<N>();
+    __shared__ double2 buf[L * LL];
+    __shared__ double2 tw[N];
+    __shared__ double red[kBlock];
+    load_twiddles<N>(tw, a.tw);
+    const long row0 = (long)blockIdx.x * TJ;
+    for (int e = threadIdx.x; e < L * (N / 2 + 1); e += kBlock) {
+        const int l = e % L, kx = e / L;
+        const long r = row0 + 2 * l;
+        double2 X1 = double2{0.0, 0.0}, X2 = double2{0.0, 0.0};
+        if (r < a.nrows) X1 = a.spec[kx * a.kxs + r];
+        if (r + 1 < a.nrows) X2 = a.spec[kx * a.kxs + r + 1];
+        double2 *line = buf + l * LL;
+        if (kx == 0 || kx == N / 2) {
+            line[lds_pad(kx)] = double2{X1.x, X2.x};   // Im of bins 0 and n/2 ignored
+        } else {
+            line[lds_pad(kx)] = double2{X1.x - X2.y, X1.y + X2.x};
+            line[lds_pad(N - kx)] = double2{X1.x + X2.y, X2.x - X1.y};
+        }
+    }
+    __syncthreads();
+    fft_lds<N, 1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
+    double acc = 0.0;
+    for (int e = threadIdx.x; e < TJ * N; e += kBlock) {
+        const int row = e / N, i = e % N;
+        const long r = row0 + row;
+        if (r >= a.nrows) continue;
+        const double2 z = buf[(row >> 1) * LL + lds_pad(i)];
+        const double val = (row & 1) ? z.y : z.x;
+        const long j = r % a.ny, k = r / a.ny;
+        a.out[(i + a.gl) + a.sx * (j + a.gl) + a.sy * (k + a.gl)] = val;
+        acc += val;
+    }
+    if (a.want_sum) {
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K3/K5: transforms along y; every y line spec[line*ny + 0..ny) is contiguous.
+// mode 0: plain transform times `post`; mode 1 (pp, 2-D): forward, times
+// inv_n, divide by mwx(kx)+mwy(j) (zero mode -> 0), inverse.
+// ---------------------------------------------------------------------------
+struct YArgs {
+    double2 *spec;
+    const double2 *tw;
+    long nlines;
+    double post, inv_n;
+    const double *mwx, *mwy;
+    int nz;
+};
+
+template <int N, int SIGN, int MODE>
+__global__ __launch_bounds__(kBlock) void y_kernel(YArgs a)
+{
+    constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), LL = lds_line<N>();
+    __shared__ double2 buf[L * LL];
+    __shared__ double2 tw[N];
+    load_twiddles<N>(tw, a.tw);
+    const long line0 = (long)blockIdx.x * L;
+    for (int e = threadIdx.x; e < L * N; e += kBlock) {
+        const int l = e / N, j = e % N;
+        const long ln = line0 + l;
+        buf[l * LL + lds_pad(j)] = ln < a.nlines ? a.spec[ln * N + j] : double2{0.0, 0.0};
+    }
+    __syncthreads();
+    double2 *line = buf + (threadIdx.x / T) * LL;
+    fft_lds<N, SIGN>(line, threadIdx.x % T, tw);
+    if constexpr (MODE == 1) {
+        for (int e = threadIdx.x; e < L * N; e += kBlock) {
+            const int l = e / N, j = e % N;
+            const long ln = line0 + l;
+            if (ln >= a.nlines) continue;
+            const int kx = (int)(ln / a.nz);
+            double2 X = buf[l * LL + lds_pad(j)];
+            X.x = X.x * a.inv_n;
+            X.y = X.y * a.inv_n;
+            const double s = a.mwx[kx] + a.mwy[j];
+            if (s == 0.0) X = double2{0.0, 0.0};
+            else { X.x = X.x / s; X.y = X.y / s; }
+            buf[l * LL + lds_pad(j)] = X;
+        }
+        __syncthreads();
+        fft_lds<N, 1>(line, threadIdx.x % T, tw);
+    }
+    for (int e = threadIdx.x; e < L * N; e += kBlock) {
+        const int l = e / N, j = e % N;
+        const long ln = line0 + l;
+        if (ln >= a.nlines) continue;
+        double2 X = buf[l * LL + lds_pad(j)];
+        X.x = X.x * a.post;
+        X.y = X.y * a.post;
+        a.spec[ln * N + j] = X;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K4 (ppp): z forward, /N, divide by (mwx+mwy)+mwz (zero mode -> 0), z inverse.
+// A block owns L adjacent j lines of one kx plane; z lines have stride ny.
+// ---------------------------------------------------------------------------
+struct ZArgs {
+    double2 *spec;
+    const double2 *tw;
+    int ny, nz;
+    long kxs;
+    double inv_n;
+    const double *mwx, *mwy, *mwz;
+};
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void z_solve_kernel(ZArgs a)
+{
+    constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), LL = lds_line<N>();
+    __shared__ double2 buf[L * LL];
+    __shared__ double2 tw[N];
+    load_twiddles<N>(tw, a.tw);
+    const int kx = blockIdx.y;
+    const int j0 = blockIdx.x * L;
+    double2 *base = a.spec + kx * a.kxs + j0;
+    for (int e = threadIdx.x; e < L * N; e += kBlock) {
+        const int l = e % L, k = e / L;
+        buf[l * LL + lds_pad(k)] = (j0 + l < a.ny) ? base[(long)k * a.ny + l] : double2{0.0, 0.0};
+    }
+    __syncthreads();
+    double2 *line = buf + (threadIdx.x / T) * LL;
+    fft_lds<N, -1>(line, threadIdx.x % T, tw);
+    const double mx = a.mwx[kx];
+    for (int e = threadIdx.x; e < L * N; e += kBlock) {
+        const int l = e % L, k = e / L;
+        if (j0 + l >= a.ny) continue;
+        double2 X = buf[l * LL + lds_pad(k)];
+        X.x = X.x * a.inv_n;
+        X.y = X.y * a.inv_n;
+        const double s = (mx + a.mwy[j0 + l]) + a.mwz[k];
+        if (s == 0.0) X = double2{0.0, 0.0};
+        else { X.x = X.x / s; X.y = X.y / s; }
+        buf[l * LL + lds_pad(k)] = X;
+    }
+    __syncthreads();
+    fft_lds<N, 1>(line, threadIdx.x % T, tw);
+    for (int e = threadIdx.x; e < L * N; e += kBlock) {
+        const int l = e % L, k = e / L;
+        if (j0 + l < a.ny) base[(long)k * a.ny + l] = buf[l * LL + lds_pad(k)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K4 (ppn): Thomas sweep along z for every (kx, j) column
+// (src/poisson.f90:1094-1135); one thread per column, adjacent threads own
+// adjacent j so every k-row access is coalesced.  d1 is kept in place in
+// the spectrum, c1 in a scratch array of the same layout.
+// ---------------------------------------------------------------------------
+struct ThomasArgs {
+    double2 *spec;
+    double *c1;
+    int ny, nz, hx;
+    long kxs;
+    const double *mwx, *mwy, *ta, *tb, *tc;
+};
+
+__global__ __launch_bounds__(kBlock) void thomas_kernel(ThomasArgs a)
+{
+    const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (long)a.hx * a.ny) return;
+    const int kx = (int)(t / a.ny), j = (int)(t % a.ny);
+    const long base = kx * a.kxs + j;
+    const long s = a.ny;
+    double2 *x = a.spec + base;
+    double *c1 = a.c1 + base;
+    const double mx = a.mwx[kx], my = a.mwy[j];
+    const int n = a.nz;
+    double factor = 1.0 / ((a.tb[0] + mx) + my);
+    double c1p = a.tc[0] * factor;
+    c1[0] = c1p;
+    double2 d = x[0];
+    d.x = d.x * factor;
+    d.y = d.y * factor;
+    x[0] = d;
+    for (int k = 1; k < n - 1; k++) {
+        factor = 1.0 / (((a.tb[k] + mx) + my) - a.ta[k] * c1p);
+        c1p = a.tc[k] * factor;
+        c1[k * s] = c1p;
+        const double2 r = x[k * s];
+        d.x = (r.x - a.ta[k] * d.x) * factor;
+        d.y = (r.y - a.ta[k] * d.y) * factor;
+        x[k * s] = d;
+    }
+    factor = ((a.tb[n - 1] + mx) + my) - a.ta[n - 1] * c1p;
+    const double2 r = x[(n - 1) * s];
+    if (factor != 0.0) {
+        d.x = (r.x - a.ta[n - 1] * d.x) / factor;
+        d.y = (r.y - a.ta[n - 1] * d.y) / factor;
+    } else {
+        d = double2{0.0, 0.0};
+    }
+    x[(n - 1) * s] = d;
+    double2 xn = d;
+    for (int k = n - 2; k >= 0; k--) {
+        const double2 dk = x[k * s];
+        const double ck = c1[k * s];
+        xn.x = dk.x - ck * xn.x;
+        xn.y = dk.y - ck * xn.y;
+        x[k * s] = xn;
+    }
+}
+
+// Sum of per-block partials in a fixed order, then shift = sum / N.
+__global__ __launch_bounds__(kBlock) void mean_shift_kernel(const double *part, long n, double count, double *out)
+{
+    __shared__ double red[kBlock];
+    double acc = 0.0;
+    for (long q = threadIdx.x; q < n; q += kBlock) acc += part[q];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] / count;
+}
+
+__global__ void apply_shift_kernel(double *f, long n, const double *shift)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) f[q] = f[q] - shift[0];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+#define FEN_DISPATCH_N(n, MACRO)                         \
+    switch (n) {                                         \
+    case 8: MACRO(8); break;                             \
+    case 16: MACRO(16); break;                           \
+    case 32: MACRO(32); break;                           \
+    case 64: MACRO(64); break;                           \
+    case 128: MACRO(128); break;                         \
+    case 256: MACRO(256); break;                         \
+    case 512: MACRO(512); break;                         \
+    case 1024: MACRO(1024); break;                       \
+    default: throw std::runtime_error("unsupported transform length"); \
+    }
+
+static double spec_bytes(const fen_ctx *c) { return (double)c->hx * c->ny * c->nz * 16.0; }
+
+// vel == nullptr: direct mode, the RHS is already in `direct_src` (phi array).
+void launch_div_r2c_impl(fen_ctx *c, const double *const *vel, const double *direct_src, long sx, long sy,
+                         int gl, double rho, double dt, double post)
+{
+    DivR2CArgs a{};
+    if (vel) { a.u = vel[0]; a.v = vel[1]; a.w = vel[2]; }
+    else { a.u = direct_src; a.direct = 1; a.sx = sx; a.sy = sy; a.gl = gl; }
+    a.spec = c->d_spec;
+    a.tw = c->d_twx;
+    a.b = c->box;
+    a.id = 1.0 / c->delta;
+    a.rho = rho;
+    a.dt = dt;
+    a.post = post;
+    a.nrows = (long)c->ny * c->nz;
+    a.kxs = (long)c->ny * c->nz;
+    const int tj = 2 * (kBlock / (c->nx / 8));
+    const dim3 grid((unsigned)((a.nrows + tj - 1) / tj));
+    const double bytes = (vel ? (c->dim == 3 ? 3.0 : 2.0) : 1.0) * 8.0 * c->n_int + spec_bytes(c);
+    prof_begin(c, K_DIV_R2C);
+#define L_DIV(NN)                                                                                  \
+    if (c->dim == 3) hipLaunchKernelGGL((div_r2c_kernel<1, NN>), grid, dim3(kBlock), 0, c->stream, a); \
+    else hipLaunchKernelGGL((div_r2c_kernel<0, NN>), grid, dim3(kBlock), 0, c->stream, a);
+    FEN_DISPATCH_N(c->nx, L_DIV)
+#undef L_DIV
+    prof_end(c, K_DIV_R2C, bytes);
+}
+
+void launch_y_fft(fen_ctx *c, int sign, double post)
+{
+    YArgs a{};
+    a.spec = c->d_spec;
+    a.tw = c->d_twy;
+    a.nlines = (long)c->hx * c->nz;
+    a.post = post;
+    a.nz = c->nz;
+    const int L = kBlock / (c->ny / 8);
+    const dim3 grid((unsigned)((a.nlines + L - 1) / L));
+    const int kid = sign < 0 ? K_Y_FWD : K_Y_INV;
+    prof_begin(c, kid);
+#define L_Y(NN)                                                                               \
+    if (sign < 0) hipLaunchKernelGGL((y_kernel<NN, -1, 0>), grid, dim3(kBlock), 0, c->stream, a); \
+    else hipLaunchKernelGGL((y_kernel<NN, 1, 0>), grid, dim3(kBlock), 0, c->stream, a);
+    FEN_DISPATCH_N(c->ny, L_Y)
+#undef L_Y
+    prof_end(c, kid, 2.0 * spec_bytes(c));
+}
+
+void launch_y_solve_2d(fen_ctx *c, double inv_n)
+{
+    YArgs a{};
+    a.spec = c->d_spec;
+    a.tw = c->d_twy;
+    a.nlines = (long)c->hx * c->nz;
+    a.post = 1.0;
+    a.inv_n = inv_n;
+    a.mwx = c->d_mwx;
+    a.mwy = c->d_mwy;
+    a.nz = c->nz;
+    const int L = kBlock / (c->ny / 8);
+    const dim3 grid((unsigned)((a.nlines + L - 1) / L));
+    prof_begin(c, K_Y_SOLVE);
+#define L_YS(NN) hipLaunchKernelGGL((y_kernel<NN, -1, 1>), grid, dim3(kBlock), 0, c->stream, a);
+    FEN_DISPATCH_N(c->ny, L_YS)
+#undef L_YS
+    prof_end(c, K_Y_SOLVE, 2.0 * spec_bytes(c));
+}
+
+void launch_z_solve(fen_ctx *c, double inv_n)
+{
+    ZArgs a{};
+    a.spec = c->d_spec;
+    a.tw = c->d_twz;
+    a.ny = c->ny;
+    a.nz = c->nz;
+    a.kxs = (long)c->ny * c->nz;
+    a.inv_n = inv_n;
+    a.mwx = c->d_mwx;
+    a.mwy = c->d_mwy;
+    a.mwz = c->d_mwz;
+    const int L = kBlock / (c->nz / 8);
+    const dim3 grid((unsigned)((c->ny + L - 1) / L), (unsigned)c->hx);
+    prof_begin(c, K_Z_SOLVE);
+#define L_Z(NN) hipLaunchKernelGGL((z_solve_kernel<NN>), grid, dim3(kBlock), 0, c->stream, a);
+    FEN_DISPATCH_N(c->nz, L_Z)
+#undef L_Z
+    prof_end(c, K_Z_SOLVE, 2.0 * spec_bytes(c));
+}
+
+void launch_thomas(fen_ctx *c)
+{
+    ThomasArgs a{};
+    a.spec = c->d_spec;
+    a.c1 = c->d_c1;
+    a.ny = c->ny;
+    a.nz = c->nz;
+    a.hx = c->hx;
+    a.kxs = (long)c->ny * c->nz;
+    a.mwx = c->d_mwx;
+    a.mwy = c->d_mwy;
+    a.ta = c->d_ta;
+    a.tb = c->d_tb;
+    a.tc = c->d_tc;
+    const long cols = (long)c->hx * c->ny;
+    prof_begin(c, K_THOMAS);
+    hipLaunchKernelGGL(thomas_kernel, dim3((unsigned)((cols + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, a);
+    prof_end(c, K_THOMAS, 2.0 * spec_bytes(c));
+}
+
+void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want_sum)
+{
+    C2RArgs a{};
+    a.spec = c->d_spec;
+    a.tw = c->d_twx;
+    a.out = phi;
+    a.nrows = (long)c->ny * c->nz;
+    a.kxs = a.nrows;
+    a.sx = sx;
+    a.sy = sy;
+    a.ny = c->ny;
+    a.gl = gl_out;
+    a.want_sum = want_sum ? 1 : 0;
+    a.part = c->d_part;
+    const int tj = 2 * (kBlock / (c->nx / 8));
+    const long nblk = (a.nrows + tj - 1) / tj;
+    if ((size_t)nblk > c->n_part) throw std::runtime_error("partial-sum buffer too small");
+    prof_begin(c, K_C2R);
+#define L_C(NN) hipLaunchKernelGGL((c2r_kernel<NN>), dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, a);
+    FEN_DISPATCH_N(c->nx, L_C)
+#undef L_C
+    prof_end(c, K_C2R, spec_bytes(c) + 8.0 * c->n_int);
+    if (want_sum) {
+        const double count = (double)(float)((long)c->nx * c->ny * c->nz);
+        prof_begin(c, K_REDUCE);
+        hipLaunchKernelGGL(mean_shift_kernel, dim3(1), dim3(kBlock), 0, c->stream, (const double *)c->d_part, nblk,
+                           count, c->d_shift);
+        prof_end(c, K_REDUCE, 8.0 * nblk);
+    }
+}
+
+void launch_apply_shift(fen_ctx *c, double *f, size_t n)
+{
+    hipLaunchKernelGGL(apply_shift_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, f, (long)n,
+                       (const double *)c->d_shift);
+}
+
+}  // namespace fen

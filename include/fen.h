@@ -1,0 +1,174 @@
+/*
+ * fen.h -- C-ABI of libfen, the MI355X (gfx950) implementation of FEN's
+ * single-phase fractional-step Navier-Stokes time advance.
+ *
+ * One context owns one grid, the navier_stokes_mod fields and the Poisson
+ * solver tables, all resident in HBM.  Every entry point returns an int
+ * status (FEN_OK == 0); on failure fen_last_error() describes it.  No C++
+ * exception crosses this boundary.  Arrays passed in or out are host
+ * arrays in FEN's own layout: Fortran order (i fastest), real(8), and for
+ * ghosted fields the full (nx+2)*(ny+2)*(nz+2) extent of
+ * f(lo-1:hi+1, ...) (reference src/scalar.f90:79-81).  In 2-D nz == 1 and
+ * the ghosted k extent is 0:2.
+ *
+ * Threading: one host thread per context; all device work is queued on the
+ * context's own HIP stream.  Entry points that return scalars computed on
+ * the device synchronise that stream; fen_ns_step does so only when asked
+ * for maxdiv/maxcfl or when the time step is CFL-controlled.
+ */
+#ifndef FEN_H
+#define FEN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEN_OK 0
+#define FEN_ERR_ARG (-1)          /* bad argument (size, id, pointer) */
+#define FEN_ERR_UNSUPPORTED (-2)  /* boundary combination / size not supported */
+#define FEN_ERR_HIP (-3)          /* HIP runtime failure */
+#define FEN_ERR_NONFINITE (-4)    /* NaN/Inf met by the step's checks */
+#define FEN_ERR_STATE (-5)        /* call made in the wrong state (e.g. dt_o unset) */
+
+typedef struct fen_ctx fen_ctx;
+
+/* Field ids: module variables of navier_stokes_mod
+ * (reference src/navier_stokes.f90:36-37). */
+enum {
+    FEN_P = 0,      /* pressure p, cell centre, ghosted */
+    FEN_PHI = 1,    /* pressure increment phi, cell centre, ghosted */
+    FEN_U = 2,      /* v%x, x faces, ghosted */
+    FEN_V = 3,      /* v%y, y faces, ghosted */
+    FEN_W = 4,      /* v%z, z faces, ghosted (3-D only) */
+    FEN_DVO_X = 5,  /* dv_o%x, interior only (gl = 0) */
+    FEN_DVO_Y = 6,
+    FEN_DVO_Z = 7,
+    FEN_S_X = 8,    /* volumetric source S%x, interior only */
+    FEN_S_Y = 9,
+    FEN_S_Z = 10
+};
+
+/* Physical boundary codes: the strings of grid%boundary_conditions
+ * (reference src/grid.f90:41-46, mapped in src/navier_stokes.f90:779-1017). */
+enum { FEN_BC_PERIODIC = 0, FEN_BC_WALL = 1, FEN_BC_INFLOW = 2, FEN_BC_OUTFLOW = 3 };
+
+/* Faces, in FEN's fill order (reference src/scalar.f90:254-388). */
+enum { FEN_LEFT = 0, FEN_RIGHT = 1, FEN_BOTTOM = 2, FEN_TOP = 3, FEN_FRONT = 4, FEN_BACK = 5 };
+
+/* Arithmetic of the step. */
+enum {
+    FEN_FP64 = 0,        /* everything in FP64, the reference's real(dp) */
+    FEN_FP32_REFINE = 1  /* FP32 stencils/FFTs, FP64 elimination, residual and one
+                            FP64 defect-correction solve per step */
+};
+
+/* Poisson solver kinds reported by fen_solver_kind (reference src/poisson.f90:70-111). */
+enum { FEN_SOLVER_PP = 0, FEN_SOLVER_PN = 1, FEN_SOLVER_NN = 2, FEN_SOLVER_PPP = 3,
+       FEN_SOLVER_PPN = 4, FEN_SOLVER_NPN = 5, FEN_SOLVER_NNN = 6 };
+
+/* Replaces grid%setup (src/grid.f90:67-200) + init_solver (src/solver.f90:34-99):
+ * allocate_navier_stokes_fields (src/navier_stokes.f90:752-1067) and
+ * init_poisson_solver (src/poisson.f90:57-113).  dim is 2 or 3 (CPP DIM);
+ * bc[6] holds FEN_BC_* codes for left,right,bottom,top,front,back.  Grid
+ * sizes must be powers of two (>= 8) in the FFT directions.  Returns
+ * FEN_ERR_UNSUPPORTED where the reference stops (src/poisson.f90:91-94). */
+int fen_create(fen_ctx **ctx, int dim, int nx, int ny, int nz, double lx, double ly, double lz,
+               const double *x0, const int *bc, int precision);
+
+/* Replaces destroy_solver (src/solver.f90:333-358). */
+int fen_destroy(fen_ctx *ctx);
+
+/* Message of the last failure on this context ("" if none). */
+const char *fen_last_error(const fen_ctx *ctx);
+
+/* Sets the module variables density, viscosity, g, CFL, constant_CFL
+ * (src/navier_stokes.f90:18-42). */
+int fen_set_params(fen_ctx *ctx, double density, double viscosity, const double *g, double cfl,
+                   int constant_cfl);
+
+/* Replaces set_timestep(comp_grid, dt, U) (src/navier_stokes.f90:623-666):
+ * returns dt and sets dt_o = dt. */
+int fen_set_timestep(fen_ctx *ctx, double U, double *dt);
+
+/* Assigns the module variable dt_o (src/navier_stokes.f90:33). */
+int fen_set_dt_o(fen_ctx *ctx, double dt_o);
+
+/* Reads back maxdiv, maxCFL (src/navier_stokes.f90:24), dt_o, dt_visc and
+ * dt_conv.  Synchronises.  Returns FEN_ERR_NONFINITE if the last step's
+ * checks met a NaN or Inf.  Any pointer may be NULL. */
+int fen_get_scalars(fen_ctx *ctx, double *maxdiv, double *maxcfl, double *dt_o, double *dt_visc,
+                    double *dt_conv);
+
+/* Number of doubles of a field in host layout (ghosted or interior). */
+int fen_field_count(const fen_ctx *ctx, int field, size_t *count);
+
+/* Host <-> HBM copies of a whole field (synchronous).  Uploading a source
+ * component switches the source term S/rhof on (src/navier_stokes.f90:245-255). */
+int fen_upload(fen_ctx *ctx, int field, const double *host, size_t count);
+int fen_download(fen_ctx *ctx, int field, double *host, size_t count);
+
+/* Boundary value plane of a ghosted field: f%bc%left ... f%bc%back
+ * (src/scalar.f90:28-37).  Left/right planes are (0:ny+1,0:nz+1), bottom/top
+ * (0:nx+1,0:nz+1), front/back (0:nx+1,0:ny+1), first index fastest. */
+int fen_set_bc_plane(fen_ctx *ctx, int field, int face, const double *plane, size_t count);
+
+/* Overrides a field's bc type (f%bc%type_*: 0 periodic, 1 Dirichlet,
+ * 2 Neumann; src/scalar.f90:17-26), as user programs do
+ * (test/small_test/poisson/projection/projection.f90:68-77). */
+int fen_set_bc_type(fen_ctx *ctx, int field, int face, int type);
+int fen_get_bc_type(const fen_ctx *ctx, int field, int face, int *type);
+
+/* Replaces scalar%update_ghost_nodes (src/scalar.f90:223-396) and, on one
+ * GPU, update_halos (src/halo.f90:12-43) for a resident field. */
+int fen_update_ghost_nodes(fen_ctx *ctx, int field);
+
+/* Replaces navier_stokes_solver(comp_grid, step, dt) (src/navier_stokes.f90:50-136).
+ * dt is in/out (it changes only when constant_CFL is set).  maxdiv and
+ * maxcfl may be NULL; then the step is queued without a host sync and the
+ * values are fetched later with fen_get_scalars. */
+int fen_ns_step(fen_ctx *ctx, int step, double *dt, double *maxdiv, double *maxcfl);
+
+/* Runs nsteps consecutive steps (step0+1 .. step0+nsteps) without host
+ * synchronisation; for fixed-dt runs and benchmarks. */
+int fen_ns_run(fen_ctx *ctx, int step0, int nsteps, double *dt);
+
+/* Replaces the solve_poisson procedure pointer (src/poisson.f90:21-29):
+ * solves lap(phi) = f in place on a host array with ghost level gl (0 or 1),
+ * with the same whole-array semantics as the selected reference solver. */
+int fen_poisson_solve(fen_ctx *ctx, double *f, int gl);
+
+/* Replaces add_advection(comp_grid, RHS) (src/navier_stokes.f90:261-353) on
+ * the resident velocity: rx/ry/rz are interior-sized host arrays updated as
+ * RHS = RHS - (advective terms). rz may be NULL in 2-D. */
+int fen_add_advection(fen_ctx *ctx, double *rx, double *ry, double *rz);
+
+/* Replaces print_navier_stokes_solver_status (src/navier_stokes.f90:734-748):
+ * formats the status line (no newline) into buf. */
+int fen_format_status(fen_ctx *ctx, int step, double time, double dt, char *buf, size_t len);
+
+/* Writes grid.json exactly as grid%print_json (src/grid.f90:233-265). */
+int fen_write_grid_json(const fen_ctx *ctx, const char *path);
+
+/* Reports the Poisson solver kind (FEN_SOLVER_*) and the grid spacing. */
+int fen_solver_kind(const fen_ctx *ctx, int *kind);
+int fen_grid_delta(const fen_ctx *ctx, double *delta);
+
+/* Waits for all work queued on the context's stream. */
+int fen_synchronize(fen_ctx *ctx);
+
+/* Per-kernel timing with HIP events recorded on the context's stream.
+ * While enabled every launch of kernel k is bracketed by an event pair;
+ * fen_profile_read returns the summed milliseconds, the launch count and the
+ * kernel's algorithmic HBM bytes per launch. */
+int fen_profile_enable(fen_ctx *ctx, int enable);
+int fen_profile_reset(fen_ctx *ctx);
+int fen_profile_count(void);
+const char *fen_profile_name(int kernel);
+int fen_profile_read(fen_ctx *ctx, int kernel, double *total_ms, long long *launches, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,926 @@
+/*
+ * fen_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, FP64 restatement of FEN's single-phase fractional-step
+ * Navier-Stokes step (reference: fradevita/FEN).  It is the checker for the
+ * HIP library and the CPU baseline timed beside it; the product never links
+ * it.  Every routine cites the Fortran it restates (file:line in the
+ * reference tree).  Expression trees follow Fortran's left-to-right
+ * evaluation and explicit parentheses; build with -ffp-contract=off so that
+ * no multiply-add is fused (the reference is built with gfortran without
+ * -march, which cannot fuse either).
+ *
+ * The FFT is this file's own iterative radix-2 transform (FFTW is not part
+ * of this build): FFTW's conventions are kept (unnormalised, forward sign
+ * -1, r2c keeps bins 0..n/2, c2r ignores Im of bins 0 and n/2), its
+ * rounding is not.  Sizes must be powers of two.
+ */
+#include "fen_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* -------------------------------------------------------------------------- */
+/* Fields                                                                     */
+/* -------------------------------------------------------------------------- */
+
+typedef struct {
+    double *f;       /* storage */
+    int gl;          /* ghost level 0 or 1 */
+    char loc;        /* 'c','x','y','z' (src/scalar.f90:46, src/vector.f90:58-62) */
+    int type[6];     /* bc type per face: -1 halo, 0 periodic, 1 Dirichlet, 2 Neumann */
+    double *plane[6];/* bc value planes (src/scalar.f90:89-106) */
+} ofield;
+
+typedef struct {
+    int n, logn;
+    double *wr, *wi; /* w_k = exp(-2 pi i k / n), k < n/2 */
+    int *rev;
+} offt;
+
+struct orc_ctx {
+    int dim, nx, ny, nz;
+    double lx, ly, lz, x0[3], delta;
+    int bc[6];
+    int periodic[3];
+    int solver;
+    /* navier_stokes_mod state (src/navier_stokes.f90:17-45) */
+    double density, viscosity, g[3], cfl;
+    int constant_cfl;
+    double dt_o, dt_visc, dt_conv, maxdiv, maxcfl;
+    ofield p, phi, u, v, w;
+    double *dv[3], *dvo[3], *S[3], *gp[3];
+    /* poisson_mod state (src/poisson.f90:31-49) */
+    double *mwx, *mwy, *mwz;
+    double *ta, *tb, *tc;
+    offt fx, fy, fz;
+    double *sre, *sim;  /* half spectrum (nx/2+1)*ny*nz, kx fastest */
+};
+
+#define GX(c) ((size_t)(c)->nx + 2)
+#define GY(c) ((size_t)(c)->ny + 2)
+#define GZ(c) ((size_t)(c)->nz + 2)
+#define GI(c, i, j, k) ((size_t)(i) + GX(c) * ((size_t)(j) + GY(c) * (size_t)(k)))
+#define II(c, i, j, k) ((size_t)((i) - 1) + (size_t)(c)->nx * ((size_t)((j) - 1) + (size_t)(c)->ny * (size_t)((k) - 1)))
+
+static size_t ghosted_len(const orc_ctx *c) { return GX(c) * GY(c) * GZ(c); }
+static size_t interior_len(const orc_ctx *c) { return (size_t)c->nx * c->ny * c->nz; }
+
+static size_t plane_len(const orc_ctx *c, int face)
+{
+    if (face < 2) return GY(c) * GZ(c);
+    if (face < 4) return GX(c) * GZ(c);
+    return GX(c) * GY(c);
+}
+
+static void field_alloc(orc_ctx *c, ofield *f, char loc)
+{
+    f->gl = 1;
+    f->loc = loc;
+    f->f = (double *)calloc(ghosted_len(c), sizeof(double));   /* src/scalar.f90:84 zero */
+    for (int s = 0; s < 6; s++) {
+        f->type[s] = 0;                                         /* src/scalar.f90:109-116 */
+        f->plane[s] = (double *)calloc(plane_len(c, s), sizeof(double));
+    }
+}
+
+static void field_free(ofield *f)
+{
+    free(f->f);
+    for (int s = 0; s < 6; s++) free(f->plane[s]);
+}
+
+/* -------------------------------------------------------------------------- */
+/* Ghost fill: scalar%update_ghost_nodes, src/scalar.f90:223-396.             */
+/* One rank: the 2decomp halo exchange (src/halo.f90:33) is fully overwritten */
+/* by the plane fills below, which run in the order left, right, bottom,      */
+/* top, front, back over full planes.                                         */
+/* -------------------------------------------------------------------------- */
+static void ghost_fill_array(const orc_ctx *c, double *f, char loc, const int *type, double *const *plane)
+{
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const size_t gx = GX(c), gy = GY(c), gz = GZ(c);
+    /* Left (src/scalar.f90:254-271) */
+    for (size_t k = 0; k < gz; k++)
+        for (size_t j = 0; j < gy; j++) {
+            double *row = f + gx * (j + gy * k);
+            const double bcv = plane[0][j + gy * k];
+            if (type[0] == 0) row[0] = row[nx];
+            else if (type[0] == 1) row[0] = (loc == 'x') ? bcv : 2.0 * bcv - row[1];
+            else if (type[0] == 2) row[0] = row[1];
+        }
+    /* Right (src/scalar.f90:273-291) */
+    for (size_t k = 0; k < gz; k++)
+        for (size_t j = 0; j < gy; j++) {
+            double *row = f + gx * (j + gy * k);
+            const double bcv = plane[1][j + gy * k];
+            if (type[1] == 0) row[nx + 1] = row[1];
+            else if (type[1] == 1) {
+                if (loc == 'x') { row[nx] = bcv; row[nx + 1] = bcv; }
+                else row[nx + 1] = 2.0 * bcv - row[nx];
+            } else if (type[1] == 2) row[nx + 1] = row[nx];
+        }
+    /* Bottom (src/scalar.f90:293-316); prow == 1 on one rank */
+    for (size_t k = 0; k < gz; k++)
+        for (size_t i = 0; i < gx; i++) {
+            const double bcv = plane[2][i + gx * k];
+            double *g0 = f + GI(c, i, 0, k), *g1 = f + GI(c, i, 1, k), *gn = f + GI(c, i, ny, k);
+            if (type[2] == 0) *g0 = *gn;
+            else if (type[2] == 1) *g0 = (loc == 'y') ? bcv : 2.0 * bcv - *g1;
+            else if (type[2] == 2) *g0 = *g1;
+        }
+    /* Top (src/scalar.f90:318-342) */
+    for (size_t k = 0; k < gz; k++)
+        for (size_t i = 0; i < gx; i++) {
+            const double bcv = plane[3][i + gx * k];
+            double *gt = f + GI(c, i, ny + 1, k), *g1 = f + GI(c, i, 1, k), *gn = f + GI(c, i, ny, k);
+            if (type[3] == 0) *gt = *g1;
+            else if (type[3] == 1) {
+                if (loc == 'y') { *gn = bcv; *gt = bcv; }
+                else *gt = 2.0 * bcv - *gn;
+            } else if (type[3] == 2) *gt = *gn;
+        }
+    if (c->dim != 3) return;   /* src/scalar.f90:344 #if DIM==3 */
+    /* Front (src/scalar.f90:345-365) */
+    for (size_t j = 0; j < gy; j++)
+        for (size_t i = 0; i < gx; i++) {
+            const double bcv = plane[4][i + gx * j];
+            double *g0 = f + GI(c, i, j, 0), *g1 = f + GI(c, i, j, 1), *gn = f + GI(c, i, j, nz);
+            if (type[4] == 0) *g0 = *gn;
+            else if (type[4] == 1) *g0 = (loc == 'z') ? bcv : 2.0 * bcv - *g1;
+            else if (type[4] == 2) *g0 = *g1;
+        }
+    /* Back (src/scalar.f90:367-388) */
+    for (size_t j = 0; j < gy; j++)
+        for (size_t i = 0; i < gx; i++) {
+            const double bcv = plane[5][i + gx * j];
+            double *gt = f + GI(c, i, j, nz + 1), *g1 = f + GI(c, i, j, 1), *gn = f + GI(c, i, j, nz);
+            if (type[5] == 0) *gt = *g1;
+            else if (type[5] == 1) {
+                if (loc == 'z') { *gn = bcv; *gt = bcv; }
+                else *gt = 2.0 * bcv - *gn;
+            } else if (type[5] == 2) *gt = *gn;
+        }
+}
+
+static void ghost_fill(const orc_ctx *c, ofield *f)
+{
+    ghost_fill_array(c, f->f, f->loc, f->type, f->plane);
+}
+
+/* -------------------------------------------------------------------------- */
+/* FFT (radix-2, decimation in time)                                          */
+/* -------------------------------------------------------------------------- */
+static int ilog2(int n) { int l = 0; while ((1 << l) < n) l++; return l; }
+
+static void fft_init(offt *p, int n)
+{
+    p->n = n;
+    p->logn = ilog2(n);
+    p->wr = (double *)malloc(sizeof(double) * (size_t)(n / 2 + 1));
+    p->wi = (double *)malloc(sizeof(double) * (size_t)(n / 2 + 1));
+    p->rev = (int *)malloc(sizeof(int) * (size_t)n);
+    const long double tpi = 2.0L * acosl(-1.0L);
+    for (int k = 0; k < n / 2 + 1; k++) {
+        long double a = tpi * (long double)k / (long double)n;
+        p->wr[k] = (double)cosl(a);
+        p->wi[k] = (double)(-sinl(a));
+    }
+    for (int i = 0; i < n; i++) {
+        int r = 0;
+        for (int b = 0; b < p->logn; b++) if (i & (1 << b)) r |= 1 << (p->logn - 1 - b);
+        p->rev[i] = r;
+    }
+}
+
+static void fft_free(offt *p) { free(p->wr); free(p->wi); free(p->rev); }
+
+/* In-place complex transform of one line; sign -1 = FFTW_FORWARD, +1 = FFTW_BACKWARD. */
+static void fft_line(const offt *p, double *re, double *im, int sign)
+{
+    const int n = p->n;
+    for (int i = 0; i < n; i++) {
+        int r = p->rev[i];
+        if (r > i) {
+            double t = re[i]; re[i] = re[r]; re[r] = t;
+            t = im[i]; im[i] = im[r]; im[r] = t;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len >> 1, step = n / len;
+        for (int b = 0; b < n; b += len)
+            for (int j = 0; j < half; j++) {
+                const double wr = p->wr[j * step];
+                const double wi = (sign < 0) ? p->wi[j * step] : -p->wi[j * step];
+                const int a = b + j, d = b + j + half;
+                const double tr = wr * re[d] - wi * im[d];
+                const double ti = wr * im[d] + wi * re[d];
+                re[d] = re[a] - tr; im[d] = im[a] - ti;
+                re[a] = re[a] + tr; im[a] = im[a] + ti;
+            }
+    }
+}
+
+/* r2c of n reals -> bins 0..n/2 (dfftw_plan_dft_r2c_1d). */
+static void fft_r2c_line(const offt *p, const double *x, size_t xs, double *re, double *im, size_t os,
+                         double *tr, double *ti)
+{
+    const int n = p->n;
+    for (int i = 0; i < n; i++) { tr[i] = x[(size_t)i * xs]; ti[i] = 0.0; }
+    fft_line(p, tr, ti, -1);
+    for (int k = 0; k <= n / 2; k++) { re[(size_t)k * os] = tr[k]; im[(size_t)k * os] = ti[k]; }
+}
+
+/* c2r of bins 0..n/2 -> n reals (dfftw_plan_dft_c2r_1d): Hermitian extension,
+ * Im of bins 0 and n/2 ignored. */
+static void fft_c2r_line(const offt *p, const double *re, const double *im, size_t is, double *x, size_t xs,
+                         double *tr, double *ti)
+{
+    const int n = p->n;
+    for (int k = 0; k <= n / 2; k++) { tr[k] = re[(size_t)k * is]; ti[k] = im[(size_t)k * is]; }
+    ti[0] = 0.0;
+    if (n > 1) ti[n / 2] = 0.0;
+    for (int k = 1; k < n / 2; k++) { tr[n - k] = tr[k]; ti[n - k] = -ti[k]; }
+    fft_line(p, tr, ti, +1);
+    for (int i = 0; i < n; i++) x[(size_t)i * xs] = tr[i];
+}
+
+void orc_fft_c2c(int n, double *re, double *im, int sign)
+{
+    offt p; fft_init(&p, n); fft_line(&p, re, im, sign); fft_free(&p);
+}
+
+void orc_fft_r2c(int n, const double *x, double *re, double *im)
+{
+    offt p; fft_init(&p, n);
+    double *t = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    fft_r2c_line(&p, x, 1, re, im, 1, t, t + n);
+    free(t); fft_free(&p);
+}
+
+void orc_fft_c2r(int n, const double *re, const double *im, double *x)
+{
+    offt p; fft_init(&p, n);
+    double *t = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    fft_c2r_line(&p, re, im, 1, x, 1, t, t + n);
+    free(t); fft_free(&p);
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* -------------------------------------------------------------------------- */
+/* Poisson fast direct solvers (src/poisson.f90)                              */
+/* -------------------------------------------------------------------------- */
+
+/* Modified wavenumber, src/poisson.f90:141,628,646,664 (factor 2 pi, periodic)
+ * and :265,794,881,899 (factor pi, Neumann). */
+static double *modified_wavenumbers(int n, double delta, double factor)
+{
+    double *m = (double *)malloc(sizeof(double) * (size_t)n);
+    const double pi = acos(-1.0);                     /* src/global.f90:14 */
+    for (int i = 1; i <= n; i++)
+        m[i - 1] = 2.0 * (cos(factor * pi * ((double)i - 1.0) / (double)(float)n) - 1.0) / (delta * delta);
+    return m;
+}
+
+/* Tridiagonal coefficients, src/poisson.f90:741-757. */
+static void tridiag_coefficients(orc_ctx *c, int n, int inflow_outflow)
+{
+    const double d2 = c->delta * c->delta;
+    c->ta = (double *)malloc(sizeof(double) * (size_t)n);
+    c->tb = (double *)malloc(sizeof(double) * (size_t)n);
+    c->tc = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int k = 0; k < n; k++) { c->ta[k] = 1.0 / d2; c->tb[k] = -(2.0 / d2); c->tc[k] = 1.0 / d2; }
+    c->tb[0] = c->tb[0] + c->ta[0];
+    if (inflow_outflow) c->tb[n - 1] = c->tb[n - 1] - c->tc[n - 1];
+    else c->tb[n - 1] = c->tb[n - 1] + c->tc[n - 1];
+    c->ta[0] = 0.0;
+    c->tc[n - 1] = 0.0;
+}
+
+/* Thomas sweep, src/poisson.f90:1094-1135 (3-D form, multiply by the
+ * reciprocal pivot except on the last row, which divides). */
+static void thomas_column(int n, const double *a, const double *b, const double *cc, double mx, double my,
+                          double *re, double *im, size_t s, double *c1, double *dr, double *di)
+{
+    double factor = 1.0 / ((b[0] + mx) + my);
+    c1[0] = cc[0] * factor;
+    dr[0] = re[0] * factor; di[0] = im[0] * factor;
+    for (int k = 1; k < n - 1; k++) {
+        factor = 1.0 / (((b[k] + mx) + my) - a[k] * c1[k - 1]);
+        c1[k] = cc[k] * factor;
+        dr[k] = (re[(size_t)k * s] - a[k] * dr[k - 1]) * factor;
+        di[k] = (im[(size_t)k * s] - a[k] * di[k - 1]) * factor;
+    }
+    factor = ((b[n - 1] + mx) + my) - a[n - 1] * c1[n - 2];
+    if (factor != 0.0) {
+        dr[n - 1] = (re[(size_t)(n - 1) * s] - a[n - 1] * dr[n - 2]) / factor;
+        di[n - 1] = (im[(size_t)(n - 1) * s] - a[n - 1] * di[n - 2]) / factor;
+    } else {
+        dr[n - 1] = 0.0; di[n - 1] = 0.0;
+    }
+    re[(size_t)(n - 1) * s] = dr[n - 1]; im[(size_t)(n - 1) * s] = di[n - 1];
+    for (int k = n - 2; k >= 0; k--) {
+        re[(size_t)k * s] = dr[k] - c1[k] * re[(size_t)(k + 1) * s];
+        im[(size_t)k * s] = di[k] - c1[k] * im[(size_t)(k + 1) * s];
+    }
+}
+
+void orc_thomas(int n, const double *a, const double *b, const double *cc, double lam_x, double lam_y,
+                double *re, double *im)
+{
+    double *w = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    thomas_column(n, a, b, cc, lam_x, lam_y, re, im, 1, w, w + n, w + 2 * n);
+    free(w);
+}
+
+/* Common first stage: x r2c of every interior line of f into the spectrum. */
+static void spectrum_x_forward(orc_ctx *c, const double *f, int gl)
+{
+    const int nx = c->nx, ny = c->ny, nz = c->nz, hx = nx / 2 + 1;
+    const size_t sx = gl ? GX(c) : (size_t)nx, sy = gl ? GY(c) : (size_t)ny;
+#pragma omp parallel
+    {
+        double *t = (double *)malloc(sizeof(double) * 2 * (size_t)nx);
+#pragma omp for collapse(2) schedule(static)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++) {
+                const double *line = f + (size_t)gl + sx * ((size_t)(j - 1 + gl) + sy * (size_t)(k - 1 + gl));
+                const size_t o = (size_t)hx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+                fft_r2c_line(&c->fx, line, 1, c->sre + o, c->sim + o, 1, t, t + nx);
+            }
+        free(t);
+    }
+}
+
+static void spectrum_x_backward(orc_ctx *c, double *f, int gl)
+{
+    const int nx = c->nx, ny = c->ny, nz = c->nz, hx = nx / 2 + 1;
+    const size_t sx = gl ? GX(c) : (size_t)nx, sy = gl ? GY(c) : (size_t)ny;
+#pragma omp parallel
+    {
+        double *t = (double *)malloc(sizeof(double) * 2 * (size_t)nx);
+#pragma omp for collapse(2) schedule(static)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++) {
+                double *line = f + (size_t)gl + sx * ((size_t)(j - 1 + gl) + sy * (size_t)(k - 1 + gl));
+                const size_t o = (size_t)hx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+                fft_c2r_line(&c->fx, c->sre + o, c->sim + o, 1, line, 1, t, t + nx);
+            }
+        free(t);
+    }
+}
+
+/* c2c along y (dir 1) or z (dir 2) of every spectrum line, sign -1/+1. */
+static void spectrum_c2c(orc_ctx *c, int dir, int sign)
+{
+    const int ny = c->ny, nz = c->nz, hx = c->nx / 2 + 1;
+    const offt *p = dir == 1 ? &c->fy : &c->fz;
+    const int n = p->n, nother = dir == 1 ? nz : ny;
+    const size_t stride = dir == 1 ? (size_t)hx : (size_t)hx * (size_t)ny;
+#pragma omp parallel
+    {
+        double *t = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+#pragma omp for collapse(2) schedule(static)
+        for (int o = 0; o < nother; o++)
+            for (int kx = 0; kx < hx; kx++) {
+                const size_t base = dir == 1 ? (size_t)kx + (size_t)hx * (size_t)ny * (size_t)o
+                                             : (size_t)kx + (size_t)hx * (size_t)o;
+                for (int l = 0; l < n; l++) { t[l] = c->sre[base + stride * l]; t[n + l] = c->sim[base + stride * l]; }
+                fft_line(p, t, t + n, sign);
+                for (int l = 0; l < n; l++) { c->sre[base + stride * l] = t[l]; c->sim[base + stride * l] = t[n + l]; }
+            }
+        free(t);
+    }
+}
+
+static void spectrum_scale(orc_ctx *c, double s)
+{
+    const size_t n = (size_t)(c->nx / 2 + 1) * c->ny * c->nz;
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < n; q++) { c->sre[q] = c->sre[q] / s; c->sim[q] = c->sim[q] / s; }
+}
+
+/* Sum of the interior in Fortran loop order (k, j, i), sequential. */
+static double interior_sum(const orc_ctx *c, const double *f, int gl)
+{
+    const size_t sx = gl ? GX(c) : (size_t)c->nx, sy = gl ? GY(c) : (size_t)c->ny;
+    double s = 0.0;
+    for (int k = 1; k <= c->nz; k++)
+        for (int j = 1; j <= c->ny; j++) {
+            const double *line = f + (size_t)gl + sx * ((size_t)(j - 1 + gl) + sy * (size_t)(k - 1 + gl));
+            for (int i = 0; i < c->nx; i++) s = s + line[i];
+        }
+    return s;
+}
+
+static size_t array_len(const orc_ctx *c, int gl) { return gl ? ghosted_len(c) : interior_len(c); }
+
+/* poisson_solver_ppp, src/poisson.f90:941-1034. */
+static void poisson_ppp(orc_ctx *c, double *f, int gl)
+{
+    const int hx = c->nx / 2 + 1, ny = c->ny, nz = c->nz;
+    spectrum_x_forward(c, f, gl);
+    spectrum_c2c(c, 1, -1);
+    spectrum_c2c(c, 2, -1);
+    spectrum_scale(c, (double)(float)(c->nx * c->ny * c->nz));                 /* :992 */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < hx; i++) {                                      /* :995-1005 */
+                const size_t q = (size_t)i + (size_t)hx * ((size_t)j + (size_t)ny * (size_t)k);
+                const double s = (c->mwx[i] + c->mwy[j]) + c->mwz[k];
+                if (s == 0.0) { c->sre[q] = 0.0; c->sim[q] = 0.0; }
+                else { c->sre[q] = c->sre[q] / s; c->sim[q] = c->sim[q] / s; }
+            }
+    spectrum_c2c(c, 2, +1);
+    spectrum_c2c(c, 1, +1);
+    spectrum_x_backward(c, f, gl);
+}
+
+/* poisson_solver_pp, src/poisson.f90:416-505. */
+static void poisson_pp(orc_ctx *c, double *f, int gl)
+{
+    const int hx = c->nx / 2 + 1, ny = c->ny;
+    spectrum_x_forward(c, f, gl);
+    spectrum_c2c(c, 1, -1);
+    spectrum_scale(c, (double)(float)(c->nx * c->ny));                         /* :458 */
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < hx; i++) {                                          /* :461-471 */
+            const size_t q = (size_t)i + (size_t)hx * (size_t)j;
+            const double s = c->mwx[i] + c->mwy[j];
+            if (s == 0.0) { c->sre[q] = 0.0; c->sim[q] = 0.0; }
+            else { c->sre[q] = c->sre[q] / s; c->sim[q] = c->sim[q] / s; }
+        }
+    spectrum_c2c(c, 1, +1);
+    spectrum_x_backward(c, f, gl);
+    const double mean = interior_sum(c, f, gl);                                 /* :491-498 */
+    const double sub = mean / (double)(float)(c->nx * c->ny * c->nz);
+    const size_t n = array_len(c, gl);
+    for (size_t q = 0; q < n; q++) f[q] = f[q] - sub;                           /* :503 whole array */
+}
+
+/* poisson_solver_ppn, src/poisson.f90:1038-1173. */
+static void poisson_ppn(orc_ctx *c, double *f, int gl)
+{
+    const int hx = c->nx / 2 + 1, ny = c->ny, nz = c->nz;
+    spectrum_x_forward(c, f, gl);
+    spectrum_scale(c, (double)(float)c->nx);                                    /* :1074 */
+    spectrum_c2c(c, 1, -1);
+    spectrum_scale(c, (double)(float)c->ny);                                    /* :1087 */
+#pragma omp parallel
+    {
+        double *w = (double *)malloc(sizeof(double) * 3 * (size_t)nz);
+#pragma omp for collapse(2) schedule(static)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < hx; i++) {
+                const size_t q = (size_t)i + (size_t)hx * (size_t)j;
+                thomas_column(nz, c->ta, c->tb, c->tc, c->mwx[i], c->mwy[j], c->sre + q, c->sim + q,
+                              (size_t)hx * (size_t)ny, w, w + nz, w + 2 * nz);
+            }
+        free(w);
+    }
+    spectrum_c2c(c, 1, +1);
+    const size_t n = array_len(c, gl);
+    memset(f, 0, n * sizeof(double));                                           /* :1151 */
+    spectrum_x_backward(c, f, gl);
+    const double mean = interior_sum(c, f, gl);                                 /* :1159-1166 */
+    const double sub = mean / (double)(float)(c->nx * c->ny * c->nz);
+    for (size_t q = 0; q < n; q++) f[q] = f[q] - sub;                           /* :1171 */
+}
+
+void orc_poisson_solve(orc_ctx *c, double *f, int gl)
+{
+    switch (c->solver) {
+    case ORC_PP: poisson_pp(c, f, gl); break;
+    case ORC_PPP: poisson_ppp(c, f, gl); break;
+    case ORC_PPN: poisson_ppn(c, f, gl); break;
+    default: break;
+    }
+}
+
+/* -------------------------------------------------------------------------- */
+/* Setup                                                                      */
+/* -------------------------------------------------------------------------- */
+
+/* BC mapping of allocate_navier_stokes_fields, src/navier_stokes.f90:779-1017.
+ * Returns the field bc type for field id on face s given physical code pc. */
+static int field_bc_type(int field, int face, int pc, int *ok)
+{
+    const int vel = (field == ORC_U || field == ORC_V || field == ORC_W);
+    *ok = 1;
+    switch (pc) {
+    case ORC_BC_PERIODIC: return 0;
+    case ORC_BC_WALL: return vel ? 1 : 2;
+    case ORC_BC_INFLOW:
+        if (face == 5) { *ok = 0; return 0; }          /* back face has no Inflow branch (:989-1016) */
+        if (!vel) return 2;
+        if (face < 2) return field == ORC_U ? 1 : 2;     /* left/right (:800-809, :845-854) */
+        return 1;
+    case ORC_BC_OUTFLOW:
+        if (face == 4) { *ok = 0; return 0; }          /* front face has no Outflow branch (:960-987) */
+        return vel ? 2 : 1;
+    }
+    *ok = 0;
+    return 0;
+}
+
+orc_ctx *orc_create(int dim, int nx, int ny, int nz, double lx, double ly, double lz,
+                    const double *x0, const int *bc)
+{
+    if (dim == 2) nz = 1;
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->dim = dim; c->nx = nx; c->ny = ny; c->nz = nz;
+    c->lx = lx; c->ly = ly; c->lz = lz;
+    for (int d = 0; d < 3; d++) c->x0[d] = x0 ? x0[d] : 0.0;
+    for (int s = 0; s < 6; s++) c->bc[s] = (dim == 3 || s < 4) ? bc[s] : ORC_BC_PERIODIC;
+    c->delta = lx / (double)(float)nx;                                          /* src/grid.f90:140 */
+    c->periodic[0] = c->bc[0] == ORC_BC_PERIODIC && c->bc[1] == ORC_BC_PERIODIC;   /* :105-121 */
+    c->periodic[1] = c->bc[2] == ORC_BC_PERIODIC && c->bc[3] == ORC_BC_PERIODIC;
+    c->periodic[2] = dim == 3 ? (c->bc[4] == ORC_BC_PERIODIC && c->bc[5] == ORC_BC_PERIODIC) : 1;
+
+    /* Solver dispatch, src/poisson.f90:70-111. */
+    const int *pb = c->periodic;
+    if (dim == 3) {
+        if (pb[0] && pb[1] && pb[2]) c->solver = ORC_PPP;
+        else if (pb[0] && pb[1] && !pb[2]) c->solver = ORC_PPN;
+        else { free(c); return NULL; }   /* npn / nnn: not restated yet */
+    } else {
+        if (pb[0] && pb[1]) c->solver = ORC_PP;
+        else { free(c); return NULL; }
+    }
+
+    /* allocate_navier_stokes_fields defaults (src/navier_stokes.f90:18-24). */
+    c->density = 1.0; c->viscosity = 1.0; c->cfl = 1.0;
+    field_alloc(c, &c->p, 'c');
+    field_alloc(c, &c->phi, 'c');
+    field_alloc(c, &c->u, 'x');
+    field_alloc(c, &c->v, 'y');
+    field_alloc(c, &c->w, 'z');
+    for (int d = 0; d < 3; d++) {
+        c->dv[d] = (double *)calloc(interior_len(c), sizeof(double));
+        c->dvo[d] = (double *)calloc(interior_len(c), sizeof(double));
+        c->S[d] = (double *)calloc(interior_len(c), sizeof(double));
+        c->gp[d] = (double *)calloc(interior_len(c), sizeof(double));
+    }
+    ofield *fl[5] = {&c->p, &c->phi, &c->u, &c->v, &c->w};
+    const int ids[5] = {ORC_P, ORC_PHI, ORC_U, ORC_V, ORC_W};
+    for (int q = 0; q < 5; q++)
+        for (int s = 0; s < (dim == 3 ? 6 : 4); s++) {
+            int ok;
+            fl[q]->type[s] = field_bc_type(ids[q], s, c->bc[s], &ok);
+        }
+
+    /* init_poisson_* (src/poisson.f90:117-172, 601-763). */
+    fft_init(&c->fx, nx);
+    fft_init(&c->fy, ny);
+    fft_init(&c->fz, nz);
+    c->mwx = modified_wavenumbers(nx, c->delta, 2.0);
+    c->mwy = modified_wavenumbers(ny, c->delta, 2.0);
+    if (c->solver == ORC_PPP) c->mwz = modified_wavenumbers(nz, c->delta, 2.0);
+    if (c->solver == ORC_PPN)
+        tridiag_coefficients(c, nz, c->bc[4] == ORC_BC_INFLOW && c->bc[5] == ORC_BC_OUTFLOW);
+    const size_t ns = (size_t)(nx / 2 + 1) * ny * nz;
+    c->sre = (double *)calloc(ns, sizeof(double));
+    c->sim = (double *)calloc(ns, sizeof(double));
+    return c;
+}
+
+void orc_destroy(orc_ctx *c)
+{
+    if (!c) return;
+    field_free(&c->p); field_free(&c->phi); field_free(&c->u); field_free(&c->v); field_free(&c->w);
+    for (int d = 0; d < 3; d++) { free(c->dv[d]); free(c->dvo[d]); free(c->S[d]); free(c->gp[d]); }
+    fft_free(&c->fx); fft_free(&c->fy); fft_free(&c->fz);
+    free(c->mwx); free(c->mwy); free(c->mwz);
+    free(c->ta); free(c->tb); free(c->tc);
+    free(c->sre); free(c->sim);
+    free(c);
+}
+
+int orc_solver_kind(const orc_ctx *c) { return c->solver; }
+double orc_delta(const orc_ctx *c) { return c->delta; }
+
+void orc_set_params(orc_ctx *c, double density, double viscosity, const double *g, double cfl, int constant_cfl)
+{
+    c->density = density; c->viscosity = viscosity;
+    for (int d = 0; d < 3; d++) c->g[d] = g ? g[d] : 0.0;
+    c->cfl = cfl; c->constant_cfl = constant_cfl;
+}
+
+/* set_timestep, src/navier_stokes.f90:623-666 (single-phase branch). */
+double orc_set_timestep(orc_ctx *c, double U)
+{
+    const double d = c->delta;
+    c->dt_conv = c->cfl * d / U;
+    c->dt_visc = 0.125 * d * d * c->density / c->viscosity;
+    if (c->dim == 3) c->dt_visc = (1.0 / 6.0) * d * d * c->density / c->viscosity;
+    const double dt = c->dt_conv < c->dt_visc ? c->dt_conv : c->dt_visc;
+    c->dt_o = dt;
+    return dt;
+}
+
+void orc_set_dt_o(orc_ctx *c, double dt_o) { c->dt_o = dt_o; }
+double orc_get_dt_o(const orc_ctx *c) { return c->dt_o; }
+
+static ofield *ghosted(orc_ctx *c, int field)
+{
+    switch (field) {
+    case ORC_P: return &c->p;
+    case ORC_PHI: return &c->phi;
+    case ORC_U: return &c->u;
+    case ORC_V: return &c->v;
+    case ORC_W: return &c->w;
+    }
+    return NULL;
+}
+
+double *orc_field(orc_ctx *c, int field)
+{
+    ofield *f = ghosted(c, field);
+    if (f) return f->f;
+    if (field >= ORC_DVO_X && field <= ORC_DVO_Z) return c->dvo[field - ORC_DVO_X];
+    if (field >= ORC_S_X && field <= ORC_S_Z) return c->S[field - ORC_S_X];
+    return NULL;
+}
+
+size_t orc_field_len(const orc_ctx *c, int field)
+{
+    return field <= ORC_W ? ghosted_len(c) : interior_len(c);
+}
+
+double *orc_bc_plane(orc_ctx *c, int field, int face)
+{
+    ofield *f = ghosted(c, field);
+    return f ? f->plane[face] : NULL;
+}
+
+size_t orc_bc_plane_len(const orc_ctx *c, int face) { return plane_len(c, face); }
+
+int orc_bc_type(const orc_ctx *c, int field, int face)
+{
+    ofield *f = ghosted((orc_ctx *)c, field);
+    return f ? f->type[face] : -99;
+}
+
+void orc_set_bc_type(orc_ctx *c, int field, int face, int type)
+{
+    ofield *f = ghosted(c, field);
+    if (f) f->type[face] = type;
+}
+
+void orc_update_ghost_nodes(orc_ctx *c, int field)
+{
+    ofield *f = ghosted(c, field);
+    if (f) ghost_fill(c, f);
+}
+
+/* -------------------------------------------------------------------------- */
+/* Operators (src/fields.f90) and the momentum right-hand side                */
+/* -------------------------------------------------------------------------- */
+
+/* add_advection, src/navier_stokes.f90:261-353; R is interior-sized and is
+ * accumulated into (R - ... ). */
+static void add_advection(const orc_ctx *c, const double *u, const double *v, const double *w,
+                          double *rx, double *ry, double *rz)
+{
+    const int nx = c->nx, ny = c->ny, nz = c->nz, d3 = c->dim == 3;
+    const double id = 1.0 / c->delta;
+    const ptrdiff_t sj = (ptrdiff_t)GX(c), sk = (ptrdiff_t)(GX(c) * GY(c));
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t g = GI(c, i, j, k);
+                const size_t q = II(c, i, j, k);
+                const double *U = u + g, *V = v + g, *W = w + g;
+                /* x-momentum (:297-310) */
+                double uuip = 0.25 * ((U[1] + U[0]) * (U[1] + U[0]));
+                double uuim = 0.25 * ((U[-1] + U[0]) * (U[-1] + U[0]));
+                double uvjp = ((U[sj] + U[0]) * (V[1] + V[0])) * 0.25;
+                double uvjm = ((U[0] + U[-sj]) * (V[1 - sj] + V[-sj])) * 0.25;
+                double r = (rx[q] - (uuip - uuim) * id) - (uvjp - uvjm) * id;
+                if (d3) {
+                    double uwkp = ((U[sk] + U[0]) * (W[1] + W[0])) * 0.25;
+                    double uwkm = ((U[0] + U[-sk]) * (W[1 - sk] + W[-sk])) * 0.25;
+                    r = r - (uwkp - uwkm) * id;
+                }
+                rx[q] = r;
+                /* y-momentum (:315-327) */
+                double vuip = ((V[1] + V[0]) * (U[sj] + U[0])) * 0.25;
+                double vuim = ((V[0] + V[-1]) * (U[-1 + sj] + U[-1])) * 0.25;
+                double vvjp = 0.25 * ((V[sj] + V[0]) * (V[sj] + V[0]));
+                double vvjm = 0.25 * ((V[-sj] + V[0]) * (V[-sj] + V[0]));
+                r = (ry[q] - (vuip - vuim) * id) - (vvjp - vvjm) * id;
+                if (d3) {
+                    double vwkp = ((V[sk] + V[0]) * (W[sj] + W[0])) * 0.25;
+                    double vwkm = ((V[0] + V[-sk]) * (W[sj - sk] + W[-sk])) * 0.25;
+                    r = r - (vwkp - vwkm) * id;
+                }
+                ry[q] = r;
+                if (d3) {
+                    /* z-momentum (:334-347) */
+                    double wuip = ((W[0] + W[1]) * (U[0] + U[sk])) * 0.25;
+                    double wuim = ((W[0] + W[-1]) * (U[-1] + U[-1 + sk])) * 0.25;
+                    double wvjp = ((W[0] + W[sj]) * (V[0] + V[sk])) * 0.25;
+                    double wvjm = ((W[0] + W[-sj]) * (V[-sj] + V[-sj + sk])) * 0.25;
+                    double wwkp = ((W[0] + W[sk]) * (W[0] + W[sk])) * 0.25;
+                    double wwkm = ((W[0] + W[-sk]) * (W[0] + W[-sk])) * 0.25;
+                    rz[q] = ((rz[q] - (wuip - wuim) * id) - (wvjp - wvjm) * id) - (wwkp - wwkm) * id;
+                }
+            }
+}
+
+void orc_add_advection(orc_ctx *c, double *rx, double *ry, double *rz)
+{
+    add_advection(c, c->u.f, c->v.f, c->w.f, rx, ry, rz);
+}
+
+/* add_diffusion, constant-viscosity branch (src/navier_stokes.f90:384-404) with
+ * laplacian_of_vector (src/fields.f90:298-343), then the body-force term
+ * (src/navier_stokes.f90:245-255).  rhof == 0.5*(rho+rho) on a constant
+ * density field (src/fields.f90:175-206). */
+static void add_diffusion_and_source(const orc_ctx *c, double rhof, double *rx, double *ry, double *rz)
+{
+    const int nx = c->nx, ny = c->ny, nz = c->nz, d3 = c->dim == 3;
+    const double id2 = 1.0 / (c->delta * c->delta);
+    const double mu = c->viscosity;
+    const ptrdiff_t sj = (ptrdiff_t)GX(c), sk = (ptrdiff_t)(GX(c) * GY(c));
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t g = GI(c, i, j, k);
+                const size_t q = II(c, i, j, k);
+                const double *U = c->u.f + g, *V = c->v.f + g, *W = c->w.f + g;
+                double lu = (((U[1] - 2.0 * U[0]) + U[-1]) + ((U[sj] - 2.0 * U[0]) + U[-sj])) * id2;
+                double lv = (((V[1] - 2.0 * V[0]) + V[-1]) + ((V[sj] - 2.0 * V[0]) + V[-sj])) * id2;
+                double lw = 0.0;
+                if (d3) {
+                    lu = lu + ((U[sk] - 2.0 * U[0]) + U[-sk]) * id2;
+                    lv = lv + ((V[sk] - 2.0 * V[0]) + V[-sk]) * id2;
+                    lw = ((((W[1] - 2.0 * W[0]) + W[-1]) + ((W[sj] - 2.0 * W[0]) + W[-sj])) +
+                          ((W[sk] - 2.0 * W[0]) + W[-sk])) * id2;
+                }
+                rx[q] = rx[q] + mu * lu / rhof;
+                ry[q] = ry[q] + mu * lv / rhof;
+                if (d3) rz[q] = rz[q] + mu * lw / rhof;
+                rx[q] = rx[q] + c->S[0][q] / rhof;
+                ry[q] = ry[q] + c->S[1][q] / rhof;
+                if (d3) rz[q] = rz[q] + c->S[2][q] / rhof;
+            }
+}
+
+/* gradient_of_scalar, src/fields.f90:31-64. */
+static void gradient(const orc_ctx *c, const double *s, double *gx, double *gy, double *gz)
+{
+    const int nx = c->nx, ny = c->ny, nz = c->nz, d3 = c->dim == 3;
+    const double id = 1.0 / c->delta;
+    const ptrdiff_t sj = (ptrdiff_t)GX(c), sk = (ptrdiff_t)(GX(c) * GY(c));
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const double *S = s + GI(c, i, j, k);
+                const size_t q = II(c, i, j, k);
+                gx[q] = (S[1] - S[0]) * id;
+                gy[q] = (S[sj] - S[0]) * id;
+                if (d3) gz[q] = (S[sk] - S[0]) * id;
+            }
+}
+
+void orc_gradient(orc_ctx *c, const double *s, double *gx, double *gy, double *gz)
+{
+    gradient(c, s, gx, gy, gz);
+}
+
+/* divergence_of_vector, src/fields.f90:120-153; out is interior-sized. */
+static void divergence(const orc_ctx *c, const double *u, const double *v, const double *w, double *out)
+{
+    const int nx = c->nx, ny = c->ny, nz = c->nz, d3 = c->dim == 3;
+    const double id = 1.0 / c->delta;
+    const ptrdiff_t sj = (ptrdiff_t)GX(c), sk = (ptrdiff_t)(GX(c) * GY(c));
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t g = GI(c, i, j, k);
+                double d = (u[g] - u[g - 1]) * id + (v[g] - v[g - sj]) * id;
+                if (d3) d = d + (w[g] - w[g - sk]) * id;
+                out[II(c, i, j, k)] = d;
+            }
+}
+
+void orc_divergence(orc_ctx *c, const double *u, const double *v, const double *w, double *div)
+{
+    divergence(c, u, v, w, div);
+}
+
+/* -------------------------------------------------------------------------- */
+/* navier_stokes_solver, src/navier_stokes.f90:50-136                         */
+/* -------------------------------------------------------------------------- */
+
+static double max_velocity_sum(const orc_ctx *c)
+{
+    double mv = 0.0;
+    for (int k = 1; k <= c->nz; k++)
+        for (int j = 1; j <= c->ny; j++)
+            for (int i = 1; i <= c->nx; i++) {
+                const size_t g = GI(c, i, j, k);
+                double vel = fabs(c->u.f[g]) + fabs(c->v.f[g]);
+                if (c->dim == 3) vel = vel + fabs(c->w.f[g]);
+                if (vel > mv) mv = vel;
+            }
+    return mv;
+}
+
+/* update_timestep, src/navier_stokes.f90:670-730 (single-phase branch). */
+static void update_timestep(orc_ctx *c, double *dt)
+{
+    c->dt_o = *dt;
+    const double mv = max_velocity_sum(c);
+    c->dt_conv = mv > 0.0 ? c->cfl * c->delta / mv : 1.0;
+    *dt = c->dt_conv < c->dt_visc ? c->dt_conv : c->dt_visc;
+    if (*dt > 1.1 * c->dt_o) *dt = 1.1 * c->dt_o;
+}
+
+void orc_ns_step(orc_ctx *c, int step, double *dt_io, double *maxdiv, double *maxcfl)
+{
+    (void)step;
+    const int nx = c->nx, ny = c->ny, nz = c->nz, d3 = c->dim == 3;
+    if (c->constant_cfl) update_timestep(c, dt_io);                            /* :78 */
+    const double dt = *dt_io;
+    const double rhof = 0.5 * (c->density + c->density);                       /* center_to_face */
+    const size_t ni = interior_len(c);
+
+    /* predicted_velocity_field (:140-213) */
+    const double A = 1.0 + 0.5 * dt / c->dt_o;
+    const double B = -(0.5 * dt / c->dt_o);
+    for (int d = 0; d < 3; d++) memset(c->dv[d], 0, ni * sizeof(double));     /* :227-231 */
+    add_advection(c, c->u.f, c->v.f, c->w.f, c->dv[0], c->dv[1], c->dv[2]);   /* :234 */
+    add_diffusion_and_source(c, rhof, c->dv[0], c->dv[1], c->dv[2]);          /* :237, :245-255 */
+    gradient(c, c->p.f, c->gp[0], c->gp[1], c->gp[2]);                         /* :165 */
+    double *vel[3] = {c->u.f, c->v.f, c->w.f};
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t g = GI(c, i, j, k), q = II(c, i, j, k);
+                for (int d = 0; d < (d3 ? 3 : 2); d++) {
+                    const double rhs = ((-(c->gp[d][q] / rhof) + A * c->dv[d][q]) + B * c->dvo[d][q]) + c->g[d];
+                    vel[d][g] = vel[d][g] + dt * rhs;                          /* :187-198 */
+                }
+            }
+    for (int d = 0; d < (d3 ? 3 : 2); d++) memcpy(c->dvo[d], c->dv[d], ni * sizeof(double));  /* :201-205 */
+    ghost_fill(c, &c->u); ghost_fill(c, &c->v); if (d3) ghost_fill(c, &c->w); /* :208 */
+
+    /* Poisson right-hand side (:111-121) */
+    double *div = c->gp[0];   /* scratch */
+    divergence(c, c->u.f, c->v.f, c->w.f, div);
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                c->phi.f[GI(c, i, j, k)] = div[II(c, i, j, k)] * c->density / dt;
+    orc_poisson_solve(c, c->phi.f, 1);                                          /* :123 */
+    ghost_fill(c, &c->phi);                                                     /* :124 */
+
+    /* correct_velocity_field (:505-546) */
+    gradient(c, c->phi.f, c->gp[0], c->gp[1], c->gp[2]);
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t g = GI(c, i, j, k), q = II(c, i, j, k);
+                for (int d = 0; d < (d3 ? 3 : 2); d++)
+                    vel[d][g] = vel[d][g] - c->gp[d][q] * dt / rhof;
+            }
+    ghost_fill(c, &c->u); ghost_fill(c, &c->v); if (d3) ghost_fill(c, &c->w);
+
+    /* update_pressure (:550-566): whole array, then ghosts */
+    const size_t ng = ghosted_len(c);
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < ng; q++) c->p.f[q] = c->p.f[q] + c->phi.f[q];
+    ghost_fill(c, &c->p);
+
+    /* checks (:570-619) */
+    divergence(c, c->u.f, c->v.f, c->w.f, div);
+    double md = -HUGE_VAL;
+    for (size_t q = 0; q < ni; q++) if (div[q] > md) md = div[q];
+    c->maxdiv = md;
+    c->maxcfl = dt * max_velocity_sum(c) / c->delta;
+    if (maxdiv) *maxdiv = c->maxdiv;
+    if (maxcfl) *maxcfl = c->maxcfl;
+}
