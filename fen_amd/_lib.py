@@ -74,6 +74,7 @@ def load():
         "fen_profile_count": (c_int, []),
         "fen_profile_name": (ctypes.c_char_p, [c_int]),
         "fen_profile_read": (c_int, [vp, c_int, dp, ctypes.POINTER(ctypes.c_longlong), dp]),
+        "fen_profile_predecessor": (c_int, [vp, c_int, ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

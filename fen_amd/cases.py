@@ -100,18 +100,16 @@ def tgv3d(N=256, Re=1600.0, steps=200, eps=1e-3, cfl=0.5, Uref=2.0, seed=SEED):
     f = np.arange(1, N + 1, dtype=np.float64) * d
     n3 = N ** 3
 
-    def grid(a, b, cc):
-        return np.meshgrid(a, b, cc, indexing="ij")
+    def outer3(a, b, cc):
+        return np.asfortranarray((a[:, None, None] * b[None, :, None]) * cc[None, None, :])
 
-    X, Y, Z = grid(f, c, c)
-    u = np.sin(X) * np.cos(Y) * np.cos(Z)
-    X, Y, Z = grid(c, f, c)
-    v = -np.cos(X) * np.sin(Y) * np.cos(Z)
-    w = np.zeros((N, N, N))
+    u = outer3(np.sin(f), np.cos(c), np.cos(c))
+    v = outer3(-np.cos(c), np.sin(f), np.cos(c))
+    w = np.zeros((N, N, N), order="F")
     for comp, arr in enumerate((u, v, w)):
         arr += eps * (2.0 * uniform01(comp, n3, seed).reshape((N, N, N), order="F") - 1.0)
-    X, Y, Z = grid(c, c, c)
-    p = (np.cos(2.0 * X) + np.cos(2.0 * Y)) * (np.cos(2.0 * Z) + 2.0) / 16.0
+    c2 = np.cos(2.0 * c)
+    p = np.asfortranarray(((c2[:, None, None] + c2[None, :, None]) * (c2[None, None, :] + 2.0)) / 16.0)
     dt_conv = cfl * d / Uref
     dt_visc = (1.0 / 6.0) * d * d * 1.0 / nu
     dt = min(dt_conv, dt_visc)

@@ -22,28 +22,21 @@ using fen::GhostDesc;
 
 namespace fen {
 
-void prof_begin(fen_ctx *c, int k)
-{
-    if (!c->prof_on) return;
-    ProfSlot &s = c->prof[k];
-    if (s.used + 2 > s.ev.size()) {
-        hipEvent_t e0, e1;
-        FEN_HIP_CHECK(hipEventCreate(&e0));
-        FEN_HIP_CHECK(hipEventCreate(&e1));
-        s.ev.push_back(e0);
-        s.ev.push_back(e1);
-    }
-    FEN_HIP_CHECK(hipEventRecord(s.ev[s.used], c->stream));
-}
+void prof_begin(fen_ctx *c, int k) { (void)c; (void)k; }
 
 void prof_end(fen_ctx *c, int k, double bytes)
 {
-    if (!c->prof_on) return;
-    ProfSlot &s = c->prof[k];
-    FEN_HIP_CHECK(hipEventRecord(s.ev[s.used + 1], c->stream));
-    s.used += 2;
-    s.bytes = bytes;
-    s.launches++;
+    const long long seq = c->prof_seq++;
+    if (!(c->prof_mask & (1u << k))) return;
+    if (c->prof_used == c->prof_pool.size()) {
+        hipEvent_t e;
+        FEN_HIP_CHECK(hipEventCreate(&e));
+        c->prof_pool.push_back(e);
+    }
+    hipEvent_t e = c->prof_pool[c->prof_used++];
+    FEN_HIP_CHECK(hipEventRecord(e, c->stream));
+    c->prof_marks.push_back(ProfMark{k, seq, e});
+    c->prof[k].bytes = bytes;
 }
 }  // namespace fen
 
@@ -151,7 +144,19 @@ GhostDesc make_desc(fen_ctx *c, int field, double *arr)
         d.plane[f] = c->d_plane[s][f];
     }
     d.loc = field_loc(field);
+    d.groups = 7;
     return d;
+}
+
+// Ghost groups left after a producer that wrote the images along `img` axes.
+int remaining_groups(const fen_ctx *c, int img) { return ~img & (c->dim == 3 ? 7 : 3); }
+
+void refresh_images(fen_ctx *c)
+{
+    c->img_vel = fen::image_axes(c->bc_type[FEN_U], c->dim) & fen::image_axes(c->bc_type[FEN_V], c->dim);
+    if (c->dim == 3) c->img_vel &= fen::image_axes(c->bc_type[FEN_W], c->dim);
+    c->img_p = fen::image_axes(c->bc_type[FEN_P], c->dim);
+    c->img_phi = fen::image_axes(c->bc_type[FEN_PHI], c->dim);
 }
 
 double *field_ptr(fen_ctx *c, int field)
@@ -177,7 +182,7 @@ void free_all(fen_ctx *c)
     f(c->d_twx); f(c->d_twy); f(c->d_twz); f(c->d_part); f(c->d_scal);
     for (int s = 0; s < 5; s++) for (int p = 0; p < 6; p++) f(c->d_plane[s][p]);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
-    for (auto &s : c->prof) for (auto e : s.ev) (void)hipEventDestroy(e);
+    for (auto e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -218,9 +223,10 @@ void step_fp64(fen_ctx *c, double dt)
     c->last_dt = dt;
 
     launch_momentum(c, dt, A, B);
-    {
+    if (const int rest = remaining_groups(c, c->img_vel)) {
         GhostDesc g[3] = {make_desc(c, FEN_U, c->d_us[0]), make_desc(c, FEN_V, c->d_us[1]),
                           make_desc(c, FEN_W, c->d_us[2])};
+        for (auto &d : g) d.groups = rest;
         launch_ghost_desc(c, g, d3 ? 3 : 2);
     }
     const double *vs[3] = {c->d_us[0], c->d_us[1], c->d_us[2]};
@@ -243,29 +249,29 @@ void step_fp64(fen_ctx *c, double dt)
     default: throw std::runtime_error("solver not available");
     }
     const bool mean = c->solver == FEN_SOLVER_PP || c->solver == FEN_SOLVER_PPN;
-    launch_c2r(c, c->d_phi, 1, c->box.gx, c->box.gxy, mean);
+    bool dirichlet = false;
+    for (int f = 0; f < 6; f++) dirichlet = dirichlet || c->bc_type[FEN_PHI][f] == 1;
+    // images written by the c2r only when no shift has to be applied first
+    const int img_phi = (mean && dirichlet) ? 0 : c->img_phi;
+    launch_c2r(c, c->d_phi, 1, c->box.gx, c->box.gxy, mean, img_phi);
     c->shift_active = mean;
-    if (mean) {
-        bool dirichlet = false;
-        for (int f = 0; f < 6; f++) dirichlet = dirichlet || c->bc_type[FEN_PHI][f] == 1;
-        if (dirichlet) {   // the shift does not commute with 2*bc - f: apply it first
-            launch_apply_shift(c, c->d_phi, c->n_ghost);
-            c->shift_active = false;
-        }
+    if (mean && dirichlet) {   // the shift does not commute with 2*bc - f: apply it first
+        launch_apply_shift(c, c->d_phi, c->n_ghost);
+        c->shift_active = false;
     }
-    {
+    if (const int rest = remaining_groups(c, img_phi)) {
         GhostDesc g = make_desc(c, FEN_PHI, c->d_phi);
+        g.groups = rest;
         launch_ghost_desc(c, &g, 1);
     }
-    launch_project(c, dt);
-    {
+    launch_project(c, dt, c->d_scal);
+    if (const int rest = remaining_groups(c, c->img_vel & c->img_p)) {
         GhostDesc g[4] = {make_desc(c, FEN_U, c->d_u[0]), make_desc(c, FEN_V, c->d_u[1]),
                           d3 ? make_desc(c, FEN_W, c->d_u[2]) : make_desc(c, FEN_P, c->d_p),
                           make_desc(c, FEN_P, c->d_p)};
+        for (auto &d : g) d.groups = rest;
         launch_ghost_desc(c, g, d3 ? 4 : 3);
     }
-    const double *vn[3] = {c->d_u[0], c->d_u[1], c->d_u[2]};
-    launch_checks(c, vn, c->d_scal);
     c->checks_valid = true;
 }
 
@@ -319,6 +325,7 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
             c->bc_type[s][f] = (dim == 3 || f < 4) ? map_bc(s, f, c->bc_phys[f], &ok) : 0;
             if (!ok) { delete c; return FEN_ERR_UNSUPPORTED; }
         }
+    refresh_images(c);
     Box &b = c->box;
     b.nx = nx; b.ny = ny; b.nz = nz; b.dim = dim;
     b.gx = nx + 2;
@@ -357,7 +364,8 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
         c->d_twx = dev_copy(twiddles(nx));
         c->d_twy = dev_copy(twiddles(ny));
         if (c->solver == FEN_SOLVER_PPP) c->d_twz = dev_copy(twiddles(nz));
-        c->n_part = std::max<size_t>(3 * 1024, (size_t)ny * nz / 2 + 1);
+        c->n_part = std::max<size_t>(std::max<size_t>(3 * 1024, (size_t)ny * nz / 2 + 1),
+                                     3 * (size_t)fen::stencil_blocks(c->box));
         c->d_part = dev_alloc<double>(c->n_part);
         c->d_scal = dev_alloc<double>(8);
         c->d_shift = c->d_scal + 3;
@@ -509,6 +517,7 @@ int fen_set_bc_type(fen_ctx *c, int field, int face, int type)
     const int s = field_slot(field);
     if (s < 0 || face < 0 || face > 5 || type < 0 || type > 2) return fail(c, FEN_ERR_ARG, "bad field, face or type");
     c->bc_type[s][face] = type;
+    refresh_images(c);
     return FEN_OK;
 }
 
@@ -604,7 +613,7 @@ int fen_poisson_solve(fen_ctx *c, double *f, int gl)
         default: return fail(c, FEN_ERR_UNSUPPORTED, "solver not available");
         }
         const bool mean = c->solver != FEN_SOLVER_PPP;
-        fen::launch_c2r(c, buf, gl, sx, sy, mean);
+        fen::launch_c2r(c, buf, gl, sx, sy, mean, 0);
         std::vector<double> host(n);
         FEN_HIP_CHECK(hipMemcpyAsync(host.data(), buf, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         if (mean)
@@ -639,6 +648,7 @@ int fen_add_advection(fen_ctx *c, double *rx, double *ry, double *rz)
         for (int d = 0; d < nd; d++)
             FEN_HIP_CHECK(hipMemcpyAsync(r[d], h[d], c->n_int * sizeof(double), hipMemcpyHostToDevice, c->stream));
         fen::launch_advection_only(c, r[0], r[1], r[2]);
+        fen::prof_end(c, fen::K_MOMENTUM, 0.0);
         for (int d = 0; d < nd; d++)
             FEN_HIP_CHECK(hipMemcpyAsync(h[d], r[d], c->n_int * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         sync(c);
@@ -708,10 +718,10 @@ int fen_synchronize(fen_ctx *c)
     return FEN_OK;
 }
 
-int fen_profile_enable(fen_ctx *c, int enable)
+int fen_profile_enable(fen_ctx *c, int mask)
 {
     if (!c) return FEN_ERR_ARG;
-    c->prof_on = enable != 0;
+    c->prof_mask = (unsigned)mask;
     return FEN_OK;
 }
 
@@ -720,7 +730,8 @@ int fen_profile_reset(fen_ctx *c)
     if (!c) return FEN_ERR_ARG;
     FEN_TRY(c, {
         sync(c);
-        for (auto &s : c->prof) { s.used = 0; s.ms = 0.0; s.launches = 0; }
+        c->prof_marks.clear();
+        c->prof_used = 0;
     });
     return FEN_OK;
 }
@@ -734,17 +745,31 @@ int fen_profile_read(fen_ctx *c, int k, double *total_ms, long long *launches, d
     if (!c || k < 0 || k >= fen::K_COUNT) return FEN_ERR_ARG;
     FEN_TRY(c, {
         sync(c);
-        fen::ProfSlot &s = c->prof[k];
         double ms = 0.0;
-        for (size_t p = 0; p + 1 < s.used; p += 2) {
+        long long n = 0;
+        for (size_t m = 1; m < c->prof_marks.size(); m++) {
+            const fen::ProfMark &cur = c->prof_marks[m], &prev = c->prof_marks[m - 1];
+            if (cur.kernel != k || prev.seq != cur.seq - 1) continue;
             float e = 0.0f;
-            FEN_HIP_CHECK(hipEventElapsedTime(&e, s.ev[p], s.ev[p + 1]));
+            FEN_HIP_CHECK(hipEventElapsedTime(&e, prev.ev, cur.ev));
             ms += e;
+            n++;
         }
         if (total_ms) *total_ms = ms;
-        if (launches) *launches = s.launches;
-        if (bytes) *bytes = s.bytes;
+        if (launches) *launches = n;
+        if (bytes) *bytes = c->prof[k].bytes;
     });
+    return FEN_OK;
+}
+
+int fen_profile_predecessor(fen_ctx *c, int k, int *prev)
+{
+    if (!c || !prev || k < 0 || k >= fen::K_COUNT) return FEN_ERR_ARG;
+    *prev = -1;
+    for (size_t m = 1; m < c->prof_marks.size(); m++) {
+        const fen::ProfMark &cur = c->prof_marks[m], &pm = c->prof_marks[m - 1];
+        if (cur.kernel == k && pm.seq == cur.seq - 1) { *prev = pm.kernel; break; }
+    }
     return FEN_OK;
 }
 

@@ -39,8 +39,50 @@ struct GhostDesc {
     double *f;
     const double *plane[6];
     int type[6];
-    int loc;   // 'c', 'x', 'y', 'z'
+    int loc;      // 'c', 'x', 'y', 'z'
+    int groups;   // ghost groups to fill: bit 0 x faces, bit 1 y faces, bit 2 z faces
 };
+
+// Axes whose ghost layers a producer kernel writes itself: the longest
+// prefix of (x, y, z) along which the field is periodic on both faces.
+// The remaining axes are left to the ghost kernel, which runs after the
+// producer and so sees the periodic images already in place.
+__host__ __device__ inline int image_axes(const int *type, int dim)
+{
+    int f = 0;
+    if (type[0] == 0 && type[1] == 0) f |= 1; else return f;
+    if (type[2] == 0 && type[3] == 0) f |= 2; else return f;
+    if (dim == 3 && type[4] == 0 && type[5] == 0) f |= 4;
+    return f;
+}
+
+// Writes v to every periodic image of interior cell (i,j,k) along the axes
+// in `ax` (bit 0 x, 1 y, 2 z): the ghost cells whose final value is a copy
+// of this cell (src/scalar.f90:254-388, periodic branches).
+__device__ __forceinline__ void write_images(double *f, const Box &b, int ax, int i, int j, int k, double v)
+{
+    const int ii = (ax & 1) ? (i == 1 ? b.nx + 1 : (i == b.nx ? 0 : -1)) : -1;
+    const int jj = (ax & 2) ? (j == 1 ? b.ny + 1 : (j == b.ny ? 0 : -1)) : -1;
+    const int kk = (ax & 4) ? (k == 1 ? b.nz + 1 : (k == b.nz ? 0 : -1)) : -1;
+    if (ii < 0 && jj < 0 && kk < 0) return;
+    if (ii >= 0) f[b.g(ii, j, k)] = v;
+    if (jj >= 0) f[b.g(i, jj, k)] = v;
+    if (kk >= 0) f[b.g(i, j, kk)] = v;
+    if (ii >= 0 && jj >= 0) f[b.g(ii, jj, k)] = v;
+    if (ii >= 0 && kk >= 0) f[b.g(ii, j, kk)] = v;
+    if (jj >= 0 && kk >= 0) f[b.g(i, jj, kk)] = v;
+    if (ii >= 0 && jj >= 0 && kk >= 0) f[b.g(ii, jj, kk)] = v;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; this
+// renumbering gives every XCD one contiguous range of the grid (a k-slab),
+// so the stencil neighbours a workgroup needs are in its own XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    if (nb & 7) return b;
+    const int per = nb >> 3;
+    return (b & 7) * per + (b >> 3);
+}
 
 // Kernels whose launches are timed by the profiler.
 enum KernelId {
@@ -59,12 +101,17 @@ enum KernelId {
     K_COUNT
 };
 
+// One profiled launch: an event recorded on the stream right after it.  A
+// launch's duration is the time between its event and the event of the launch
+// just before it in the stream (seq - 1), when that one was profiled too.
+struct ProfMark {
+    int kernel;
+    long long seq;
+    hipEvent_t ev;
+};
+
 struct ProfSlot {
-    std::vector<hipEvent_t> ev;   // begin/end pairs
-    size_t used = 0;
-    double ms = 0.0;
-    long long launches = 0;
-    double bytes = 0.0;
+    double bytes = 0.0;   // algorithmic HBM bytes per launch (last launch)
 };
 
 }  // namespace fen
@@ -109,6 +156,8 @@ struct fen_ctx {
     double *d_shift = nullptr;   // device copy of the subtracted mean
     bool shift_active = false;
 
+    int img_vel = 0, img_p = 0, img_phi = 0;   // image_axes of (u,v,w), p, phi
+
     // reductions
     double *d_part = nullptr;    // per-block partials
     size_t n_part = 0;
@@ -120,7 +169,11 @@ struct fen_ctx {
     hipStream_t stream = nullptr;
     std::string err;
 
-    bool prof_on = false;
+    unsigned prof_mask = 0;
+    long long prof_seq = 0;                 // launches issued on the stream
+    std::vector<fen::ProfMark> prof_marks;  // chronological
+    std::vector<hipEvent_t> prof_pool;      // events owned by the context
+    size_t prof_used = 0;
     fen::ProfSlot prof[fen::K_COUNT];
 };
 
@@ -136,9 +189,10 @@ void launch_y_fft(fen_ctx *c, int sign, double post_scale);
 void launch_y_solve_2d(fen_ctx *c, double inv_n);
 void launch_z_solve(fen_ctx *c, double inv_n);
 void launch_thomas(fen_ctx *c);
-void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want_sum);
-void launch_project(fen_ctx *c, double dt);
+void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want_sum, int img);
+void launch_project(fen_ctx *c, double dt, double *checks_out);
 void launch_checks(fen_ctx *c, const double *const *vel, double *out2);
+long stencil_blocks(const Box &b);   // workgroups of the momentum / projection grids
 void launch_apply_shift(fen_ctx *c, double *f, size_t n);
 void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz);
 

@@ -87,6 +87,8 @@ struct C2RArgs {
     long nrows, kxs, sx, sy;
     int ny, gl, want_sum;
     double *part;
+    Box b;     // ghosted box of `out` when img != 0
+    int img;   // periodic image axes to write (phi in the time step)
 };
 
 template <int N>
@@ -123,6 +125,7 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs a)
         const double val = (row & 1) ? z.y : z.x;
         const long j = r % a.ny, k = r / a.ny;
         a.out[(i + a.gl) + a.sx * (j + a.gl) + a.sy * (k + a.gl)] = val;
+        if (a.img) write_images(a.out, a.b, a.img, i + 1, (int)j + 1, (int)k + 1, val);
         acc += val;
     }
     if (a.want_sum) {
@@ -452,7 +455,7 @@ void launch_thomas(fen_ctx *c)
     prof_end(c, K_THOMAS, 2.0 * spec_bytes(c));
 }
 
-void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want_sum)
+void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want_sum, int img)
 {
     C2RArgs a{};
     a.spec = c->d_spec;
@@ -466,6 +469,8 @@ void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want
     a.gl = gl_out;
     a.want_sum = want_sum ? 1 : 0;
     a.part = c->d_part;
+    a.b = c->box;
+    a.img = img;
     const int tj = 2 * (kBlock / (c->nx / 8));
     const long nblk = (a.nrows + tj - 1) / tj;
     if ((size_t)nblk > c->n_part) throw std::runtime_error("partial-sum buffer too small");
@@ -487,6 +492,7 @@ void launch_apply_shift(fen_ctx *c, double *f, size_t n)
 {
     hipLaunchKernelGGL(apply_shift_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, f, (long)n,
                        (const double *)c->d_shift);
+    prof_end(c, K_REDUCE, 16.0 * n);
 }
 
 }  // namespace fen

@@ -3,13 +3,18 @@
 // Every expression keeps the reference's evaluation order (Fortran left to
 // right, explicit parentheses) and the library is compiled with
 // -ffp-contract=off, so these kernels reproduce the CPU arithmetic bit for bit.
+//
+// Producers (momentum, projection) write the periodic images of the boundary
+// cells they compute, so a periodic ghost fill costs no extra pass; the ghost
+// kernel then runs only over the axes that are not periodic.
 #include <stdexcept>
 
 #include "fen_internal.hpp"
+#include "stencil_march.hpp"
 
 namespace fen {
 
-constexpr int kTX = 64, kTY = 4;
+constexpr int kTX = 64, kTY = 4, kThreads = kTX * kTY;
 
 // ---------------------------------------------------------------------------
 // Ghost fill: scalar%update_ghost_nodes (src/scalar.f90:223-396) evaluated
@@ -24,11 +29,9 @@ constexpr int kTX = 64, kTY = 4;
 struct GhostLaunch {
     GhostDesc d[4];
     Box b;
-    long nx_cells, ny_cells, nz_cells;   // cells per group, per field
-    int ngx, ngy, ngz;                    // planes per group (2 or 3)
+    long nx_cells, ny_cells, nz_cells;   // cells per plane of each group
+    int ngx[4], ngy[4], ngz[4];           // planes per group and field (0 when skipped)
 };
-
-__device__ __forceinline__ double plane_at(const double *p, long idx) { return p[idx]; }
 
 __device__ double ghost_s1(const GhostDesc &d, const Box &b, int i, int j, int k)
 {
@@ -37,7 +40,7 @@ __device__ double ghost_s1(const GhostDesc &d, const Box &b, int i, int j, int k
         const int t = d.type[0];
         if (t == 0) return d.f[b.g(b.nx, j, k)];
         if (t == 1) {
-            const double bc = plane_at(d.plane[0], pj);
+            const double bc = d.plane[0][pj];
             return d.loc == 'x' ? bc : 2.0 * bc - d.f[b.g(1, j, k)];
         }
         if (t == 2) return d.f[b.g(1, j, k)];
@@ -47,13 +50,13 @@ __device__ double ghost_s1(const GhostDesc &d, const Box &b, int i, int j, int k
         const int t = d.type[1];
         if (t == 0) return d.f[b.g(1, j, k)];
         if (t == 1) {
-            const double bc = plane_at(d.plane[1], pj);
+            const double bc = d.plane[1][pj];
             return d.loc == 'x' ? bc : 2.0 * bc - d.f[b.g(b.nx, j, k)];
         }
         if (t == 2) return d.f[b.g(b.nx, j, k)];
         return d.f[b.g(i, j, k)];
     }
-    if (i == b.nx && d.type[1] == 1 && d.loc == 'x') return plane_at(d.plane[1], pj);
+    if (i == b.nx && d.type[1] == 1 && d.loc == 'x') return d.plane[1][pj];
     return d.f[b.g(i, j, k)];
 }
 
@@ -64,7 +67,7 @@ __device__ double ghost_s2(const GhostDesc &d, const Box &b, int i, int j, int k
         const int t = d.type[2];
         if (t == 0) return ghost_s1(d, b, i, b.ny, k);
         if (t == 1) {
-            const double bc = plane_at(d.plane[2], pi);
+            const double bc = d.plane[2][pi];
             return d.loc == 'y' ? bc : 2.0 * bc - ghost_s1(d, b, i, 1, k);
         }
         if (t == 2) return ghost_s1(d, b, i, 1, k);
@@ -74,13 +77,13 @@ __device__ double ghost_s2(const GhostDesc &d, const Box &b, int i, int j, int k
         const int t = d.type[3];
         if (t == 0) return ghost_s1(d, b, i, 1, k);
         if (t == 1) {
-            const double bc = plane_at(d.plane[3], pi);
+            const double bc = d.plane[3][pi];
             return d.loc == 'y' ? bc : 2.0 * bc - ghost_s1(d, b, i, b.ny, k);
         }
         if (t == 2) return ghost_s1(d, b, i, b.ny, k);
         return ghost_s1(d, b, i, j, k);
     }
-    if (j == b.ny && d.type[3] == 1 && d.loc == 'y') return plane_at(d.plane[3], pi);
+    if (j == b.ny && d.type[3] == 1 && d.loc == 'y') return d.plane[3][pi];
     return ghost_s1(d, b, i, j, k);
 }
 
@@ -92,7 +95,7 @@ __device__ double ghost_s3(const GhostDesc &d, const Box &b, int i, int j, int k
             const int t = d.type[4];
             if (t == 0) return ghost_s2(d, b, i, j, b.nz);
             if (t == 1) {
-                const double bc = plane_at(d.plane[4], pij);
+                const double bc = d.plane[4][pij];
                 return d.loc == 'z' ? bc : 2.0 * bc - ghost_s2(d, b, i, j, 1);
             }
             if (t == 2) return ghost_s2(d, b, i, j, 1);
@@ -102,187 +105,123 @@ __device__ double ghost_s3(const GhostDesc &d, const Box &b, int i, int j, int k
             const int t = d.type[5];
             if (t == 0) return ghost_s2(d, b, i, j, 1);
             if (t == 1) {
-                const double bc = plane_at(d.plane[5], pij);
+                const double bc = d.plane[5][pij];
                 return d.loc == 'z' ? bc : 2.0 * bc - ghost_s2(d, b, i, j, b.nz);
             }
             if (t == 2) return ghost_s2(d, b, i, j, b.nz);
             return ghost_s2(d, b, i, j, k);
         }
-        if (k == b.nz && d.type[5] == 1 && d.loc == 'z') return plane_at(d.plane[5], pij);
+        if (k == b.nz && d.type[5] == 1 && d.loc == 'z') return d.plane[5][pij];
     }
     return ghost_s2(d, b, i, j, k);
 }
 
-// Map group-local plane index p to the coordinate: 0 -> 0, 1 -> n+1, 2 -> n.
+// Group plane index p -> coordinate: 0 -> 0, 1 -> n+1, 2 -> n (normal overwrite).
 __device__ __forceinline__ int plane_coord(int p, int n) { return p == 0 ? 0 : (p == 1 ? n + 1 : n); }
 
 __global__ __launch_bounds__(256) void ghost_kernel(GhostLaunch a)
 {
-    const GhostDesc &d = a.d[blockIdx.y];
+    const int q = blockIdx.y;
+    const GhostDesc &d = a.d[q];
     const Box &b = a.b;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long tx = a.nx_cells * a.ngx, ty = a.ny_cells * a.ngy, tz = a.nz_cells * a.ngz;
-    const int k_lo = b.dim == 3 ? 1 : 1, k_n = b.nz;
+    const long tx = a.nx_cells * a.ngx[q], ty = a.ny_cells * a.ngy[q], tz = a.nz_cells * a.ngz[q];
     int i, j, k;
     if (t < tx) {
-        // x group: i ghost (or overwritten normal plane), j and k interior
+        // x faces: i ghost (or the overwritten normal plane), j and k interior
         const int p = (int)(t / a.nx_cells);
         const long r = t % a.nx_cells;
         i = plane_coord(p, b.nx);
         j = (int)(r % b.ny) + 1;
-        k = (int)(r / b.ny) + k_lo;
+        k = (int)(r / b.ny) + 1;
     } else if ((t -= tx) < ty) {
-        // y group: j ghost, all i, k interior
+        // y faces: j ghost, every i, k interior
         const int p = (int)(t / a.ny_cells);
         const long r = t % a.ny_cells;
         j = plane_coord(p, b.ny);
         i = (int)(r % (b.nx + 2));
-        k = (int)(r / (b.nx + 2)) + k_lo;
+        k = (int)(r / (b.nx + 2)) + 1;
     } else if ((t -= ty) < tz) {
-        // z group: k ghost, all i and j
+        // z faces: k ghost, every i and j
         const int p = (int)(t / a.nz_cells);
         const long r = t % a.nz_cells;
-        k = plane_coord(p, k_n);
+        k = plane_coord(p, b.nz);
         i = (int)(r % (b.nx + 2));
         j = (int)(r / (b.nx + 2));
     } else {
         return;
     }
-    const double val = ghost_s3(d, b, i, j, k);
-    d.f[b.g(i, j, k)] = val;
-}
-
-static int planes_for(const GhostDesc *d, int n, int face_hi, int loc)
-{
-    // 3 planes if any field has a Dirichlet rule on its normal high face.
-    for (int q = 0; q < n; q++)
-        if (d[q].type[face_hi] == 1 && d[q].loc == loc) return 3;
-    return 2;
+    d.f[b.g(i, j, k)] = ghost_s3(d, b, i, j, k);
 }
 
 void launch_ghost_desc(fen_ctx *c, const GhostDesc *descs, int n)
 {
     if (n <= 0 || n > 4) throw std::runtime_error("ghost fill of 1..4 fields");
     GhostLaunch a{};
-    for (int q = 0; q < n; q++) a.d[q] = descs[q];
-    a.b = c->box;
     const Box &b = c->box;
-    a.ngx = planes_for(descs, n, FEN_RIGHT, 'x');
-    a.ngy = planes_for(descs, n, FEN_TOP, 'y');
-    a.ngz = b.dim == 3 ? planes_for(descs, n, FEN_BACK, 'z') : 0;
+    a.b = b;
     a.nx_cells = (long)b.ny * b.nz;
     a.ny_cells = (long)(b.nx + 2) * b.nz;
     a.nz_cells = (long)(b.nx + 2) * (b.ny + 2);
-    const long total = a.nx_cells * a.ngx + a.ny_cells * a.ngy + a.nz_cells * a.ngz;
-    const dim3 grid((unsigned)((total + 255) / 256), (unsigned)n);
+    long most = 0;
+    double cells = 0.0;
+    for (int q = 0; q < n; q++) {
+        const GhostDesc &d = descs[q];
+        a.d[q] = d;
+        a.ngx[q] = (d.groups & 1) ? (d.type[FEN_RIGHT] == 1 && d.loc == 'x' ? 3 : 2) : 0;
+        a.ngy[q] = (d.groups & 2) ? (d.type[FEN_TOP] == 1 && d.loc == 'y' ? 3 : 2) : 0;
+        a.ngz[q] = (b.dim == 3 && (d.groups & 4)) ? (d.type[FEN_BACK] == 1 && d.loc == 'z' ? 3 : 2) : 0;
+        const long tot = a.nx_cells * a.ngx[q] + a.ny_cells * a.ngy[q] + a.nz_cells * a.ngz[q];
+        most = tot > most ? tot : most;
+        cells += (double)tot;
+    }
+    if (most == 0) return;
+    const dim3 grid((unsigned)((most + 255) / 256), (unsigned)n);
     prof_begin(c, K_GHOST);
     hipLaunchKernelGGL(ghost_kernel, grid, dim3(256), 0, c->stream, a);
-    prof_end(c, K_GHOST, 16.0 * total * n);
+    prof_end(c, K_GHOST, 16.0 * cells);
 }
 
-// ---------------------------------------------------------------------------
-// K1: momentum.  dv = advection (src/navier_stokes.f90:261-353) + mu lap(v)/rhof
-// (:384-404, src/fields.f90:298-343) + S/rhof (:245-255); then
-// RHS = -grad(p)/rhof + A dv + B dv_o + g and v* = v + dt RHS (:157-198);
-// dv_o = dv (:201-205).  rhof == density exactly for a constant density
-// field (0.5*(rho+rho) == rho), mu == viscosity.
-// ---------------------------------------------------------------------------
-struct MomArgs {
-    const double *u, *v, *w, *p;
-    double *us, *vs, *ws;
-    double *dvo0, *dvo1, *dvo2;
-    const double *s0, *s1, *s2;
-    Box b;
-    double id, id2, mu, rhof, dt, A, B, g0, g1, g2;
-};
-
-template <int D3, int SRC>
-__global__ __launch_bounds__(kTX * kTY) void momentum_kernel(MomArgs a)
+// 2-D: one cell per thread, neighbours straight from memory (the whole
+// 128^2 problem fits in cache).
+template <int SRC>
+__global__ __launch_bounds__(kThreads) void momentum_kernel_2d(MomArgs a)
 {
-    const int i = blockIdx.x * kTX + threadIdx.x + 1;
-    const int j = blockIdx.y * kTY + threadIdx.y + 1;
-    const int k = D3 ? (int)blockIdx.z + 1 : 1;
+    const int i = blockIdx.x * kTX + (threadIdx.x % kTX) + 1;
+    const int j = blockIdx.y * kTY + (threadIdx.x / kTX) + 1;
     if (i > a.b.nx || j > a.b.ny) return;
-    const long g = a.b.g(i, j, k);
-    const long q = a.b.q(i, j, k);
-    const long sj = a.b.gx, sk = a.b.gxy;
-    const double *U = a.u + g, *V = a.v + g, *W = a.w + g;
-    const double id = a.id, id2 = a.id2;
+    const long g = a.b.g(i, j, 1), sj = a.b.gx;
+    auto at = [&](const double *f) {
+        return [=](int di, int dj, int dk) { return f[g + di + sj * dj]; };
+    };
+    const long q = a.b.q(i, j, 1);
+    momentum_cell<0, SRC>(a, i, j, 1, at(a.u), at(a.v), at(a.w), at(a.p), a.dvo0[q], a.dvo1[q], 0.0);
+}
 
-    // ---- x momentum
-    double r;
-    {
-        const double uuip = 0.25 * ((U[1] + U[0]) * (U[1] + U[0]));
-        const double uuim = 0.25 * ((U[-1] + U[0]) * (U[-1] + U[0]));
-        const double uvjp = ((U[sj] + U[0]) * (V[1] + V[0])) * 0.25;
-        const double uvjm = ((U[0] + U[-sj]) * (V[1 - sj] + V[-sj])) * 0.25;
-        r = (0.0 - (uuip - uuim) * id) - (uvjp - uvjm) * id;
-        if constexpr (D3) {
-            const double uwkp = ((U[sk] + U[0]) * (W[1] + W[0])) * 0.25;
-            const double uwkm = ((U[0] + U[-sk]) * (W[1 - sk] + W[-sk])) * 0.25;
-            r = r - (uwkp - uwkm) * id;
-        }
-    }
-    double lu = (((U[1] - 2.0 * U[0]) + U[-1]) + ((U[sj] - 2.0 * U[0]) + U[-sj])) * id2;
-    if constexpr (D3) lu = lu + ((U[sk] - 2.0 * U[0]) + U[-sk]) * id2;
-    r = r + a.mu * lu / a.rhof;
-    if constexpr (SRC) r = r + a.s0[q] / a.rhof;
-    else r = r + 0.0;
-    const double dvx = r;
+// 3-D launch shapes (see stencil_march.hpp), chosen by a sweep of tile
+// height, planes per workgroup and ring unrolling on MI355X at 256^3.
+constexpr int MTY = 4, MKB = 32, MUN = 3;   // momentum
+constexpr int PTY = 4, PKB = 16, PUN = 1;   // projection
+constexpr int MTX = 64;
 
-    // ---- y momentum
-    {
-        const double vuip = ((V[1] + V[0]) * (U[sj] + U[0])) * 0.25;
-        const double vuim = ((V[0] + V[-1]) * (U[-1 + sj] + U[-1])) * 0.25;
-        const double vvjp = 0.25 * ((V[sj] + V[0]) * (V[sj] + V[0]));
-        const double vvjm = 0.25 * ((V[-sj] + V[0]) * (V[-sj] + V[0]));
-        r = (0.0 - (vuip - vuim) * id) - (vvjp - vvjm) * id;
-        if constexpr (D3) {
-            const double vwkp = ((V[sk] + V[0]) * (W[sj] + W[0])) * 0.25;
-            const double vwkm = ((V[0] + V[-sk]) * (W[sj - sk] + W[-sk])) * 0.25;
-            r = r - (vwkp - vwkm) * id;
-        }
+static void grid_blocks(const Box &b, int ty, int kb, int *nbx, int *nby, int *nbz)
+{
+    if (b.dim == 3) {
+        *nbx = (b.nx + MTX - 1) / MTX;
+        *nby = (b.ny + ty - 1) / ty;
+        *nbz = (b.nz + kb - 1) / kb;
+    } else {
+        *nbx = (b.nx + kTX - 1) / kTX;
+        *nby = (b.ny + kTY - 1) / kTY;
+        *nbz = 1;
     }
-    double lv = (((V[1] - 2.0 * V[0]) + V[-1]) + ((V[sj] - 2.0 * V[0]) + V[-sj])) * id2;
-    if constexpr (D3) lv = lv + ((V[sk] - 2.0 * V[0]) + V[-sk]) * id2;
-    r = r + a.mu * lv / a.rhof;
-    if constexpr (SRC) r = r + a.s1[q] / a.rhof;
-    else r = r + 0.0;
-    const double dvy = r;
+}
 
-    const double *P = a.p + g;
-    const double A = a.A, B = a.B, dt = a.dt;
-    {
-        const double gp = (P[1] - P[0]) * id;
-        const double rhs = ((-(gp / a.rhof) + A * dvx) + B * a.dvo0[q]) + a.g0;
-        a.us[g] = U[0] + dt * rhs;
-        a.dvo0[q] = dvx;
-    }
-    {
-        const double gp = (P[sj] - P[0]) * id;
-        const double rhs = ((-(gp / a.rhof) + A * dvy) + B * a.dvo1[q]) + a.g1;
-        a.vs[g] = V[0] + dt * rhs;
-        a.dvo1[q] = dvy;
-    }
-    if constexpr (D3) {
-        // ---- z momentum
-        const double wuip = ((W[0] + W[1]) * (U[0] + U[sk])) * 0.25;
-        const double wuim = ((W[0] + W[-1]) * (U[-1] + U[-1 + sk])) * 0.25;
-        const double wvjp = ((W[0] + W[sj]) * (V[0] + V[sk])) * 0.25;
-        const double wvjm = ((W[0] + W[-sj]) * (V[-sj] + V[-sj + sk])) * 0.25;
-        const double wwkp = ((W[0] + W[sk]) * (W[0] + W[sk])) * 0.25;
-        const double wwkm = ((W[0] + W[-sk]) * (W[0] + W[-sk])) * 0.25;
-        r = ((0.0 - (wuip - wuim) * id) - (wvjp - wvjm) * id) - (wwkp - wwkm) * id;
-        const double lw = ((((W[1] - 2.0 * W[0]) + W[-1]) + ((W[sj] - 2.0 * W[0]) + W[-sj])) +
-                           ((W[sk] - 2.0 * W[0]) + W[-sk])) * id2;
-        r = r + a.mu * lw / a.rhof;
-        if constexpr (SRC) r = r + a.s2[q] / a.rhof;
-        else r = r + 0.0;
-        const double gp = (P[sk] - P[0]) * id;
-        const double rhs = ((-(gp / a.rhof) + A * r) + B * a.dvo2[q]) + a.g2;
-        a.ws[g] = W[0] + dt * rhs;
-        a.dvo2[q] = r;
-    }
+long stencil_blocks(const Box &b)   // workgroups of the projection (one partial each)
+{
+    int nbx, nby, nbz;
+    grid_blocks(b, PTY, PKB, &nbx, &nby, &nbz);
+    return (long)nbx * nby * nbz;
 }
 
 void launch_momentum(fen_ctx *c, double dt, double A, double B)
@@ -299,18 +238,22 @@ void launch_momentum(fen_ctx *c, double dt, double A, double B)
     a.rhof = 0.5 * (c->density + c->density);
     a.dt = dt; a.A = A; a.B = B;
     a.g0 = c->g[0]; a.g1 = c->g[1]; a.g2 = c->g[2];
-    const Box &b = c->box;
-    const dim3 grid((unsigned)((b.nx + kTX - 1) / kTX), (unsigned)((b.ny + kTY - 1) / kTY),
-                    (unsigned)(b.dim == 3 ? b.nz : 1));
-    const int d3 = b.dim == 3;
+    a.img = c->img_vel;
+    int nbz;
+    grid_blocks(c->box, MTY, MKB, &a.nbx, &a.nby, &nbz);
+    const int d3 = c->box.dim == 3;
     const double words = d3 ? (c->has_source ? 16.0 : 13.0) : (c->has_source ? 11.0 : 9.0);
     prof_begin(c, K_MOMENTUM);
     if (d3) {
-        if (c->has_source) hipLaunchKernelGGL((momentum_kernel<1, 1>), grid, dim3(kTX, kTY), 0, c->stream, a);
-        else hipLaunchKernelGGL((momentum_kernel<1, 0>), grid, dim3(kTX, kTY), 0, c->stream, a);
+        const dim3 grid((unsigned)(a.nbx * a.nby * nbz));
+        if (c->has_source)
+            hipLaunchKernelGGL((momentum_march_kernel<1, MTY, MKB, MUN>), grid, dim3(64 * MTY), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((momentum_march_kernel<0, MTY, MKB, MUN>), grid, dim3(64 * MTY), 0, c->stream, a);
     } else {
-        if (c->has_source) hipLaunchKernelGGL((momentum_kernel<0, 1>), grid, dim3(kTX, kTY), 0, c->stream, a);
-        else hipLaunchKernelGGL((momentum_kernel<0, 0>), grid, dim3(kTX, kTY), 0, c->stream, a);
+        const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
+        if (c->has_source) hipLaunchKernelGGL((momentum_kernel_2d<1>), grid, dim3(kThreads), 0, c->stream, a);
+        else hipLaunchKernelGGL((momentum_kernel_2d<0>), grid, dim3(kThreads), 0, c->stream, a);
     }
     prof_end(c, K_MOMENTUM, words * 8.0 * c->n_int);
 }
@@ -318,11 +261,11 @@ void launch_momentum(fen_ctx *c, double dt, double A, double B)
 // add_advection alone on the resident velocity (reference test
 // test/small_test/navier_stokes/advection/advection.f90:50): R -= adv.
 template <int D3>
-__global__ __launch_bounds__(kTX * kTY) void advection_kernel(const double *u, const double *v, const double *w,
-                                                              double *rx, double *ry, double *rz, Box b, double id)
+__global__ __launch_bounds__(kThreads) void advection_kernel(const double *u, const double *v, const double *w,
+                                                             double *rx, double *ry, double *rz, Box b, double id)
 {
-    const int i = blockIdx.x * kTX + threadIdx.x + 1;
-    const int j = blockIdx.y * kTY + threadIdx.y + 1;
+    const int i = blockIdx.x * kTX + (threadIdx.x % kTX) + 1;
+    const int j = blockIdx.y * kTY + (threadIdx.x / kTX) + 1;
     const int k = D3 ? (int)blockIdx.z + 1 : 1;
     if (i > b.nx || j > b.ny) return;
     const long g = b.g(i, j, k), q = b.q(i, j, k);
@@ -367,81 +310,100 @@ void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz)
     const dim3 grid((unsigned)((b.nx + kTX - 1) / kTX), (unsigned)((b.ny + kTY - 1) / kTY),
                     (unsigned)(b.dim == 3 ? b.nz : 1));
     if (b.dim == 3)
-        hipLaunchKernelGGL((advection_kernel<1>), grid, dim3(kTX, kTY), 0, c->stream, c->d_u[0], c->d_u[1],
+        hipLaunchKernelGGL((advection_kernel<1>), grid, dim3(kThreads), 0, c->stream, c->d_u[0], c->d_u[1],
                            c->d_u[2], rx, ry, rz, b, 1.0 / c->delta);
     else
-        hipLaunchKernelGGL((advection_kernel<0>), grid, dim3(kTX, kTY), 0, c->stream, c->d_u[0], c->d_u[1],
+        hipLaunchKernelGGL((advection_kernel<0>), grid, dim3(kThreads), 0, c->stream, c->d_u[0], c->d_u[1],
                            c->d_u[2], rx, ry, rz, b, 1.0 / c->delta);
 }
 
-// ---------------------------------------------------------------------------
-// K7: correct_velocity_field (src/navier_stokes.f90:505-546) and the interior
-// of update_pressure (:550-566): v = v* - ((grad(phi)*dt)/rhof), p = p + phi.
-// phi may carry a pending mean shift (pp, ppn) that is subtracted here,
-// value by value, exactly where the reference has already subtracted it.
-// ---------------------------------------------------------------------------
-struct ProjArgs {
-    const double *us, *vs, *ws, *phi;
-    double *u, *v, *w, *p;
-    const double *shift;
-    Box b;
-    double id, dt, rhof;
-};
-
-template <int D3, int SHIFT>
-__global__ __launch_bounds__(kTX * kTY) void project_kernel(ProjArgs a)
+template <int SHIFT>
+__global__ __launch_bounds__(kThreads) void project_kernel_2d(ProjArgs a)
 {
-    const int i = blockIdx.x * kTX + threadIdx.x + 1;
-    const int j = blockIdx.y * kTY + threadIdx.y + 1;
-    const int k = D3 ? (int)blockIdx.z + 1 : 1;
-    if (i > a.b.nx || j > a.b.ny) return;
-    const long g = a.b.g(i, j, k);
-    const long sj = a.b.gx, sk = a.b.gxy;
-    const double *F = a.phi + g;
-    double s = 0.0;
-    if constexpr (SHIFT) s = a.shift[0];
-    auto ph = [&](long o) -> double {
-        if constexpr (SHIFT) return F[o] - s;
-        else return F[o];
-    };
-    const double f0 = ph(0);
-    const double gx = (ph(1) - f0) * a.id;
-    const double gy = (ph(sj) - f0) * a.id;
-    a.u[g] = a.us[g] - gx * a.dt / a.rhof;
-    a.v[g] = a.vs[g] - gy * a.dt / a.rhof;
-    if constexpr (D3) {
-        const double gz = (ph(sk) - f0) * a.id;
-        a.w[g] = a.ws[g] - gz * a.dt / a.rhof;
+    const int i = blockIdx.x * kTX + (threadIdx.x % kTX) + 1;
+    const int j = blockIdx.y * kTY + (threadIdx.x / kTX) + 1;
+    const Box &b = a.b;
+    double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
+    if (i <= b.nx && j <= b.ny) {
+        const long g = b.g(i, j, 1), sj = b.gx;
+        const double s = SHIFT ? a.shift[0] : 0.0;
+        auto F = [&](int di, int dj, int dk) {
+            if constexpr (SHIFT) return a.phi[g + di + sj * dj] - s;
+            else return a.phi[g + di + sj * dj];
+        };
+        project_cell<0>(a, i, j, 1, F, a.us[g], a.vs[g], 0.0, a.p[g], a.us[g - 1], a.vs[g - sj], 0.0, md, mv, bad);
     }
-    a.p[g] = a.p[g] + f0;
+    block_max3<kThreads>(md, mv, bad, a.part, (long)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
-void launch_project(fen_ctx *c, double dt)
+__global__ __launch_bounds__(256) void checks_final_kernel(const double *part, long n, double *out)
+{
+    __shared__ double sd[256], sv[256], sf[256];
+    double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
+    for (long q = threadIdx.x; q < n; q += 256) {
+        if (part[3 * q] > md) md = part[3 * q];
+        if (part[3 * q + 1] > mv) mv = part[3 * q + 1];
+        if (part[3 * q + 2] > bad) bad = part[3 * q + 2];
+    }
+    sd[threadIdx.x] = md; sv[threadIdx.x] = mv; sf[threadIdx.x] = bad;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            if (sd[threadIdx.x + s] > sd[threadIdx.x]) sd[threadIdx.x] = sd[threadIdx.x + s];
+            if (sv[threadIdx.x + s] > sv[threadIdx.x]) sv[threadIdx.x] = sv[threadIdx.x + s];
+            if (sf[threadIdx.x + s] > sf[threadIdx.x]) sf[threadIdx.x] = sf[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = sd[0]; out[1] = sv[0]; out[2] = sf[0]; }
+}
+
+void launch_project(fen_ctx *c, double dt, double *checks_out)
 {
     ProjArgs a{};
     a.us = c->d_us[0]; a.vs = c->d_us[1]; a.ws = c->d_us[2]; a.phi = c->d_phi;
     a.u = c->d_u[0]; a.v = c->d_u[1]; a.w = c->d_u[2]; a.p = c->d_p;
     a.shift = c->d_shift;
+    a.part = c->d_part;
+    const int lo[3] = {FEN_LEFT, FEN_BOTTOM, FEN_FRONT}, hi[3] = {FEN_RIGHT, FEN_TOP, FEN_BACK};
+    for (int d = 0; d < 3; d++) {
+        const int slot = FEN_U + d;
+        a.lo_type[d] = c->bc_type[slot][lo[d]];
+        a.hi_type[d] = c->bc_type[slot][hi[d]];
+        a.lo_plane[d] = c->d_plane[slot][lo[d]];
+        a.hi_plane[d] = c->d_plane[slot][hi[d]];
+    }
     a.b = c->box;
     a.id = 1.0 / c->delta;
     a.dt = dt;
     a.rhof = 0.5 * (c->density + c->density);
-    const Box &b = c->box;
-    const dim3 grid((unsigned)((b.nx + kTX - 1) / kTX), (unsigned)((b.ny + kTY - 1) / kTY),
-                    (unsigned)(b.dim == 3 ? b.nz : 1));
+    a.img = c->img_vel & c->img_p;
+    int nbz;
+    grid_blocks(c->box, PTY, PKB, &a.nbx, &a.nby, &nbz);
+    const long nblk = (long)a.nbx * a.nby * nbz;
+    if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
     const bool sh = c->shift_active;
+    const int d3 = c->box.dim == 3;
     prof_begin(c, K_PROJECT);
-#define L_P(D, S) hipLaunchKernelGGL((project_kernel<D, S>), grid, dim3(kTX, kTY), 0, c->stream, a)
-    if (b.dim == 3) { if (sh) L_P(1, 1); else L_P(1, 0); }
-    else { if (sh) L_P(0, 1); else L_P(0, 0); }
-#undef L_P
-    prof_end(c, K_PROJECT, (b.dim == 3 ? 9.0 : 7.0) * 8.0 * c->n_int);
+    if (d3) {
+        const dim3 grid((unsigned)nblk);
+        if (sh) hipLaunchKernelGGL((project_march_kernel<1, PTY, PKB, PUN>), grid, dim3(64 * PTY), 0, c->stream, a);
+        else hipLaunchKernelGGL((project_march_kernel<0, PTY, PKB, PUN>), grid, dim3(64 * PTY), 0, c->stream, a);
+    } else {
+        const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
+        if (sh) hipLaunchKernelGGL((project_kernel_2d<1>), grid, dim3(kThreads), 0, c->stream, a);
+        else hipLaunchKernelGGL((project_kernel_2d<0>), grid, dim3(kThreads), 0, c->stream, a);
+    }
+    prof_end(c, K_PROJECT, (d3 ? 9.0 : 7.0) * 8.0 * c->n_int);
+    prof_begin(c, K_REDUCE);
+    hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(256), 0, c->stream, (const double *)c->d_part, nblk,
+                       checks_out);
+    prof_end(c, K_REDUCE, 24.0 * nblk);
 }
 
 // ---------------------------------------------------------------------------
-// K8: checks (src/navier_stokes.f90:570-619): signed max of div(v) and
-// max(|u|+|v|+|w|) over the interior; flags NaN/Inf.  Max is exact, so the
-// result does not depend on the reduction order.
+// Stand-alone checks on the resident velocity (used before the first step
+// of a CFL-controlled run): same quantities as above, ghosts already filled.
 // ---------------------------------------------------------------------------
 constexpr int kCheckBlocks = 1024;
 
@@ -482,28 +444,6 @@ __global__ __launch_bounds__(256) void checks_kernel(const double *u, const doub
     }
 }
 
-__global__ __launch_bounds__(256) void checks_final_kernel(const double *part, int n, double *out)
-{
-    __shared__ double sd[256], sv[256], sf[256];
-    double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
-    for (int q = threadIdx.x; q < n; q += 256) {
-        if (part[3 * q] > md) md = part[3 * q];
-        if (part[3 * q + 1] > mv) mv = part[3 * q + 1];
-        if (part[3 * q + 2] > bad) bad = part[3 * q + 2];
-    }
-    sd[threadIdx.x] = md; sv[threadIdx.x] = mv; sf[threadIdx.x] = bad;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            if (sd[threadIdx.x + s] > sd[threadIdx.x]) sd[threadIdx.x] = sd[threadIdx.x + s];
-            if (sv[threadIdx.x + s] > sv[threadIdx.x]) sv[threadIdx.x] = sv[threadIdx.x + s];
-            if (sf[threadIdx.x + s] > sf[threadIdx.x]) sf[threadIdx.x] = sf[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { out[0] = sd[0]; out[1] = sv[0]; out[2] = sf[0]; }
-}
-
 void launch_checks(fen_ctx *c, const double *const *vel, double *out)
 {
     const Box &b = c->box;
@@ -520,8 +460,7 @@ void launch_checks(fen_ctx *c, const double *const *vel, double *out)
                            b, id, c->d_part);
     prof_end(c, K_CHECKS, (b.dim == 3 ? 3.0 : 2.0) * 8.0 * c->n_int);
     prof_begin(c, K_REDUCE);
-    hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(256), 0, c->stream, (const double *)c->d_part, (int)nblk,
-                       out);
+    hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(256), 0, c->stream, (const double *)c->d_part, nblk, out);
     prof_end(c, K_REDUCE, 24.0 * nblk);
 }
 

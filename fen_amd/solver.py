@@ -179,8 +179,22 @@ class Solver:
         self._check(self.lib.fen_write_grid_json(self.h, path.encode()))
 
     # -- profiling ---------------------------------------------------------
-    def profile(self, enable=True):
-        self._check(self.lib.fen_profile_enable(self.h, int(enable)))
+    def profile(self, kernels=None):
+        """Time the named kernels (all if None, none if an empty list) with HIP events."""
+        if kernels is None:
+            mask = -1
+        else:
+            names = [self.lib.fen_profile_name(k).decode() for k in range(self.lib.fen_profile_count())]
+            mask = 0
+            for kname in kernels:
+                mask |= 1 << names.index(kname)
+        self._check(self.lib.fen_profile_enable(self.h, mask))
+
+    def profile_predecessor(self, kernel):
+        names = [self.lib.fen_profile_name(k).decode() for k in range(self.lib.fen_profile_count())]
+        k = ctypes.c_int()
+        self._check(self.lib.fen_profile_predecessor(self.h, names.index(kernel), ctypes.byref(k)))
+        return names[k.value] if k.value >= 0 else None
 
     def profile_reset(self):
         self._check(self.lib.fen_profile_reset(self.h))

@@ -159,14 +159,19 @@ int fen_grid_delta(const fen_ctx *ctx, double *delta);
 int fen_synchronize(fen_ctx *ctx);
 
 /* Per-kernel timing with HIP events recorded on the context's stream.
- * While enabled every launch of kernel k is bracketed by an event pair;
- * fen_profile_read returns the summed milliseconds, the launch count and the
- * kernel's algorithmic HBM bytes per launch. */
-int fen_profile_enable(fen_ctx *ctx, int enable);
+ * mask selects kernels by bit (1 << k, k < fen_profile_count()); -1 times
+ * all of them, 0 none.  An event is recorded after every launch of a
+ * selected kernel; a launch's duration is the time from the event of the
+ * launch just before it on the stream (which must be selected too) to its own.
+ * fen_profile_read returns the summed milliseconds and count of the timed
+ * launches and the kernel's algorithmic HBM bytes per launch;
+ * fen_profile_predecessor names the kernel launched just before kernel k. */
+int fen_profile_enable(fen_ctx *ctx, int mask);
 int fen_profile_reset(fen_ctx *ctx);
 int fen_profile_count(void);
 const char *fen_profile_name(int kernel);
 int fen_profile_read(fen_ctx *ctx, int kernel, double *total_ms, long long *launches, double *bytes);
+int fen_profile_predecessor(fen_ctx *ctx, int kernel, int *prev);
 
 #ifdef __cplusplus
 }

@@ -161,3 +161,36 @@ def test_run_to_run_bitwise_256():
     assert np.array_equal(b.download(U), ua)
     md = b.scalars()["maxdiv"]
     assert abs(md) < 1e-10
+
+
+def _checks_numpy(u, v, w, dim, delta):
+    """checks (src/navier_stokes.f90:570-619) evaluated on downloaded ghosted fields."""
+    idl = 1.0 / delta
+    if dim == 2:
+        U, Vv = u[:, :, 1], v[:, :, 1]
+        d = (U[1:-1, 1:-1] - U[:-2, 1:-1]) * idl + (Vv[1:-1, 1:-1] - Vv[1:-1, :-2]) * idl
+        vel = np.abs(U[1:-1, 1:-1]) + np.abs(Vv[1:-1, 1:-1])
+    else:
+        I = (slice(1, -1),) * 3
+        d = (u[I] - u[:-2, 1:-1, 1:-1]) * idl + (v[I] - v[1:-1, :-2, 1:-1]) * idl
+        d = d + (w[I] - w[1:-1, 1:-1, :-2]) * idl
+        vel = np.abs(u[I]) + np.abs(v[I]) + np.abs(w[I])
+    return d.max(), vel.max()
+
+
+@pytest.mark.parametrize("which", ["tgv2d", "tgv3d", "channel"])
+def test_fused_checks_equal_checks_on_final_fields(which):
+    if which == "tgv2d":
+        case = cases.tgv2d(64, steps=3)
+    elif which == "tgv3d":
+        case = cases.tgv3d(32, steps=3)
+    else:
+        case = cases.channel(32, 16, 16, steps=3)
+    g = make_gpu(case)
+    for s in range(1, 4):
+        dt, md, mc = g.navier_stokes_solver(s, case.dt)
+    u, v = g.download(U), g.download(V)
+    w = g.download(W) if case.dim == 3 else None
+    dmax, vmax = _checks_numpy(u, v, w, case.dim, g.delta)
+    assert md == dmax
+    assert mc == dt * vmax / g.delta
