@@ -194,3 +194,20 @@ def test_fused_checks_equal_checks_on_final_fields(which):
     dmax, vmax = _checks_numpy(u, v, w, case.dim, g.delta)
     assert md == dmax
     assert mc == dt * vmax / g.delta
+
+
+@pytest.mark.parametrize("name", ["tgv2d_16_10steps", "tgv3d_16_5steps", "channel_16x8x8_5steps"])
+def test_gpu_matches_golden_fixtures(name):
+    import os
+    from tests.golden.make_golden import FIXTURES
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    make, fields = FIXTURES[name]
+    case = make()
+    g = make_gpu(case)
+    for s in range(1, case.steps + 1):
+        dt, md, mc = g.navier_stokes_solver(s, case.dt)
+    assert abs(mc - float(ref["maxcfl"])) <= 1e-12 * abs(float(ref["maxcfl"]))
+    for f in fields:
+        key = {P: "p", U: "u", V: "v", W: "w"}[f]
+        e = rel_err(live(g.download(f), case.dim), live(ref[key], case.dim))
+        assert e < TOL, (key, e)
