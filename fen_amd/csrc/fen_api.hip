@@ -432,8 +432,10 @@ int fen_get_scalars(fen_ctx *c, double *maxdiv, double *maxcfl, double *dt_o, do
         if (dt_o) *dt_o = c->dt_o;
         if (dt_visc) *dt_visc = c->dt_visc;
         if (dt_conv) *dt_conv = c->dt_conv;
-        if (maxdiv || maxcfl) {
-            if (!c->checks_valid) return fail(c, FEN_ERR_STATE, "no step has run since the last upload");
+        if ((maxdiv || maxcfl) && !c->checks_valid) {   // no step yet: nothing measured
+            if (maxdiv) *maxdiv = 0.0;
+            if (maxcfl) *maxcfl = 0.0;
+        } else if (maxdiv || maxcfl) {
             FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal, c->d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             sync(c);
             if (maxdiv) *maxdiv = c->h_scal[0];

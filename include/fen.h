@@ -97,7 +97,8 @@ int fen_set_dt_o(fen_ctx *ctx, double dt_o);
 
 /* Reads back maxdiv, maxCFL (src/navier_stokes.f90:24), dt_o, dt_visc and
  * dt_conv.  Synchronises.  Returns FEN_ERR_NONFINITE if the last step's
- * checks met a NaN or Inf.  Any pointer may be NULL. */
+ * checks met a NaN or Inf.  maxdiv and maxcfl read 0 until a step has run
+ * after the last velocity upload.  Any pointer may be NULL. */
 int fen_get_scalars(fen_ctx *ctx, double *maxdiv, double *maxcfl, double *dt_o, double *dt_visc,
                     double *dt_conv);
 
