@@ -32,12 +32,15 @@ WORDS_3D = 37           # algorithmic words per cell per step (3-D, ppp)
 
 
 def read_traffic(kernel, workload):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE),
-    as committed in profiles/traffic_<workload>.txt (lines `kernel bytes`)."""
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    calibrated on a copy kernel), as committed in profiles/traffic_<workload>.txt
+    (written by fen_amd.rocprof_summary: lines `kernel bytes ...`)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
+        if line.startswith("#"):
+            continue
         parts = line.split()
         if len(parts) >= 2 and parts[0] == kernel:
             try:
@@ -45,6 +48,26 @@ def read_traffic(kernel, workload):
             except ValueError:
                 return None
     return None
+
+
+def copy_probe(nbytes, reps=3):
+    """A plain copy kernel of known size (the HIP runtime's device-to-device
+    copy kernel): reads nbytes and writes nbytes per launch."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), n) != 0:
+            raise RuntimeError("hipMalloc failed in copy probe")
+    if hip.hipMemset(src, 0, n) != 0:
+        raise RuntimeError("hipMemset failed in copy probe")
+    for _ in range(reps):
+        if hip.hipMemcpy(dst, src, n, 3) != 0:   # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed in copy probe")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
 
 
 def cpu_baseline(N, budget_s):
@@ -80,6 +103,9 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid size N (N^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle steps")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="BYTES",
+                    help="after the run, launch a plain device copy over BYTES bytes three times: "
+                         "the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE calibration")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -102,7 +128,8 @@ def main():
     g.profile_reset()
     g.run(args.warmup, 3, dt)
     prof = g.profile_read()
-    dominant = max(prof, key=lambda k: prof[k]["ms"])
+    # the longest launch (each stencil/FFT kernel runs once per step)
+    dominant = max(prof, key=lambda k: prof[k]["ms"] / prof[k]["launches"])
     breakdown = {k: round(1e3 * v["ms"] / v["launches"], 2) for k, v in prof.items()}   # us per launch
     pred = g.profile_predecessor(dominant)
     g.profile([])
@@ -159,7 +186,9 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    if args.copy_probe > 0:
+        copy_probe(int(args.copy_probe))
 
 
 if __name__ == "__main__":
