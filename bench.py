@@ -12,6 +12,10 @@ advanced one full step per second) with all fields resident in HBM.
 its average duration, timed with HIP events recorded on the library's own
 stream inside the timed region; `step_roofline` applies the same
 algorithmic model (37 words per cell per step) to the whole step.
+`fp32_refine` repeats the measurement in the FP32 form (FP32 fields,
+stencils and FFTs; FP64 divergence residual, projection update and
+elimination) and reports its speed-up and its relative L2 errors in u, v,
+w, p and max|div u| against the FP64 run after the same steps.
 `cpu_baseline` times the oracle (this repository's C restatement of the
 reference algorithm, OpenMP) on the same workload for a bounded number of
 steps on this machine's host cores.
@@ -95,38 +99,22 @@ def cpu_baseline(N, budget_s):
             "sample": f"oracle (C, OpenMP {cores} threads) on tgv3d {N}^3 FP64: 1 warm-up + {nsteps} timed steps"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=256, help="grid size N (N^3)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle steps")
-    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="BYTES",
-                    help="after the run, launch a plain device copy over BYTES bytes three times: "
-                         "the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE calibration")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
-        sys.exit(2)
-
+def measure(N, steps, warmup, precision):
+    """Warm-up, one untimed calibration of the per-kernel breakdown, then the
+    timed steps.  Returns (solver, result dict)."""
     from fen_amd import cases
     from tests.common import make_gpu
 
-    N = args.n
-    case = cases.tgv3d(N, steps=args.steps)
-    g = make_gpu(case)
+    case = cases.tgv3d(N, steps=steps)
+    g = make_gpu(case, precision=precision)
     dt = case.dt
 
     # warm-up first: the first launch of each kernel pays code-object loading
-    g.run(0, args.warmup, dt)
+    g.run(0, warmup, dt)
     # calibration (untimed): which kernel dominates the step
     g.profile(None)
     g.profile_reset()
-    g.run(args.warmup, 3, dt)
+    g.run(warmup, 3, dt)
     prof = g.profile_read()
     # the longest launch (each stencil/FFT kernel runs once per step)
     dominant = max(prof, key=lambda k: prof[k]["ms"] / prof[k]["launches"])
@@ -134,27 +122,114 @@ def main():
     pred = g.profile_predecessor(dominant)
     g.profile([])
 
-    step0 = 3 + args.warmup
+    step0 = 3 + warmup
     # timed region: one event after the dominant kernel and one after the
     # launch preceding it, so its duration is measured on the library's stream
     g.profile([dominant] + ([pred] if pred and pred != dominant else []))
     g.profile_reset()
     g.synchronize()
     t0 = time.perf_counter()
-    g.run(step0, args.steps, dt)
+    g.run(step0, steps, dt)
     g.synchronize()
     t1 = time.perf_counter()
     kp = g.profile_read()[dominant]
     g.profile([])
     sc = g.scalars()
 
-    ms = 1e3 * (t1 - t0) / args.steps
-    value = N ** 3 / (ms * 1e-3)
+    ms = 1e3 * (t1 - t0) / steps
     avg_ms = kp["ms"] / kp["launches"]
-    achieved = kp["bytes"] / (avg_ms * 1e-3) / 1e9
+    return g, {"ms": ms, "value": N ** 3 / (ms * 1e-3), "dominant": dominant, "avg_ms": avg_ms,
+               "bytes": kp["bytes"], "achieved": kp["bytes"] / (avg_ms * 1e-3) / 1e9, "breakdown": breakdown,
+               "maxdiv": sc["maxdiv"], "dt": dt, "steps_total": step0 + steps}
+
+
+def field_errors(g32, g64, dim=3):
+    """Relative L2 errors (interior) of the FP32 form against FP64, per
+    component and for the velocity field as one vector."""
+    import numpy as np
+    from fen_amd._lib import P, U, V, W
+    out, num, den = {}, 0.0, 0.0
+    for name, f in (("u", U), ("v", V), ("w", W), ("p", P)):
+        a = g32.download(f)[1:-1, 1:-1, 1:-1]
+        b = g64.download(f)[1:-1, 1:-1, 1:-1]
+        e2, b2 = float(np.linalg.norm((a - b).ravel())) ** 2, float(np.linalg.norm(b.ravel())) ** 2
+        out[name] = (e2 / b2) ** 0.5
+        if f != P:
+            num, den = num + e2, den + b2
+        del a, b
+    out["velocity"] = (num / den) ** 0.5
+    return out
+
+
+def fp32_accuracy(N, steps):
+    """Both forms over `steps` steps of the Taylor-Green case at N^3 from the
+    same seeded state; errors of the FP32 form against FP64."""
+    from fen_amd import cases
+    from fen_amd._lib import FP32_REFINE
+    from tests.common import make_gpu
+
+    case = cases.tgv3d(N, steps=steps)
+    res = {}
+    for prec in (0, FP32_REFINE):
+        g = make_gpu(case, precision=prec)
+        g.run(0, steps - 1, case.dt)
+        _, md, _ = g.navier_stokes_solver(steps, case.dt)
+        res[prec] = (g, md)
+    errs = field_errors(res[FP32_REFINE][0], res[0][0])
+    return {"workload": f"3-D Taylor-Green vortex {N}^3, Re 1600, {steps} steps", "grid": [N, N, N],
+            "steps": steps, "rel_l2_error_vs_fp64": errs, "maxdiv_fp64": res[0][1],
+            "maxdiv_fp32_refine": res[FP32_REFINE][1]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="grid size N (N^3)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+refine form")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle steps")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="BYTES",
+                    help="after the run, launch a plain device copy over BYTES bytes three times: "
+                         "the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE calibration")
+    ap.add_argument("--fp32-accuracy", type=int, nargs=2, metavar=("N", "STEPS"),
+                    help="only report the FP32+refine errors against FP64 at N^3 after STEPS steps")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    if args.fp32_accuracy:
+        print(json.dumps(fp32_accuracy(*args.fp32_accuracy)), flush=True)
+        return
+
+    from fen_amd._lib import FP32_REFINE
+
+    N = args.n
+    g64, r64 = measure(N, args.steps, args.warmup, 0)
     workload = f"tgv3d_{N}_fp64"
-    traffic = read_traffic(dominant, workload)
-    step_gbs = WORDS_3D * 8.0 * N ** 3 / (ms * 1e-3) / 1e9
+    traffic = read_traffic(r64["dominant"], workload)
+    step_gbs = WORDS_3D * 8.0 * N ** 3 / (r64["ms"] * 1e-3) / 1e9
+
+    fp32 = None
+    if not args.no_fp32:
+        g32, r32 = measure(N, args.steps, args.warmup, FP32_REFINE)
+        step32 = WORDS_3D * 4.0 * N ** 3 / (r32["ms"] * 1e-3) / 1e9
+        fp32 = {"value": r32["value"], "unit": "cell-updates/s", "ms_per_step": r32["ms"], "dtype": "f32+f64",
+                "speedup_vs_fp64": r64["ms"] / r32["ms"],
+                "roofline": {"bound": "hbm", "kernel": r32["dominant"], "achieved": r32["achieved"],
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r32["achieved"] / HBM_PEAK_GBS,
+                             "traffic": read_traffic(r32["dominant"], f"tgv3d_{N}_fp32"),
+                             "bytes_per_launch": r32["bytes"], "avg_launch_ms": r32["avg_ms"]},
+                "step_roofline": {"words_per_cell": WORDS_3D, "bytes_per_word": 4, "achieved": step32,
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step32 / HBM_PEAK_GBS},
+                "kernel_us_per_launch": r32["breakdown"],
+                "error_vs_fp64": dict(steps=r32["steps_total"], rel_l2=field_errors(g32, g64),
+                                      maxdiv_fp64=r64["maxdiv"], maxdiv_fp32_refine=r32["maxdiv"])}
+        g32.close()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -162,30 +237,32 @@ def main():
 
     out = {
         "metric": "NS cell-updates/s at 256^3, FP64 vs FP32+refine; % HBM roofline",
-        "value": value,
+        "value": r64["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": r64["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"3-D Taylor-Green vortex {N}^3 triply periodic, Re 1600, FP64",
-                   "grid": [N, N, N], "solver": "ppp", "dt": dt, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": kp["bytes"], "avg_launch_ms": avg_ms},
+                   "grid": [N, N, N], "solver": "ppp", "dt": r64["dt"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": r64["dominant"], "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": r64["bytes"], "avg_launch_ms": r64["avg_ms"]},
         "step_roofline": {"words_per_cell": WORDS_3D, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
-        "kernel_us_per_launch": breakdown,
-        "maxdiv": sc["maxdiv"],
+        "kernel_us_per_launch": r64["breakdown"],
+        "maxdiv": r64["maxdiv"],
+        "fp32_refine": fp32,
         "cpu_baseline": cpu,
     }
     if cpu:
-        out["gpu_over_cpu"] = value / cpu["value"]
+        out["gpu_over_cpu"] = r64["value"] / cpu["value"]
+    g64.close()
     print(json.dumps(out), flush=True)
     if args.copy_probe > 0:
         copy_probe(int(args.copy_probe))

@@ -19,6 +19,7 @@
 
 using fen::Box;
 using fen::GhostDesc;
+using fen::as;
 
 namespace fen {
 
@@ -100,15 +101,61 @@ std::vector<double> modified_wavenumbers(int n, double delta, double factor)
     return m;
 }
 
-std::vector<double2> twiddles(int n)
+template <class R> std::vector<typename fen::Cplx<R>::type> twiddles(int n)
 {
-    std::vector<double2> w(n);
+    std::vector<typename fen::Cplx<R>::type> w(n);
     const long double tpi = 2.0L * std::acos(-1.0L);
     for (int m = 0; m < n; m++) {
         const long double a = tpi * (long double)m / (long double)n;
-        w[m] = double2{(double)std::cos(a), (double)(-std::sin(a))};
+        w[m] = {(R)std::cos(a), (R)(-std::sin(a))};
     }
     return w;
+}
+
+// Element size of the context's fields.
+size_t elem(const fen_ctx *c) { return c->precision == FEN_FP64 ? sizeof(double) : sizeof(float); }
+
+void *dev_alloc_bytes(size_t bytes)
+{
+    void *p = nullptr;
+    FEN_HIP_CHECK(hipMalloc(&p, bytes));
+    FEN_HIP_CHECK(hipMemset(p, 0, bytes));
+    return p;
+}
+
+// Calls f(R{}) with R the context's field element type.
+template <class F> void with_type(const fen_ctx *c, F f)
+{
+    if (c->precision == FEN_FP64) f(double{});
+    else f(float{});
+}
+
+// Host FP64 <-> device R copies (FP32: converted through a host buffer, so
+// every value is rounded to nearest once).
+void copy_in(fen_ctx *c, void *dst, const double *src, size_t n)
+{
+    if (c->precision == FEN_FP64) {
+        FEN_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        FEN_HIP_CHECK(hipStreamSynchronize(c->stream));
+        return;
+    }
+    std::vector<float> h(n);
+    for (size_t q = 0; q < n; q++) h[q] = (float)src[q];
+    FEN_HIP_CHECK(hipMemcpyAsync(dst, h.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    FEN_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void copy_out(fen_ctx *c, double *dst, const void *src, size_t n)
+{
+    if (c->precision == FEN_FP64) {
+        FEN_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        FEN_HIP_CHECK(hipStreamSynchronize(c->stream));
+        return;
+    }
+    std::vector<float> h(n);
+    FEN_HIP_CHECK(hipMemcpyAsync(h.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    FEN_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < n; q++) dst[q] = (double)h[q];
 }
 
 template <typename T> T *dev_alloc(size_t n)
@@ -134,14 +181,14 @@ size_t plane_count(const fen_ctx *c, int face)
     return gx * gy;
 }
 
-GhostDesc make_desc(fen_ctx *c, int field, double *arr)
+template <class R> GhostDesc<R> make_desc(fen_ctx *c, int field, void *arr)
 {
-    GhostDesc d{};
-    d.f = arr;
+    GhostDesc<R> d{};
+    d.f = as<R>(arr);
     const int s = field_slot(field);
     for (int f = 0; f < 6; f++) {
         d.type[f] = c->bc_type[s][f];
-        d.plane[f] = c->d_plane[s][f];
+        d.plane[f] = as<R>(c->d_plane[s][f]);
     }
     d.loc = field_loc(field);
     d.groups = 7;
@@ -159,7 +206,7 @@ void refresh_images(fen_ctx *c)
     c->img_phi = fen::image_axes(c->bc_type[FEN_PHI], c->dim);
 }
 
-double *field_ptr(fen_ctx *c, int field)
+void *field_ptr(fen_ctx *c, int field)
 {
     switch (field) {
     case FEN_P: return c->d_p;
@@ -177,7 +224,7 @@ void free_all(fen_ctx *c)
 {
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     for (int d = 0; d < 3; d++) { f(c->d_u[d]); f(c->d_us[d]); f(c->d_dvo[d]); f(c->d_S[d]); }
-    f(c->d_p); f(c->d_phi); f(c->d_spec); f(c->d_c1);
+    f(c->d_p); f(c->d_phi); f(c->d_spec); f(c->d_c1); f(c->d_dd);
     f(c->d_mwx); f(c->d_mwy); f(c->d_mwz); f(c->d_ta); f(c->d_tb); f(c->d_tc);
     f(c->d_twx); f(c->d_twy); f(c->d_twz); f(c->d_part); f(c->d_scal);
     for (int s = 0; s < 5; s++) for (int p = 0; p < 6; p++) f(c->d_plane[s][p]);
@@ -214,7 +261,7 @@ std::string fortran_e(double x, int w, int d)
 }
 
 // One time step; see the kernel sequence at the top of this file.
-void step_fp64(fen_ctx *c, double dt)
+template <class R> void advance(fen_ctx *c, double dt)
 {
     using namespace fen;
     const bool d3 = c->dim == 3;
@@ -222,29 +269,29 @@ void step_fp64(fen_ctx *c, double dt)
     const double B = -(0.5 * dt / c->dt_o);
     c->last_dt = dt;
 
-    launch_momentum(c, dt, A, B);
+    launch_momentum<R>(c, dt, A, B);
     if (const int rest = remaining_groups(c, c->img_vel)) {
-        GhostDesc g[3] = {make_desc(c, FEN_U, c->d_us[0]), make_desc(c, FEN_V, c->d_us[1]),
-                          make_desc(c, FEN_W, c->d_us[2])};
+        GhostDesc<R> g[3] = {make_desc<R>(c, FEN_U, c->d_us[0]), make_desc<R>(c, FEN_V, c->d_us[1]),
+                             make_desc<R>(c, FEN_W, c->d_us[2])};
         for (auto &d : g) d.groups = rest;
-        launch_ghost_desc(c, g, d3 ? 3 : 2);
+        launch_ghost_desc<R>(c, g, d3 ? 3 : 2);
     }
-    const double *vs[3] = {c->d_us[0], c->d_us[1], c->d_us[2]};
+    const R *vs[3] = {as<R>(c->d_us[0]), as<R>(c->d_us[1]), as<R>(c->d_us[2])};
     const double post_x = c->solver == FEN_SOLVER_PPN ? 1.0 / (double)c->nx : 1.0;
-    launch_div_r2c_impl(c, vs, nullptr, 0, 0, 0, c->density, dt, post_x);
+    launch_div_r2c_impl<R>(c, vs, nullptr, 0, 0, 0, c->density, dt, post_x);
     switch (c->solver) {
     case FEN_SOLVER_PPP:
-        launch_y_fft(c, -1, 1.0);
-        launch_z_solve(c, 1.0 / (double)(float)((long)c->nx * c->ny * c->nz));
-        launch_y_fft(c, +1, 1.0);
+        launch_y_fft<R>(c, -1, 1.0);
+        launch_z_solve<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny * c->nz));
+        launch_y_fft<R>(c, +1, 1.0);
         break;
     case FEN_SOLVER_PPN:
-        launch_y_fft(c, -1, 1.0 / (double)c->ny);
-        launch_thomas(c);
-        launch_y_fft(c, +1, 1.0);
+        launch_y_fft<R>(c, -1, 1.0 / (double)c->ny);
+        launch_thomas<R>(c);
+        launch_y_fft<R>(c, +1, 1.0);
         break;
     case FEN_SOLVER_PP:
-        launch_y_solve_2d(c, 1.0 / (double)(float)((long)c->nx * c->ny));
+        launch_y_solve_2d<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny));
         break;
     default: throw std::runtime_error("solver not available");
     }
@@ -253,24 +300,24 @@ void step_fp64(fen_ctx *c, double dt)
     for (int f = 0; f < 6; f++) dirichlet = dirichlet || c->bc_type[FEN_PHI][f] == 1;
     // images written by the c2r only when no shift has to be applied first
     const int img_phi = (mean && dirichlet) ? 0 : c->img_phi;
-    launch_c2r(c, c->d_phi, 1, c->box.gx, c->box.gxy, mean, img_phi);
+    launch_c2r<R>(c, as<R>(c->d_phi), 1, c->box.gx, c->box.gxy, mean, img_phi);
     c->shift_active = mean;
     if (mean && dirichlet) {   // the shift does not commute with 2*bc - f: apply it first
-        launch_apply_shift(c, c->d_phi, c->n_ghost);
+        launch_apply_shift<R>(c, as<R>(c->d_phi), c->n_ghost);
         c->shift_active = false;
     }
     if (const int rest = remaining_groups(c, img_phi)) {
-        GhostDesc g = make_desc(c, FEN_PHI, c->d_phi);
+        GhostDesc<R> g = make_desc<R>(c, FEN_PHI, c->d_phi);
         g.groups = rest;
-        launch_ghost_desc(c, &g, 1);
+        launch_ghost_desc<R>(c, &g, 1);
     }
-    launch_project(c, dt, c->d_scal);
+    launch_project<R>(c, dt, c->d_scal);
     if (const int rest = remaining_groups(c, c->img_vel & c->img_p)) {
-        GhostDesc g[4] = {make_desc(c, FEN_U, c->d_u[0]), make_desc(c, FEN_V, c->d_u[1]),
-                          d3 ? make_desc(c, FEN_W, c->d_u[2]) : make_desc(c, FEN_P, c->d_p),
-                          make_desc(c, FEN_P, c->d_p)};
+        GhostDesc<R> g[4] = {make_desc<R>(c, FEN_U, c->d_u[0]), make_desc<R>(c, FEN_V, c->d_u[1]),
+                             d3 ? make_desc<R>(c, FEN_W, c->d_u[2]) : make_desc<R>(c, FEN_P, c->d_p),
+                             make_desc<R>(c, FEN_P, c->d_p)};
         for (auto &d : g) d.groups = rest;
-        launch_ghost_desc(c, g, d3 ? 4 : 3);
+        launch_ghost_desc<R>(c, g, d3 ? 4 : 3);
     }
     c->checks_valid = true;
 }
@@ -294,7 +341,7 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
     if (dim != 2 && dim != 3) return FEN_ERR_ARG;
     if (!bc) return FEN_ERR_ARG;
     if (dim == 2) nz = 1;
-    if (precision != FEN_FP64) return FEN_ERR_UNSUPPORTED;
+    if (precision != FEN_FP64 && precision != FEN_FP32_REFINE) return FEN_ERR_UNSUPPORTED;
     fen_ctx *c = new fen_ctx();
     c->dim = dim; c->nx = nx; c->ny = ny; c->nz = nz;
     c->lx = lx; c->ly = ly; c->lz = lz;
@@ -335,18 +382,22 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
     c->hx = nx / 2 + 1;
     try {
         FEN_HIP_CHECK(hipStreamCreate(&c->stream));
+        const size_t es = elem(c);
         for (int d = 0; d < 3; d++) {
-            c->d_u[d] = dev_alloc<double>(c->n_ghost);
-            c->d_us[d] = dev_alloc<double>(c->n_ghost);
-            c->d_dvo[d] = dev_alloc<double>(c->n_int);
+            c->d_u[d] = dev_alloc_bytes(c->n_ghost * es);
+            c->d_us[d] = dev_alloc_bytes(c->n_ghost * es);
+            c->d_dvo[d] = dev_alloc_bytes(c->n_int * es);
         }
-        c->d_p = dev_alloc<double>(c->n_ghost);
-        c->d_phi = dev_alloc<double>(c->n_ghost);
+        c->d_p = dev_alloc_bytes(c->n_ghost * es);
+        c->d_phi = dev_alloc_bytes(c->n_ghost * es);
         for (int s = 0; s < 5; s++)
-            for (int f = 0; f < 6; f++) c->d_plane[s][f] = dev_alloc<double>(plane_count(c, f));
+            for (int f = 0; f < 6; f++) c->d_plane[s][f] = dev_alloc_bytes(plane_count(c, f) * es);
         const size_t nspec = (size_t)c->hx * ny * nz;
-        c->d_spec = dev_alloc<double2>(nspec);
-        if (c->solver == FEN_SOLVER_PPN) c->d_c1 = dev_alloc<double>(nspec);
+        c->d_spec = dev_alloc_bytes(nspec * 2 * es);
+        if (c->solver == FEN_SOLVER_PPN) {
+            c->d_c1 = dev_alloc<double>(nspec);
+            if (precision != FEN_FP64) c->d_dd = dev_alloc<double2>(nspec);
+        }
         c->d_mwx = dev_copy(modified_wavenumbers(nx, c->delta, 2.0));
         c->d_mwy = dev_copy(modified_wavenumbers(ny, c->delta, 2.0));
         if (c->solver == FEN_SOLVER_PPP) c->d_mwz = dev_copy(modified_wavenumbers(nz, c->delta, 2.0));
@@ -361,9 +412,12 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
             cc[nz - 1] = 0.0;
             c->d_ta = dev_copy(a); c->d_tb = dev_copy(bb); c->d_tc = dev_copy(cc);
         }
-        c->d_twx = dev_copy(twiddles(nx));
-        c->d_twy = dev_copy(twiddles(ny));
-        if (c->solver == FEN_SOLVER_PPP) c->d_twz = dev_copy(twiddles(nz));
+        with_type(c, [&](auto tag) {
+            using R = decltype(tag);
+            c->d_twx = dev_copy(twiddles<R>(nx));
+            c->d_twy = dev_copy(twiddles<R>(ny));
+            if (c->solver == FEN_SOLVER_PPP) c->d_twz = dev_copy(twiddles<R>(nz));
+        });
         c->n_part = std::max<size_t>(std::max<size_t>(3 * 1024, (size_t)ny * nz / 2 + 1),
                                      3 * (size_t)fen::stencil_blocks(c->box));
         c->d_part = dev_alloc<double>(c->n_part);
@@ -463,12 +517,10 @@ int fen_upload(fen_ctx *c, int field, const double *host, size_t count)
     if (field == FEN_W && c->dim == 2) return fail(c, FEN_ERR_ARG, "no w component in 2-D");
     FEN_TRY(c, {
         if (field >= FEN_S_X && !c->d_S[0]) {
-            for (int d = 0; d < 3; d++) c->d_S[d] = dev_alloc<double>(c->n_int);
+            for (int d = 0; d < 3; d++) c->d_S[d] = dev_alloc_bytes(c->n_int * elem(c));
         }
         if (field >= FEN_S_X) c->has_source = true;
-        double *dst = field_ptr(c, field);
-        FEN_HIP_CHECK(hipMemcpyAsync(dst, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        sync(c);
+        copy_in(c, field_ptr(c, field), host, n);
         if (field == FEN_PHI) c->shift_active = false;
         if (field >= FEN_U && field <= FEN_W) c->checks_valid = false;
     });
@@ -482,18 +534,21 @@ int fen_download(fen_ctx *c, int field, double *host, size_t count)
     if (fen_field_count(c, field, &n) != FEN_OK || n != count)
         return fail(c, FEN_ERR_ARG, "field id or element count does not match the grid");
     FEN_TRY(c, {
-        const double *src = field_ptr(c, field);
+        const void *src = field_ptr(c, field);
         if (!src) {   // source never set: zero, as allocated by the reference
             std::memset(host, 0, n * sizeof(double));
             return FEN_OK;
         }
-        FEN_HIP_CHECK(hipMemcpyAsync(host, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        if (field == FEN_PHI && c->shift_active)
-            FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal + 3, c->d_shift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        sync(c);
+        copy_out(c, host, src, n);
         if (field == FEN_PHI && c->shift_active) {
+            FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal + 3, c->d_shift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            sync(c);
             const double s = c->h_scal[3];
-            for (size_t q = 0; q < n; q++) host[q] = host[q] - s;
+            if (c->precision == FEN_FP64) {
+                for (size_t q = 0; q < n; q++) host[q] = host[q] - s;
+            } else {   // as the device applies it: FP64 difference rounded to R
+                for (size_t q = 0; q < n; q++) host[q] = (double)(float)(host[q] - s);
+            }
         }
     });
     return FEN_OK;
@@ -505,11 +560,7 @@ int fen_set_bc_plane(fen_ctx *c, int field, int face, const double *plane, size_
     const int s = field_slot(field);
     if (s < 0 || face < 0 || face > 5) return fail(c, FEN_ERR_ARG, "bad field or face");
     if (count != plane_count(c, face)) return fail(c, FEN_ERR_ARG, "plane size does not match the face");
-    FEN_TRY(c, {
-        FEN_HIP_CHECK(hipMemcpyAsync(c->d_plane[s][face], plane, count * sizeof(double), hipMemcpyHostToDevice,
-                                     c->stream));
-        sync(c);
-    });
+    FEN_TRY(c, copy_in(c, c->d_plane[s][face], plane, count));
     return FEN_OK;
 }
 
@@ -538,12 +589,15 @@ int fen_update_ghost_nodes(fen_ctx *c, int field)
     if (field_slot(field) < 0) return fail(c, FEN_ERR_ARG, "field has no ghost nodes");
     if (field == FEN_W && c->dim == 2) return fail(c, FEN_ERR_ARG, "no w component in 2-D");
     FEN_TRY(c, {
-        if (field == FEN_PHI && c->shift_active) {
-            fen::launch_apply_shift(c, c->d_phi, c->n_ghost);
-            c->shift_active = false;
-        }
-        GhostDesc g = make_desc(c, field, field_ptr(c, field));
-        fen::launch_ghost_desc(c, &g, 1);
+        with_type(c, [&](auto tag) {
+            using R = decltype(tag);
+            if (field == FEN_PHI && c->shift_active) {
+                fen::launch_apply_shift<R>(c, as<R>(c->d_phi), c->n_ghost);
+                c->shift_active = false;
+            }
+            GhostDesc<R> g = make_desc<R>(c, field, field_ptr(c, field));
+            fen::launch_ghost_desc<R>(c, &g, 1);
+        });
         FEN_HIP_CHECK(hipGetLastError());
     });
     return FEN_OK;
@@ -559,8 +613,11 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
             // update_timestep (src/navier_stokes.f90:670-730)
             if (!c->have_dt_visc) return fail(c, FEN_ERR_STATE, "constant CFL needs fen_set_timestep first");
             if (!c->checks_valid) {
-                const double *vn[3] = {c->d_u[0], c->d_u[1], c->d_u[2]};
-                fen::launch_checks(c, vn, c->d_scal);
+                with_type(c, [&](auto tag) {
+                    using R = decltype(tag);
+                    const R *vn[3] = {as<R>(c->d_u[0]), as<R>(c->d_u[1]), as<R>(c->d_u[2])};
+                    fen::launch_checks<R>(c, vn, c->d_scal);
+                });
             }
             FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal, c->d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             sync(c);
@@ -570,7 +627,8 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
             *dt = c->dt_conv < c->dt_visc ? c->dt_conv : c->dt_visc;
             if (*dt > 1.1 * c->dt_o) *dt = 1.1 * c->dt_o;
         }
-        step_fp64(c, *dt);
+        const double dtv = *dt;
+        with_type(c, [&](auto tag) { advance<decltype(tag)>(c, dtv); });
         FEN_HIP_CHECK(hipGetLastError());
     });
     if (maxdiv || maxcfl) return fen_get_scalars(c, maxdiv, maxcfl, nullptr, nullptr, nullptr);
@@ -594,33 +652,37 @@ int fen_poisson_solve(fen_ctx *c, double *f, int gl)
     const long sx = gl ? c->nx + 2 : c->nx;
     const long sy = gl ? (long)(c->nx + 2) * (c->ny + 2) : (long)c->nx * c->ny;
     FEN_TRY(c, {
-        double *buf = c->d_us[0];   // scratch between steps
-        FEN_HIP_CHECK(hipMemcpyAsync(buf, f, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        void *buf = c->d_us[0];   // scratch between steps
+        copy_in(c, buf, f, n);
         const double post_x = c->solver == FEN_SOLVER_PPN ? 1.0 / (double)c->nx : 1.0;
-        fen::launch_div_r2c_impl(c, nullptr, buf, sx, sy, gl, 1.0, 1.0, post_x);
-        switch (c->solver) {
-        case FEN_SOLVER_PPP:
-            fen::launch_y_fft(c, -1, 1.0);
-            fen::launch_z_solve(c, 1.0 / (double)(float)((long)c->nx * c->ny * c->nz));
-            fen::launch_y_fft(c, +1, 1.0);
-            break;
-        case FEN_SOLVER_PPN:
-            fen::launch_y_fft(c, -1, 1.0 / (double)c->ny);
-            fen::launch_thomas(c);
-            fen::launch_y_fft(c, +1, 1.0);
-            break;
-        case FEN_SOLVER_PP:
-            fen::launch_y_solve_2d(c, 1.0 / (double)(float)((long)c->nx * c->ny));
-            break;
-        default: return fail(c, FEN_ERR_UNSUPPORTED, "solver not available");
-        }
         const bool mean = c->solver != FEN_SOLVER_PPP;
-        fen::launch_c2r(c, buf, gl, sx, sy, mean, 0);
+        with_type(c, [&](auto tag) {
+            using R = decltype(tag);
+            fen::launch_div_r2c_impl<R>(c, nullptr, as<R>(buf), sx, sy, gl, 1.0, 1.0, post_x);
+            switch (c->solver) {
+            case FEN_SOLVER_PPP:
+                fen::launch_y_fft<R>(c, -1, 1.0);
+                fen::launch_z_solve<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny * c->nz));
+                fen::launch_y_fft<R>(c, +1, 1.0);
+                break;
+            case FEN_SOLVER_PPN:
+                fen::launch_y_fft<R>(c, -1, 1.0 / (double)c->ny);
+                fen::launch_thomas<R>(c);
+                fen::launch_y_fft<R>(c, +1, 1.0);
+                break;
+            case FEN_SOLVER_PP:
+                fen::launch_y_solve_2d<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny));
+                break;
+            default: throw std::runtime_error("solver not available");
+            }
+            fen::launch_c2r<R>(c, as<R>(buf), gl, sx, sy, mean, 0);
+        });
         std::vector<double> host(n);
-        FEN_HIP_CHECK(hipMemcpyAsync(host.data(), buf, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        if (mean)
+        copy_out(c, host.data(), buf, n);
+        if (mean) {
             FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal + 3, c->d_shift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        sync(c);
+            sync(c);
+        }
         if (c->solver == FEN_SOLVER_PPN && gl) {
             // phi%f = 0 over the whole array before the c2r (src/poisson.f90:1151)
             const size_t gx = c->nx + 2, gy = c->ny + 2;
@@ -643,8 +705,9 @@ int fen_poisson_solve(fen_ctx *c, double *f, int gl)
 int fen_add_advection(fen_ctx *c, double *rx, double *ry, double *rz)
 {
     if (!c || !rx || !ry || (c->dim == 3 && !rz)) return FEN_ERR_ARG;
+    if (c->precision != FEN_FP64) return fail(c, FEN_ERR_UNSUPPORTED, "add_advection is provided for FP64 contexts");
     FEN_TRY(c, {
-        double *r[3] = {c->d_us[0], c->d_us[1], c->d_us[2]};
+        double *r[3] = {as<double>(c->d_us[0]), as<double>(c->d_us[1]), as<double>(c->d_us[2])};
         double *h[3] = {rx, ry, rz};
         const int nd = c->dim == 3 ? 3 : 2;
         for (int d = 0; d < nd; d++)

@@ -1,12 +1,14 @@
 // fen_internal.hpp -- shared definitions of libfen's host code and kernels.
 //
-// Data layout in HBM (all FP64 unless stated):
+// Data layout in HBM.  Element type R is double (FEN_FP64) or float
+// (FEN_FP32_REFINE) for every field and the spectrum; eigenvalue tables,
+// tridiagonal coefficients, the Thomas scratch and reductions are FP64.
 //  * ghosted fields (p, phi, u, v, w and the predicted velocity u*, v*, w*):
 //    FEN's own Fortran layout, (nx+2)*(ny+2)*(nz+2), i fastest, so host
 //    uploads/downloads are plain copies.  In 2-D the k extent is 0:2 and only
 //    the k = 1 plane is live.
 //  * interior-only fields (dv_o, S): nx*ny*nz, i fastest.
-//  * half spectrum of the Poisson solve: complex (double2), laid out
+//  * half spectrum of the Poisson solve: complex (double2 / float2), laid out
 //    [kx][k][j] with j fastest, kx in 0..nx/2.  Every pass of the solver
 //    then reads and writes runs of >= 128 contiguous bytes: x-lines are
 //    transformed in LDS and scattered as j-runs, y-lines are contiguous,
@@ -24,6 +26,11 @@
 
 namespace fen {
 
+// Complex element of the spectrum for a real element type.
+template <typename T> struct Cplx;
+template <> struct Cplx<double> { using type = double2; };
+template <> struct Cplx<float> { using type = float2; };
+
 // Grid extents as seen by kernels.
 struct Box {
     int nx, ny, nz, dim;
@@ -35,9 +42,9 @@ struct Box {
 };
 
 // Ghost-fill description of one field (scalar%bc, src/scalar.f90:23-47).
-struct GhostDesc {
-    double *f;
-    const double *plane[6];
+template <class R> struct GhostDesc {
+    R *f;
+    const R *plane[6];
     int type[6];
     int loc;      // 'c', 'x', 'y', 'z'
     int groups;   // ghost groups to fill: bit 0 x faces, bit 1 y faces, bit 2 z faces
@@ -59,7 +66,8 @@ __host__ __device__ inline int image_axes(const int *type, int dim)
 // Writes v to every periodic image of interior cell (i,j,k) along the axes
 // in `ax` (bit 0 x, 1 y, 2 z): the ghost cells whose final value is a copy
 // of this cell (src/scalar.f90:254-388, periodic branches).
-__device__ __forceinline__ void write_images(double *f, const Box &b, int ax, int i, int j, int k, double v)
+template <class R>
+__device__ __forceinline__ void write_images(R *f, const Box &b, int ax, int i, int j, int k, R v)
 {
     const int ii = (ax & 1) ? (i == 1 ? b.nx + 1 : (i == b.nx ? 0 : -1)) : -1;
     const int jj = (ax & 2) ? (j == 1 ? b.ny + 1 : (j == b.ny ? 0 : -1)) : -1;
@@ -133,25 +141,26 @@ struct fen_ctx {
     fen::Box box{};
     size_t n_ghost = 0, n_int = 0;
 
-    // bc types and device bc planes of p, phi, u, v, w
+    // bc types and device bc planes (element type R) of p, phi, u, v, w
     int bc_type[5][6] = {};
-    double *d_plane[5][6] = {};
+    void *d_plane[5][6] = {};
 
-    // resident fields
-    double *d_u[3] = {nullptr, nullptr, nullptr};    // velocity (state)
-    double *d_us[3] = {nullptr, nullptr, nullptr};   // predicted velocity u*
-    double *d_p = nullptr, *d_phi = nullptr;
-    double *d_dvo[3] = {nullptr, nullptr, nullptr};
-    double *d_S[3] = {nullptr, nullptr, nullptr};
+    // resident fields, element type R (fen::as<R>(...) at the use site)
+    void *d_u[3] = {nullptr, nullptr, nullptr};    // velocity (state)
+    void *d_us[3] = {nullptr, nullptr, nullptr};   // predicted velocity u*
+    void *d_p = nullptr, *d_phi = nullptr;
+    void *d_dvo[3] = {nullptr, nullptr, nullptr};
+    void *d_S[3] = {nullptr, nullptr, nullptr};
     bool has_source = false;
 
     // Poisson solver state
     int hx = 0;                  // nx/2 + 1
-    double2 *d_spec = nullptr;   // [kx][k][j]
+    void *d_spec = nullptr;      // complex R, [kx][k][j]
     double *d_c1 = nullptr;      // Thomas scratch (ppn), [kx][k][j]
+    double2 *d_dd = nullptr;     // FP64 Thomas right-hand side (ppn, FP32 form), [kx][k][j]
     double *d_mwx = nullptr, *d_mwy = nullptr, *d_mwz = nullptr;
     double *d_ta = nullptr, *d_tb = nullptr, *d_tc = nullptr;
-    double2 *d_twx = nullptr, *d_twy = nullptr, *d_twz = nullptr;   // W_n^m, m < n
+    void *d_twx = nullptr, *d_twy = nullptr, *d_twz = nullptr;   // complex R, W_n^m, m < n
     double phi_shift = 0.0;      // mean/N removed lazily (pp, ppn)
     double *d_shift = nullptr;   // device copy of the subtracted mean
     bool shift_active = false;
@@ -179,22 +188,26 @@ struct fen_ctx {
 
 namespace fen {
 
-// Launchers (defined in the kernel translation units).  All enqueue on ctx->stream.
-void launch_ghost_desc(fen_ctx *c, const GhostDesc *descs, int n);
-void launch_momentum(fen_ctx *c, double dt, double A, double B);
+template <class R> inline R *as(void *p) { return static_cast<R *>(p); }
+
+// Launchers (defined in the kernel translation units, instantiated for
+// R = double and R = float).  All enqueue on ctx->stream.
+template <class R> void launch_ghost_desc(fen_ctx *c, const GhostDesc<R> *descs, int n);
+template <class R> void launch_momentum(fen_ctx *c, double dt, double A, double B);
 // vel == nullptr: the right-hand side is read from direct_src (strides sx, sy, ghost level gl)
-void launch_div_r2c_impl(fen_ctx *c, const double *const *vel, const double *direct_src, long sx, long sy, int gl,
-                         double rho, double dt, double post);
-void launch_y_fft(fen_ctx *c, int sign, double post_scale);
-void launch_y_solve_2d(fen_ctx *c, double inv_n);
-void launch_z_solve(fen_ctx *c, double inv_n);
-void launch_thomas(fen_ctx *c);
-void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want_sum, int img);
-void launch_project(fen_ctx *c, double dt, double *checks_out);
-void launch_checks(fen_ctx *c, const double *const *vel, double *out2);
+template <class R>
+void launch_div_r2c_impl(fen_ctx *c, const R *const *vel, const R *direct_src, long sx, long sy, int gl, double rho,
+                         double dt, double post);
+template <class R> void launch_y_fft(fen_ctx *c, int sign, double post_scale);
+template <class R> void launch_y_solve_2d(fen_ctx *c, double inv_n);
+template <class R> void launch_z_solve(fen_ctx *c, double inv_n);
+template <class R> void launch_thomas(fen_ctx *c);
+template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img);
+template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out);
+template <class R> void launch_checks(fen_ctx *c, const R *const *vel, double *out2);
 long stencil_blocks(const Box &b);   // workgroups of the momentum / projection grids
-void launch_apply_shift(fen_ctx *c, double *f, size_t n);
-void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz);
+template <class R> void launch_apply_shift(fen_ctx *c, R *f, size_t n);
+void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz);   // FP64 contexts
 
 // Profiling helpers.
 void prof_begin(fen_ctx *c, int k);

@@ -17,10 +17,6 @@
 
 namespace fen {
 
-template <typename T> struct Cplx;
-template <> struct Cplx<double> { using type = double2; };
-template <> struct Cplx<float> { using type = float2; };
-
 template <typename C> __device__ __forceinline__ C c_add(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
 template <typename C> __device__ __forceinline__ C c_sub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
 template <typename C> __device__ __forceinline__ C c_mul(C a, C b)

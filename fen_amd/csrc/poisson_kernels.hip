@@ -1,4 +1,7 @@
-// poisson_kernels.hip -- FP64 fast direct Poisson solver passes.
+// poisson_kernels.hip -- fast direct Poisson solver passes, for element type
+// R = double (FP64) and R = float (the FP32 form: FP32 transforms in LDS; the
+// divergence, the eigenvalue divides and the Thomas sweep compute in FP64 and
+// round once to R).
 //
 // Reference: poisson_solver_ppp (src/poisson.f90:941-1034),
 // poisson_solver_ppn (:1038-1173), poisson_solver_pp (:416-505) and the
@@ -20,10 +23,11 @@ template <int N> constexpr int lines_per_block() { return kBlock / (N / 8); }
 // K2: phi = (div(u*) * rho) / dt on TJ consecutive (j,k) rows, x r2c of row
 // pairs packed as one complex line, spectrum scattered as j-runs.
 // ---------------------------------------------------------------------------
-struct DivR2CArgs {
-    const double *u, *v, *w;
-    double2 *spec;
-    const double2 *tw;
+template <class R> struct DivR2CArgs {
+    using C = typename Cplx<R>::type;
+    const R *u, *v, *w;
+    C *spec;
+    const C *tw;
     Box b;
     double id, rho, dt, post;
     long nrows, kxs;
@@ -32,12 +36,13 @@ struct DivR2CArgs {
     int gl;
 };
 
-template <int D3, int N>
-__global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs a)
+template <class R, int D3, int N>
+__global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs<R> a)
 {
+    using C = typename Cplx<R>::type;
     constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), TJ = 2 * L, LL = lds_line<N>();
-    __shared__ double2 buf[L * LL];
-    __shared__ double2 tw[N];
+    __shared__ C buf[L * LL];
+    __shared__ C tw[N];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
     const int ny = a.b.ny;
@@ -51,14 +56,15 @@ __global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs a)
                 val = a.u[(i - 1 + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
             } else {
                 const long g = a.b.g(i, j, k);
-                double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
-                if constexpr (D3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
+                double d = ((double)a.u[g] - (double)a.u[g - 1]) * a.id +
+                           ((double)a.v[g] - (double)a.v[g - a.b.gx]) * a.id;
+                if constexpr (D3) d = d + ((double)a.w[g] - (double)a.w[g - a.b.gxy]) * a.id;
                 val = d * a.rho / a.dt;
             }
         }
-        double2 *line = buf + (row >> 1) * LL;
-        if (row & 1) line[lds_pad(i - 1)].y = val;
-        else line[lds_pad(i - 1)].x = val;
+        C *line = buf + (row >> 1) * LL;
+        if (row & 1) line[lds_pad(i - 1)].y = (R)val;
+        else line[lds_pad(i - 1)].x = (R)val;
     }
     __syncthreads();
     fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
@@ -66,13 +72,14 @@ __global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs a)
         const int row = e % TJ, kx = e / TJ;
         const long r = row0 + row;
         if (r >= a.nrows) continue;
-        const double2 *line = buf + (row >> 1) * LL;
-        const double2 zk = line[lds_pad(kx)], zn = line[lds_pad((N - kx) & (N - 1))];
-        double2 X;
-        if ((row & 1) == 0) X = double2{(zk.x + zn.x) * 0.5, (zk.y - zn.y) * 0.5};
-        else X = double2{(zk.y + zn.y) * 0.5, (zn.x - zk.x) * 0.5};
-        X.x = X.x * a.post;
-        X.y = X.y * a.post;
+        const C *line = buf + (row >> 1) * LL;
+        const C zk = line[lds_pad(kx)], zn = line[lds_pad((N - kx) & (N - 1))];
+        const R h = R(0.5), post = (R)a.post;   // post is 1 or 1/nx: exact in R
+        C X;
+        if ((row & 1) == 0) X = C{(zk.x + zn.x) * h, (zk.y - zn.y) * h};
+        else X = C{(zk.y + zn.y) * h, (zn.x - zk.x) * h};
+        X.x = X.x * post;
+        X.y = X.y * post;
         a.spec[kx * a.kxs + r] = X;
     }
 }
@@ -80,10 +87,11 @@ __global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs a)
 // ---------------------------------------------------------------------------
 // K6: x c2r of row pairs -> phi rows (+ per-block sums for the mean).
 // ---------------------------------------------------------------------------
-struct C2RArgs {
-    const double2 *spec;
-    const double2 *tw;
-    double *out;
+template <class R> struct C2RArgs {
+    using C = typename Cplx<R>::type;
+    const C *spec;
+    const C *tw;
+    R *out;
     long nrows, kxs, sx, sy;
     int ny, gl, want_sum;
     double *part;
@@ -91,27 +99,28 @@ struct C2RArgs {
     int img;   // periodic image axes to write (phi in the time step)
 };
 
-template <int N>
-__global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs a)
+template <class R, int N>
+__global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs<R> a)
 {
+    using C = typename Cplx<R>::type;
     constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), TJ = 2 * L, LL = lds_line<N>();
-    __shared__ double2 buf[L * LL];
-    __shared__ double2 tw[N];
+    __shared__ C buf[L * LL];
+    __shared__ C tw[N];
     __shared__ double red[kBlock];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
     for (int e = threadIdx.x; e < L * (N / 2 + 1); e += kBlock) {
         const int l = e % L, kx = e / L;
         const long r = row0 + 2 * l;
-        double2 X1 = double2{0.0, 0.0}, X2 = double2{0.0, 0.0};
+        C X1 = C{R(0), R(0)}, X2 = C{R(0), R(0)};
         if (r < a.nrows) X1 = a.spec[kx * a.kxs + r];
         if (r + 1 < a.nrows) X2 = a.spec[kx * a.kxs + r + 1];
-        double2 *line = buf + l * LL;
+        C *line = buf + l * LL;
         if (kx == 0 || kx == N / 2) {
-            line[lds_pad(kx)] = double2{X1.x, X2.x};   // Im of bins 0 and n/2 ignored
+            line[lds_pad(kx)] = C{X1.x, X2.x};   // Im of bins 0 and n/2 ignored
         } else {
-            line[lds_pad(kx)] = double2{X1.x - X2.y, X1.y + X2.x};
-            line[lds_pad(N - kx)] = double2{X1.x + X2.y, X2.x - X1.y};
+            line[lds_pad(kx)] = C{X1.x - X2.y, X1.y + X2.x};
+            line[lds_pad(N - kx)] = C{X1.x + X2.y, X2.x - X1.y};
         }
     }
     __syncthreads();
@@ -121,12 +130,12 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs a)
         const int row = e / N, i = e % N;
         const long r = row0 + row;
         if (r >= a.nrows) continue;
-        const double2 z = buf[(row >> 1) * LL + lds_pad(i)];
-        const double val = (row & 1) ? z.y : z.x;
+        const C z = buf[(row >> 1) * LL + lds_pad(i)];
+        const R val = (row & 1) ? z.y : z.x;
         const long j = r % a.ny, k = r / a.ny;
         a.out[(i + a.gl) + a.sx * (j + a.gl) + a.sy * (k + a.gl)] = val;
         if (a.img) write_images(a.out, a.b, a.img, i + 1, (int)j + 1, (int)k + 1, val);
-        acc += val;
+        acc += (double)val;
     }
     if (a.want_sum) {
         red[threadIdx.x] = acc;
@@ -144,30 +153,43 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs a)
 // mode 0: plain transform times `post`; mode 1 (pp, 2-D): forward, times
 // inv_n, divide by mwx(kx)+mwy(j) (zero mode -> 0), inverse.
 // ---------------------------------------------------------------------------
-struct YArgs {
-    double2 *spec;
-    const double2 *tw;
+template <class R> struct YArgs {
+    using C = typename Cplx<R>::type;
+    C *spec;
+    const C *tw;
     long nlines;
     double post, inv_n;
     const double *mwx, *mwy;
     int nz;
 };
 
-template <int N, int SIGN, int MODE>
-__global__ __launch_bounds__(kBlock) void y_kernel(YArgs a)
+// Scales X by s1 and divides by s2 in FP64, rounding once to R (for R =
+// double these are the reference's two roundings, in its order).
+// Scales X by s1 and divides by s2 in FP64, rounding once to R (for R =
+// double these are the reference's two roundings, in its order).
+template <class C> __device__ __forceinline__ C scale_div(C X, double s1, double s2)
 {
+    using R = decltype(X.x);
+    const double x = (double)X.x * s1, y = (double)X.y * s1;
+    return C{(R)(x / s2), (R)(y / s2)};
+}
+
+template <class R, int N, int SIGN, int MODE>
+__global__ __launch_bounds__(kBlock) void y_kernel(YArgs<R> a)
+{
+    using C = typename Cplx<R>::type;
     constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), LL = lds_line<N>();
-    __shared__ double2 buf[L * LL];
-    __shared__ double2 tw[N];
+    __shared__ C buf[L * LL];
+    __shared__ C tw[N];
     load_twiddles<N>(tw, a.tw);
     const long line0 = (long)blockIdx.x * L;
     for (int e = threadIdx.x; e < L * N; e += kBlock) {
         const int l = e / N, j = e % N;
         const long ln = line0 + l;
-        buf[l * LL + lds_pad(j)] = ln < a.nlines ? a.spec[ln * N + j] : double2{0.0, 0.0};
+        buf[l * LL + lds_pad(j)] = ln < a.nlines ? a.spec[ln * N + j] : C{R(0), R(0)};
     }
     __syncthreads();
-    double2 *line = buf + (threadIdx.x / T) * LL;
+    C *line = buf + (threadIdx.x / T) * LL;
     fft_lds<N, SIGN>(line, threadIdx.x % T, tw);
     if constexpr (MODE == 1) {
         for (int e = threadIdx.x; e < L * N; e += kBlock) {
@@ -175,12 +197,10 @@ __global__ __launch_bounds__(kBlock) void y_kernel(YArgs a)
             const long ln = line0 + l;
             if (ln >= a.nlines) continue;
             const int kx = (int)(ln / a.nz);
-            double2 X = buf[l * LL + lds_pad(j)];
-            X.x = X.x * a.inv_n;
-            X.y = X.y * a.inv_n;
+            C X = buf[l * LL + lds_pad(j)];
             const double s = a.mwx[kx] + a.mwy[j];
-            if (s == 0.0) X = double2{0.0, 0.0};
-            else { X.x = X.x / s; X.y = X.y / s; }
+            if (s == 0.0) X = C{R(0), R(0)};
+            else X = scale_div(X, a.inv_n, s);
             buf[l * LL + lds_pad(j)] = X;
         }
         __syncthreads();
@@ -190,9 +210,9 @@ __global__ __launch_bounds__(kBlock) void y_kernel(YArgs a)
         const int l = e / N, j = e % N;
         const long ln = line0 + l;
         if (ln >= a.nlines) continue;
-        double2 X = buf[l * LL + lds_pad(j)];
-        X.x = X.x * a.post;
-        X.y = X.y * a.post;
+        C X = buf[l * LL + lds_pad(j)];
+        X.x = (R)((double)X.x * a.post);
+        X.y = (R)((double)X.y * a.post);
         a.spec[ln * N + j] = X;
     }
 }
@@ -201,42 +221,42 @@ __global__ __launch_bounds__(kBlock) void y_kernel(YArgs a)
 // K4 (ppp): z forward, /N, divide by (mwx+mwy)+mwz (zero mode -> 0), z inverse.
 // A block owns L adjacent j lines of one kx plane; z lines have stride ny.
 // ---------------------------------------------------------------------------
-struct ZArgs {
-    double2 *spec;
-    const double2 *tw;
+template <class R> struct ZArgs {
+    using C = typename Cplx<R>::type;
+    C *spec;
+    const C *tw;
     int ny, nz;
     long kxs;
     double inv_n;
     const double *mwx, *mwy, *mwz;
 };
 
-template <int N>
-__global__ __launch_bounds__(kBlock) void z_solve_kernel(ZArgs a)
+template <class R, int N>
+__global__ __launch_bounds__(kBlock) void z_solve_kernel(ZArgs<R> a)
 {
+    using C = typename Cplx<R>::type;
     constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), LL = lds_line<N>();
-    __shared__ double2 buf[L * LL];
-    __shared__ double2 tw[N];
+    __shared__ C buf[L * LL];
+    __shared__ C tw[N];
     load_twiddles<N>(tw, a.tw);
     const int kx = blockIdx.y;
     const int j0 = blockIdx.x * L;
-    double2 *base = a.spec + kx * a.kxs + j0;
+    C *base = a.spec + kx * a.kxs + j0;
     for (int e = threadIdx.x; e < L * N; e += kBlock) {
         const int l = e % L, k = e / L;
-        buf[l * LL + lds_pad(k)] = (j0 + l < a.ny) ? base[(long)k * a.ny + l] : double2{0.0, 0.0};
+        buf[l * LL + lds_pad(k)] = (j0 + l < a.ny) ? base[(long)k * a.ny + l] : C{R(0), R(0)};
     }
     __syncthreads();
-    double2 *line = buf + (threadIdx.x / T) * LL;
+    C *line = buf + (threadIdx.x / T) * LL;
     fft_lds<N, -1>(line, threadIdx.x % T, tw);
     const double mx = a.mwx[kx];
     for (int e = threadIdx.x; e < L * N; e += kBlock) {
         const int l = e % L, k = e / L;
         if (j0 + l >= a.ny) continue;
-        double2 X = buf[l * LL + lds_pad(k)];
-        X.x = X.x * a.inv_n;
-        X.y = X.y * a.inv_n;
+        C X = buf[l * LL + lds_pad(k)];
         const double s = (mx + a.mwy[j0 + l]) + a.mwz[k];
-        if (s == 0.0) X = double2{0.0, 0.0};
-        else { X.x = X.x / s; X.y = X.y / s; }
+        if (s == 0.0) X = C{R(0), R(0)};
+        else X = scale_div(X, a.inv_n, s);
         buf[l * LL + lds_pad(k)] = X;
     }
     __syncthreads();
@@ -250,60 +270,76 @@ __global__ __launch_bounds__(kBlock) void z_solve_kernel(ZArgs a)
 // ---------------------------------------------------------------------------
 // K4 (ppn): Thomas sweep along z for every (kx, j) column
 // (src/poisson.f90:1094-1135); one thread per column, adjacent threads own
-// adjacent j so every k-row access is coalesced.  d1 is kept in place in
-// the spectrum, c1 in a scratch array of the same layout.
+// adjacent j so every k-row access is coalesced.  The sweep is FP64 for
+// either element type: c1 lives in an FP64 scratch array of the spectrum's
+// layout; d is kept in place in the spectrum when R is double, and in an
+// FP64 scratch `dd` when R is float (the back substitution of a nearly
+// singular column would amplify a rounded d by up to ~nz^2).
 // ---------------------------------------------------------------------------
-struct ThomasArgs {
-    double2 *spec;
+template <class R> struct ThomasArgs {
+    typename Cplx<R>::type *spec;
     double *c1;
+    double2 *dd;
     int ny, nz, hx;
     long kxs;
     const double *mwx, *mwy, *ta, *tb, *tc;
 };
 
-__global__ __launch_bounds__(kBlock) void thomas_kernel(ThomasArgs a)
+template <class R> __global__ __launch_bounds__(kBlock) void thomas_kernel(ThomasArgs<R> a)
 {
+    using C = typename Cplx<R>::type;
+    constexpr bool IN_PLACE = sizeof(R) == sizeof(double);
     const long t = (long)blockIdx.x * kBlock + threadIdx.x;
     if (t >= (long)a.hx * a.ny) return;
     const int kx = (int)(t / a.ny), j = (int)(t % a.ny);
     const long base = kx * a.kxs + j;
     const long s = a.ny;
-    double2 *x = a.spec + base;
+    C *x = a.spec + base;
     double *c1 = a.c1 + base;
+    double2 *dd = IN_PLACE ? nullptr : a.dd + base;
+    auto rhs = [&](long o) { const C v = x[o]; return double2{(double)v.x, (double)v.y}; };
+    auto put_d = [&](long o, double2 v) {
+        if constexpr (IN_PLACE) x[o] = C{(R)v.x, (R)v.y};
+        else dd[o] = v;
+    };
+    auto get_d = [&](long o) {
+        if constexpr (IN_PLACE) return rhs(o);
+        else return dd[o];
+    };
     const double mx = a.mwx[kx], my = a.mwy[j];
     const int n = a.nz;
     double factor = 1.0 / ((a.tb[0] + mx) + my);
     double c1p = a.tc[0] * factor;
     c1[0] = c1p;
-    double2 d = x[0];
+    double2 d = rhs(0);
     d.x = d.x * factor;
     d.y = d.y * factor;
-    x[0] = d;
+    put_d(0, d);
     for (int k = 1; k < n - 1; k++) {
         factor = 1.0 / (((a.tb[k] + mx) + my) - a.ta[k] * c1p);
         c1p = a.tc[k] * factor;
         c1[k * s] = c1p;
-        const double2 r = x[k * s];
+        const double2 r = rhs(k * s);
         d.x = (r.x - a.ta[k] * d.x) * factor;
         d.y = (r.y - a.ta[k] * d.y) * factor;
-        x[k * s] = d;
+        put_d(k * s, d);
     }
     factor = ((a.tb[n - 1] + mx) + my) - a.ta[n - 1] * c1p;
-    const double2 r = x[(n - 1) * s];
+    const double2 r = rhs((n - 1) * s);
     if (factor != 0.0) {
         d.x = (r.x - a.ta[n - 1] * d.x) / factor;
         d.y = (r.y - a.ta[n - 1] * d.y) / factor;
     } else {
         d = double2{0.0, 0.0};
     }
-    x[(n - 1) * s] = d;
+    x[(n - 1) * s] = C{(R)d.x, (R)d.y};
     double2 xn = d;
     for (int k = n - 2; k >= 0; k--) {
-        const double2 dk = x[k * s];
+        const double2 dk = get_d(k * s);
         const double ck = c1[k * s];
         xn.x = dk.x - ck * xn.x;
         xn.y = dk.y - ck * xn.y;
-        x[k * s] = xn;
+        x[k * s] = C{(R)xn.x, (R)xn.y};
     }
 }
 
@@ -322,10 +358,10 @@ __global__ __launch_bounds__(kBlock) void mean_shift_kernel(const double *part, 
     if (threadIdx.x == 0) out[0] = red[0] / count;
 }
 
-__global__ void apply_shift_kernel(double *f, long n, const double *shift)
+template <class R> __global__ void apply_shift_kernel(R *f, long n, const double *shift)
 {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) f[q] = f[q] - shift[0];
+    if (q < n) f[q] = (R)((double)f[q] - shift[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,17 +380,23 @@ __global__ void apply_shift_kernel(double *f, long n, const double *shift)
     default: throw std::runtime_error("unsupported transform length"); \
     }
 
-static double spec_bytes(const fen_ctx *c) { return (double)c->hx * c->ny * c->nz * 16.0; }
+template <class R> static double spec_bytes(const fen_ctx *c)
+{
+    return (double)c->hx * c->ny * c->nz * 2.0 * sizeof(R);
+}
+
+template <class R> using CplxT = typename Cplx<R>::type;
 
 // vel == nullptr: direct mode, the RHS is already in `direct_src` (phi array).
-void launch_div_r2c_impl(fen_ctx *c, const double *const *vel, const double *direct_src, long sx, long sy,
-                         int gl, double rho, double dt, double post)
+template <class R>
+void launch_div_r2c_impl(fen_ctx *c, const R *const *vel, const R *direct_src, long sx, long sy, int gl, double rho,
+                         double dt, double post)
 {
-    DivR2CArgs a{};
+    DivR2CArgs<R> a{};
     if (vel) { a.u = vel[0]; a.v = vel[1]; a.w = vel[2]; }
     else { a.u = direct_src; a.direct = 1; a.sx = sx; a.sy = sy; a.gl = gl; }
-    a.spec = c->d_spec;
-    a.tw = c->d_twx;
+    a.spec = as<CplxT<R>>(c->d_spec);
+    a.tw = as<CplxT<R>>(c->d_twx);
     a.b = c->box;
     a.id = 1.0 / c->delta;
     a.rho = rho;
@@ -364,21 +406,21 @@ void launch_div_r2c_impl(fen_ctx *c, const double *const *vel, const double *dir
     a.kxs = (long)c->ny * c->nz;
     const int tj = 2 * (kBlock / (c->nx / 8));
     const dim3 grid((unsigned)((a.nrows + tj - 1) / tj));
-    const double bytes = (vel ? (c->dim == 3 ? 3.0 : 2.0) : 1.0) * 8.0 * c->n_int + spec_bytes(c);
+    const double bytes = (vel ? (c->dim == 3 ? 3.0 : 2.0) : 1.0) * sizeof(R) * c->n_int + spec_bytes<R>(c);
     prof_begin(c, K_DIV_R2C);
-#define L_DIV(NN)                                                                                  \
-    if (c->dim == 3) hipLaunchKernelGGL((div_r2c_kernel<1, NN>), grid, dim3(kBlock), 0, c->stream, a); \
-    else hipLaunchKernelGGL((div_r2c_kernel<0, NN>), grid, dim3(kBlock), 0, c->stream, a);
+#define L_DIV(NN)                                                                                     \
+    if (c->dim == 3) hipLaunchKernelGGL((div_r2c_kernel<R, 1, NN>), grid, dim3(kBlock), 0, c->stream, a); \
+    else hipLaunchKernelGGL((div_r2c_kernel<R, 0, NN>), grid, dim3(kBlock), 0, c->stream, a);
     FEN_DISPATCH_N(c->nx, L_DIV)
 #undef L_DIV
     prof_end(c, K_DIV_R2C, bytes);
 }
 
-void launch_y_fft(fen_ctx *c, int sign, double post)
+template <class R> void launch_y_fft(fen_ctx *c, int sign, double post)
 {
-    YArgs a{};
-    a.spec = c->d_spec;
-    a.tw = c->d_twy;
+    YArgs<R> a{};
+    a.spec = as<CplxT<R>>(c->d_spec);
+    a.tw = as<CplxT<R>>(c->d_twy);
     a.nlines = (long)c->hx * c->nz;
     a.post = post;
     a.nz = c->nz;
@@ -386,19 +428,19 @@ void launch_y_fft(fen_ctx *c, int sign, double post)
     const dim3 grid((unsigned)((a.nlines + L - 1) / L));
     const int kid = sign < 0 ? K_Y_FWD : K_Y_INV;
     prof_begin(c, kid);
-#define L_Y(NN)                                                                               \
-    if (sign < 0) hipLaunchKernelGGL((y_kernel<NN, -1, 0>), grid, dim3(kBlock), 0, c->stream, a); \
-    else hipLaunchKernelGGL((y_kernel<NN, 1, 0>), grid, dim3(kBlock), 0, c->stream, a);
+#define L_Y(NN)                                                                                  \
+    if (sign < 0) hipLaunchKernelGGL((y_kernel<R, NN, -1, 0>), grid, dim3(kBlock), 0, c->stream, a); \
+    else hipLaunchKernelGGL((y_kernel<R, NN, 1, 0>), grid, dim3(kBlock), 0, c->stream, a);
     FEN_DISPATCH_N(c->ny, L_Y)
 #undef L_Y
-    prof_end(c, kid, 2.0 * spec_bytes(c));
+    prof_end(c, kid, 2.0 * spec_bytes<R>(c));
 }
 
-void launch_y_solve_2d(fen_ctx *c, double inv_n)
+template <class R> void launch_y_solve_2d(fen_ctx *c, double inv_n)
 {
-    YArgs a{};
-    a.spec = c->d_spec;
-    a.tw = c->d_twy;
+    YArgs<R> a{};
+    a.spec = as<CplxT<R>>(c->d_spec);
+    a.tw = as<CplxT<R>>(c->d_twy);
     a.nlines = (long)c->hx * c->nz;
     a.post = 1.0;
     a.inv_n = inv_n;
@@ -408,17 +450,17 @@ void launch_y_solve_2d(fen_ctx *c, double inv_n)
     const int L = kBlock / (c->ny / 8);
     const dim3 grid((unsigned)((a.nlines + L - 1) / L));
     prof_begin(c, K_Y_SOLVE);
-#define L_YS(NN) hipLaunchKernelGGL((y_kernel<NN, -1, 1>), grid, dim3(kBlock), 0, c->stream, a);
+#define L_YS(NN) hipLaunchKernelGGL((y_kernel<R, NN, -1, 1>), grid, dim3(kBlock), 0, c->stream, a);
     FEN_DISPATCH_N(c->ny, L_YS)
 #undef L_YS
-    prof_end(c, K_Y_SOLVE, 2.0 * spec_bytes(c));
+    prof_end(c, K_Y_SOLVE, 2.0 * spec_bytes<R>(c));
 }
 
-void launch_z_solve(fen_ctx *c, double inv_n)
+template <class R> void launch_z_solve(fen_ctx *c, double inv_n)
 {
-    ZArgs a{};
-    a.spec = c->d_spec;
-    a.tw = c->d_twz;
+    ZArgs<R> a{};
+    a.spec = as<CplxT<R>>(c->d_spec);
+    a.tw = as<CplxT<R>>(c->d_twz);
     a.ny = c->ny;
     a.nz = c->nz;
     a.kxs = (long)c->ny * c->nz;
@@ -429,17 +471,19 @@ void launch_z_solve(fen_ctx *c, double inv_n)
     const int L = kBlock / (c->nz / 8);
     const dim3 grid((unsigned)((c->ny + L - 1) / L), (unsigned)c->hx);
     prof_begin(c, K_Z_SOLVE);
-#define L_Z(NN) hipLaunchKernelGGL((z_solve_kernel<NN>), grid, dim3(kBlock), 0, c->stream, a);
+#define L_Z(NN) hipLaunchKernelGGL((z_solve_kernel<R, NN>), grid, dim3(kBlock), 0, c->stream, a);
     FEN_DISPATCH_N(c->nz, L_Z)
 #undef L_Z
-    prof_end(c, K_Z_SOLVE, 2.0 * spec_bytes(c));
+    prof_end(c, K_Z_SOLVE, 2.0 * spec_bytes<R>(c));
 }
 
-void launch_thomas(fen_ctx *c)
+template <class R> void launch_thomas(fen_ctx *c)
 {
-    ThomasArgs a{};
-    a.spec = c->d_spec;
+    ThomasArgs<R> a{};
+    a.spec = as<CplxT<R>>(c->d_spec);
     a.c1 = c->d_c1;
+    a.dd = c->d_dd;
+    if (sizeof(R) != sizeof(double) && !a.dd) throw std::runtime_error("FP64 Thomas scratch missing");
     a.ny = c->ny;
     a.nz = c->nz;
     a.hx = c->hx;
@@ -451,15 +495,18 @@ void launch_thomas(fen_ctx *c)
     a.tc = c->d_tc;
     const long cols = (long)c->hx * c->ny;
     prof_begin(c, K_THOMAS);
-    hipLaunchKernelGGL(thomas_kernel, dim3((unsigned)((cols + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, a);
-    prof_end(c, K_THOMAS, 2.0 * spec_bytes(c));
+    hipLaunchKernelGGL(thomas_kernel<R>, dim3((unsigned)((cols + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
+                       a);
+    // spectrum read + written, c1 written + read (FP64), and for R = float d written + read (FP64 complex)
+    const double n_el = (double)c->hx * c->ny * c->nz;
+    prof_end(c, K_THOMAS, 2.0 * spec_bytes<R>(c) + 16.0 * n_el + (sizeof(R) == sizeof(double) ? 0.0 : 32.0 * n_el));
 }
 
-void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want_sum, int img)
+template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img)
 {
-    C2RArgs a{};
-    a.spec = c->d_spec;
-    a.tw = c->d_twx;
+    C2RArgs<R> a{};
+    a.spec = as<CplxT<R>>(c->d_spec);
+    a.tw = as<CplxT<R>>(c->d_twx);
     a.out = phi;
     a.nrows = (long)c->ny * c->nz;
     a.kxs = a.nrows;
@@ -475,10 +522,10 @@ void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want
     const long nblk = (a.nrows + tj - 1) / tj;
     if ((size_t)nblk > c->n_part) throw std::runtime_error("partial-sum buffer too small");
     prof_begin(c, K_C2R);
-#define L_C(NN) hipLaunchKernelGGL((c2r_kernel<NN>), dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, a);
+#define L_C(NN) hipLaunchKernelGGL((c2r_kernel<R, NN>), dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, a);
     FEN_DISPATCH_N(c->nx, L_C)
 #undef L_C
-    prof_end(c, K_C2R, spec_bytes(c) + 8.0 * c->n_int);
+    prof_end(c, K_C2R, spec_bytes<R>(c) + sizeof(R) * c->n_int);
     if (want_sum) {
         const double count = (double)(float)((long)c->nx * c->ny * c->nz);
         prof_begin(c, K_REDUCE);
@@ -488,11 +535,24 @@ void launch_c2r(fen_ctx *c, double *phi, int gl_out, long sx, long sy, bool want
     }
 }
 
-void launch_apply_shift(fen_ctx *c, double *f, size_t n)
+template <class R> void launch_apply_shift(fen_ctx *c, R *f, size_t n)
 {
-    hipLaunchKernelGGL(apply_shift_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, f, (long)n,
+    hipLaunchKernelGGL(apply_shift_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, f, (long)n,
                        (const double *)c->d_shift);
-    prof_end(c, K_REDUCE, 16.0 * n);
+    prof_end(c, K_REDUCE, 2.0 * sizeof(R) * n);
 }
+
+#define FEN_INSTANTIATE(R)                                                                                     \
+    template void launch_div_r2c_impl<R>(fen_ctx *, const R *const *, const R *, long, long, int, double, double, \
+                                         double);                                                              \
+    template void launch_y_fft<R>(fen_ctx *, int, double);                                                     \
+    template void launch_y_solve_2d<R>(fen_ctx *, double);                                                     \
+    template void launch_z_solve<R>(fen_ctx *, double);                                                        \
+    template void launch_thomas<R>(fen_ctx *);                                                                 \
+    template void launch_c2r<R>(fen_ctx *, R *, int, long, long, bool, int);                                   \
+    template void launch_apply_shift<R>(fen_ctx *, R *, size_t);
+FEN_INSTANTIATE(double)
+FEN_INSTANTIATE(float)
+#undef FEN_INSTANTIATE
 
 }  // namespace fen

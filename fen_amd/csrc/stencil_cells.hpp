@@ -2,6 +2,10 @@
 // kernels, shared by every launch shape.  Expressions keep the reference's
 // evaluation order (Fortran left to right, explicit parentheses); the library
 // is built with -ffp-contract=off so they round exactly like the CPU.
+//
+// R is the element type of the fields.  The momentum stencils compute in R
+// (FP32 stencils in the FP32 form); the projection, its pressure update and
+// the checks compute in FP64 from the stored values and round once to R.
 #pragma once
 
 #include "fen_internal.hpp"
@@ -15,117 +19,118 @@ namespace fen {
 // dv_o = dv (:201-205).  rhof == density exactly for a constant density
 // field (0.5*(rho+rho) == rho), mu == viscosity.
 // ---------------------------------------------------------------------------
-struct MomArgs {
-    const double *__restrict__ u;
-    const double *__restrict__ v;
-    const double *__restrict__ w;
-    const double *__restrict__ p;
-    double *__restrict__ us;
-    double *__restrict__ vs;
-    double *__restrict__ ws;
-    double *__restrict__ dvo0;
-    double *__restrict__ dvo1;
-    double *__restrict__ dvo2;
-    const double *__restrict__ s0;
-    const double *__restrict__ s1;
-    const double *__restrict__ s2;
+template <class R> struct MomArgs {
+    const R *__restrict__ u;
+    const R *__restrict__ v;
+    const R *__restrict__ w;
+    const R *__restrict__ p;
+    R *__restrict__ us;
+    R *__restrict__ vs;
+    R *__restrict__ ws;
+    R *__restrict__ dvo0;
+    R *__restrict__ dvo1;
+    R *__restrict__ dvo2;
+    const R *__restrict__ s0;
+    const R *__restrict__ s1;
+    const R *__restrict__ s2;
     Box b;
-    double id, id2, mu, rhof, dt, A, B, g0, g1, g2;
+    R id, id2, mu, rhof, dt, A, B, g0, g1, g2;
     int nbx, nby, img;
 };
 
 // One cell of the momentum update.  U, V, W, P are accessors returning the
 // field value at offset (di, dj, dk) from cell (i, j, k); o0..o2 are the
 // old dv_o values of the cell.
-template <int D3, int SRC, class FU, class FV, class FW, class FP>
-__device__ __forceinline__ void momentum_cell(const MomArgs &a, int i, int j, int k, FU U, FV V, FW W, FP P,
-                                              double o0, double o1, double o2)
+template <int D3, int SRC, class R, class FU, class FV, class FW, class FP>
+__device__ __forceinline__ void momentum_cell(const MomArgs<R> &a, int i, int j, int k, FU U, FV V, FW W, FP P,
+                                              R o0, R o1, R o2)
 {
     const long g = a.b.g(i, j, k);
     const long q = a.b.q(i, j, k);
-    const double id = a.id, id2 = a.id2;
-    const double u0 = U(0, 0, 0), v0 = V(0, 0, 0);
+    const R id = a.id, id2 = a.id2;
+    const R c0 = R(0.0), c025 = R(0.25), c2 = R(2.0);
+    const R u0 = U(0, 0, 0), v0 = V(0, 0, 0);
 
     // ---- x momentum
-    double r;
+    R r;
     {
-        const double up = U(1, 0, 0), um = U(-1, 0, 0);
-        const double uuip = 0.25 * ((up + u0) * (up + u0));
-        const double uuim = 0.25 * ((um + u0) * (um + u0));
-        const double uvjp = ((U(0, 1, 0) + u0) * (V(1, 0, 0) + v0)) * 0.25;
-        const double uvjm = ((u0 + U(0, -1, 0)) * (V(1, -1, 0) + V(0, -1, 0))) * 0.25;
-        r = (0.0 - (uuip - uuim) * id) - (uvjp - uvjm) * id;
+        const R up = U(1, 0, 0), um = U(-1, 0, 0);
+        const R uuip = c025 * ((up + u0) * (up + u0));
+        const R uuim = c025 * ((um + u0) * (um + u0));
+        const R uvjp = ((U(0, 1, 0) + u0) * (V(1, 0, 0) + v0)) * c025;
+        const R uvjm = ((u0 + U(0, -1, 0)) * (V(1, -1, 0) + V(0, -1, 0))) * c025;
+        r = (c0 - (uuip - uuim) * id) - (uvjp - uvjm) * id;
         if constexpr (D3) {
-            const double uwkp = ((U(0, 0, 1) + u0) * (W(1, 0, 0) + W(0, 0, 0))) * 0.25;
-            const double uwkm = ((u0 + U(0, 0, -1)) * (W(1, 0, -1) + W(0, 0, -1))) * 0.25;
+            const R uwkp = ((U(0, 0, 1) + u0) * (W(1, 0, 0) + W(0, 0, 0))) * c025;
+            const R uwkm = ((u0 + U(0, 0, -1)) * (W(1, 0, -1) + W(0, 0, -1))) * c025;
             r = r - (uwkp - uwkm) * id;
         }
     }
-    double lu = (((U(1, 0, 0) - 2.0 * u0) + U(-1, 0, 0)) + ((U(0, 1, 0) - 2.0 * u0) + U(0, -1, 0))) * id2;
-    if constexpr (D3) lu = lu + ((U(0, 0, 1) - 2.0 * u0) + U(0, 0, -1)) * id2;
+    R lu = (((U(1, 0, 0) - c2 * u0) + U(-1, 0, 0)) + ((U(0, 1, 0) - c2 * u0) + U(0, -1, 0))) * id2;
+    if constexpr (D3) lu = lu + ((U(0, 0, 1) - c2 * u0) + U(0, 0, -1)) * id2;
     r = r + a.mu * lu / a.rhof;
     if constexpr (SRC) r = r + a.s0[q] / a.rhof;
-    else r = r + 0.0;
-    const double dvx = r;
+    else r = r + c0;
+    const R dvx = r;
 
     // ---- y momentum
     {
-        const double vuip = ((V(1, 0, 0) + v0) * (U(0, 1, 0) + u0)) * 0.25;
-        const double vuim = ((v0 + V(-1, 0, 0)) * (U(-1, 1, 0) + U(-1, 0, 0))) * 0.25;
-        const double vp = V(0, 1, 0), vm = V(0, -1, 0);
-        const double vvjp = 0.25 * ((vp + v0) * (vp + v0));
-        const double vvjm = 0.25 * ((vm + v0) * (vm + v0));
-        r = (0.0 - (vuip - vuim) * id) - (vvjp - vvjm) * id;
+        const R vuip = ((V(1, 0, 0) + v0) * (U(0, 1, 0) + u0)) * c025;
+        const R vuim = ((v0 + V(-1, 0, 0)) * (U(-1, 1, 0) + U(-1, 0, 0))) * c025;
+        const R vp = V(0, 1, 0), vm = V(0, -1, 0);
+        const R vvjp = c025 * ((vp + v0) * (vp + v0));
+        const R vvjm = c025 * ((vm + v0) * (vm + v0));
+        r = (c0 - (vuip - vuim) * id) - (vvjp - vvjm) * id;
         if constexpr (D3) {
-            const double vwkp = ((V(0, 0, 1) + v0) * (W(0, 1, 0) + W(0, 0, 0))) * 0.25;
-            const double vwkm = ((v0 + V(0, 0, -1)) * (W(0, 1, -1) + W(0, 0, -1))) * 0.25;
+            const R vwkp = ((V(0, 0, 1) + v0) * (W(0, 1, 0) + W(0, 0, 0))) * c025;
+            const R vwkm = ((v0 + V(0, 0, -1)) * (W(0, 1, -1) + W(0, 0, -1))) * c025;
             r = r - (vwkp - vwkm) * id;
         }
     }
-    double lv = (((V(1, 0, 0) - 2.0 * v0) + V(-1, 0, 0)) + ((V(0, 1, 0) - 2.0 * v0) + V(0, -1, 0))) * id2;
-    if constexpr (D3) lv = lv + ((V(0, 0, 1) - 2.0 * v0) + V(0, 0, -1)) * id2;
+    R lv = (((V(1, 0, 0) - c2 * v0) + V(-1, 0, 0)) + ((V(0, 1, 0) - c2 * v0) + V(0, -1, 0))) * id2;
+    if constexpr (D3) lv = lv + ((V(0, 0, 1) - c2 * v0) + V(0, 0, -1)) * id2;
     r = r + a.mu * lv / a.rhof;
     if constexpr (SRC) r = r + a.s1[q] / a.rhof;
-    else r = r + 0.0;
-    const double dvy = r;
+    else r = r + c0;
+    const R dvy = r;
 
-    const double A = a.A, B = a.B, dt = a.dt;
-    const double p0 = P(0, 0, 0);
+    const R A = a.A, B = a.B, dt = a.dt;
+    const R p0 = P(0, 0, 0);
     {
-        const double gp = (P(1, 0, 0) - p0) * id;
-        const double rhs = ((-(gp / a.rhof) + A * dvx) + B * o0) + a.g0;
-        const double val = u0 + dt * rhs;
+        const R gp = (P(1, 0, 0) - p0) * id;
+        const R rhs = ((-(gp / a.rhof) + A * dvx) + B * o0) + a.g0;
+        const R val = u0 + dt * rhs;
         a.us[g] = val;
         write_images(a.us, a.b, a.img, i, j, k, val);
         a.dvo0[q] = dvx;
     }
     {
-        const double gp = (P(0, 1, 0) - p0) * id;
-        const double rhs = ((-(gp / a.rhof) + A * dvy) + B * o1) + a.g1;
-        const double val = v0 + dt * rhs;
+        const R gp = (P(0, 1, 0) - p0) * id;
+        const R rhs = ((-(gp / a.rhof) + A * dvy) + B * o1) + a.g1;
+        const R val = v0 + dt * rhs;
         a.vs[g] = val;
         write_images(a.vs, a.b, a.img, i, j, k, val);
         a.dvo1[q] = dvy;
     }
     if constexpr (D3) {
         // ---- z momentum
-        const double w0 = W(0, 0, 0);
-        const double wuip = ((w0 + W(1, 0, 0)) * (u0 + U(0, 0, 1))) * 0.25;
-        const double wuim = ((w0 + W(-1, 0, 0)) * (U(-1, 0, 0) + U(-1, 0, 1))) * 0.25;
-        const double wvjp = ((w0 + W(0, 1, 0)) * (v0 + V(0, 0, 1))) * 0.25;
-        const double wvjm = ((w0 + W(0, -1, 0)) * (V(0, -1, 0) + V(0, -1, 1))) * 0.25;
-        const double wp = W(0, 0, 1), wm = W(0, 0, -1);
-        const double wwkp = ((w0 + wp) * (w0 + wp)) * 0.25;
-        const double wwkm = ((w0 + wm) * (w0 + wm)) * 0.25;
-        r = ((0.0 - (wuip - wuim) * id) - (wvjp - wvjm) * id) - (wwkp - wwkm) * id;
-        const double lw = ((((W(1, 0, 0) - 2.0 * w0) + W(-1, 0, 0)) + ((W(0, 1, 0) - 2.0 * w0) + W(0, -1, 0))) +
-                           ((wp - 2.0 * w0) + wm)) * id2;
+        const R w0 = W(0, 0, 0);
+        const R wuip = ((w0 + W(1, 0, 0)) * (u0 + U(0, 0, 1))) * c025;
+        const R wuim = ((w0 + W(-1, 0, 0)) * (U(-1, 0, 0) + U(-1, 0, 1))) * c025;
+        const R wvjp = ((w0 + W(0, 1, 0)) * (v0 + V(0, 0, 1))) * c025;
+        const R wvjm = ((w0 + W(0, -1, 0)) * (V(0, -1, 0) + V(0, -1, 1))) * c025;
+        const R wp = W(0, 0, 1), wm = W(0, 0, -1);
+        const R wwkp = ((w0 + wp) * (w0 + wp)) * c025;
+        const R wwkm = ((w0 + wm) * (w0 + wm)) * c025;
+        r = ((c0 - (wuip - wuim) * id) - (wvjp - wvjm) * id) - (wwkp - wwkm) * id;
+        const R lw = ((((W(1, 0, 0) - c2 * w0) + W(-1, 0, 0)) + ((W(0, 1, 0) - c2 * w0) + W(0, -1, 0))) +
+                           ((wp - c2 * w0) + wm)) * id2;
         r = r + a.mu * lw / a.rhof;
         if constexpr (SRC) r = r + a.s2[q] / a.rhof;
-        else r = r + 0.0;
-        const double gp = (P(0, 0, 1) - p0) * id;
-        const double rhs = ((-(gp / a.rhof) + A * r) + B * o2) + a.g2;
-        const double val = w0 + dt * rhs;
+        else r = r + c0;
+        const R gp = (P(0, 0, 1) - p0) * id;
+        const R rhs = ((-(gp / a.rhof) + A * r) + B * o2) + a.g2;
+        const R val = w0 + dt * rhs;
         a.ws[g] = val;
         write_images(a.ws, a.b, a.img, i, j, k, val);
         a.dvo2[q] = r;
@@ -145,19 +150,19 @@ __device__ __forceinline__ void momentum_cell(const MomArgs &a, int i, int j, in
 // A Dirichlet normal face at the high end overwrites the last cell with bc
 // before the reference's checks run; the checks see that value too.
 // ---------------------------------------------------------------------------
-struct ProjArgs {
-    const double *__restrict__ us;
-    const double *__restrict__ vs;
-    const double *__restrict__ ws;
-    const double *__restrict__ phi;
-    double *__restrict__ u;
-    double *__restrict__ v;
-    double *__restrict__ w;
-    double *__restrict__ p;
+template <class R> struct ProjArgs {
+    const R *__restrict__ us;
+    const R *__restrict__ vs;
+    const R *__restrict__ ws;
+    const R *__restrict__ phi;
+    R *__restrict__ u;
+    R *__restrict__ v;
+    R *__restrict__ w;
+    R *__restrict__ p;
     const double *shift;
     double *part;
-    const double *lo_plane[3];   // bc planes of the normal component: u left, v bottom, w front
-    const double *hi_plane[3];   // u right, v top, w back
+    const R *lo_plane[3];        // bc planes of the normal component: u left, v bottom, w front
+    const R *hi_plane[3];        // u right, v top, w back
     int lo_type[3], hi_type[3];  // their bc types
     Box b;
     double id, dt, rhof;
@@ -167,46 +172,48 @@ struct ProjArgs {
 // One cell of the projection + checks.  F(di,dj,dk) returns phi (with the
 // pending shift removed) around the cell; us0/vs0/ws0/p0 are u*, v*, w*, p at
 // the cell and usm, vsm, wsm are u*, v*, w* one cell below along their axis.
-template <int D3, class FF>
-__device__ __forceinline__ void project_cell(const ProjArgs &a, int i, int j, int k, FF F, double us0, double vs0,
+// Arithmetic is FP64; each stored value is rounded once to R, and the
+// checks see the stored (rounded) values, as a later read would.
+template <int D3, class R, class FF>
+__device__ __forceinline__ void project_cell(const ProjArgs<R> &a, int i, int j, int k, FF F, double us0, double vs0,
                                              double ws0, double p0, double usm, double vsm, double wsm, double &md,
                                              double &mv, double &bad)
 {
     const Box &b = a.b;
     const long g = b.g(i, j, k);
     const double f0 = F(0, 0, 0);
-    const double un = us0 - (F(1, 0, 0) - f0) * a.id * a.dt / a.rhof;
-    const double vn = vs0 - (F(0, 1, 0) - f0) * a.id * a.dt / a.rhof;
+    const R un = (R)(us0 - (F(1, 0, 0) - f0) * a.id * a.dt / a.rhof);
+    const R vn = (R)(vs0 - (F(0, 1, 0) - f0) * a.id * a.dt / a.rhof);
     a.u[g] = un;
     a.v[g] = vn;
     write_images(a.u, b, a.img, i, j, k, un);
     write_images(a.v, b, a.img, i, j, k, vn);
-    double wn = 0.0;
+    R wn = R(0.0);
     if constexpr (D3) {
-        wn = ws0 - (F(0, 0, 1) - f0) * a.id * a.dt / a.rhof;
+        wn = (R)(ws0 - (F(0, 0, 1) - f0) * a.id * a.dt / a.rhof);
         a.w[g] = wn;
         write_images(a.w, b, a.img, i, j, k, wn);
     }
-    const double pn = p0 + f0;
+    const R pn = (R)(p0 + f0);
     a.p[g] = pn;
     write_images(a.p, b, a.img, i, j, k, pn);
 
     // checks on the values the reference sees after its ghost fill
     const long pjk = j + (long)(b.ny + 2) * k, pik = i + (long)(b.nx + 2) * k, pij = i + (long)(b.nx + 2) * j;
-    const double uc = (i == b.nx && a.hi_type[0] == 1) ? a.hi_plane[0][pjk] : un;
-    const double vc = (j == b.ny && a.hi_type[1] == 1) ? a.hi_plane[1][pik] : vn;
+    const double uc = (i == b.nx && a.hi_type[0] == 1) ? (double)a.hi_plane[0][pjk] : (double)un;
+    const double vc = (j == b.ny && a.hi_type[1] == 1) ? (double)a.hi_plane[1][pik] : (double)vn;
     double um, vm;
-    if (i > 1 || a.lo_type[0] == 0) um = usm - (f0 - F(-1, 0, 0)) * a.id * a.dt / a.rhof;
-    else um = a.lo_type[0] == 1 ? a.lo_plane[0][pjk] : un;
-    if (j > 1 || a.lo_type[1] == 0) vm = vsm - (f0 - F(0, -1, 0)) * a.id * a.dt / a.rhof;
-    else vm = a.lo_type[1] == 1 ? a.lo_plane[1][pik] : vn;
+    if (i > 1 || a.lo_type[0] == 0) um = (R)(usm - (f0 - F(-1, 0, 0)) * a.id * a.dt / a.rhof);
+    else um = a.lo_type[0] == 1 ? (double)a.lo_plane[0][pjk] : (double)un;
+    if (j > 1 || a.lo_type[1] == 0) vm = (R)(vsm - (f0 - F(0, -1, 0)) * a.id * a.dt / a.rhof);
+    else vm = a.lo_type[1] == 1 ? (double)a.lo_plane[1][pik] : (double)vn;
     double d = (uc - um) * a.id + (vc - vm) * a.id;
     double vel = fabs(uc) + fabs(vc);
     if constexpr (D3) {
-        const double wc = (k == b.nz && a.hi_type[2] == 1) ? a.hi_plane[2][pij] : wn;
+        const double wc = (k == b.nz && a.hi_type[2] == 1) ? (double)a.hi_plane[2][pij] : (double)wn;
         double wm;
-        if (k > 1 || a.lo_type[2] == 0) wm = wsm - (f0 - F(0, 0, -1)) * a.id * a.dt / a.rhof;
-        else wm = a.lo_type[2] == 1 ? a.lo_plane[2][pij] : wn;
+        if (k > 1 || a.lo_type[2] == 0) wm = (R)(wsm - (f0 - F(0, 0, -1)) * a.id * a.dt / a.rhof);
+        else wm = a.lo_type[2] == 1 ? (double)a.lo_plane[2][pij] : (double)wn;
         d = d + (wc - wm) * a.id;
         vel = vel + fabs(wc);
     }

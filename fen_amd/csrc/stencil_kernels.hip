@@ -1,8 +1,10 @@
-// stencil_kernels.hip -- FP64 staggered-grid kernels of the time step.
+// stencil_kernels.hip -- staggered-grid kernels of the time step, for field
+// element type R = double (FP64) and R = float (the FP32 form).
 //
 // Every expression keeps the reference's evaluation order (Fortran left to
 // right, explicit parentheses) and the library is compiled with
-// -ffp-contract=off, so these kernels reproduce the CPU arithmetic bit for bit.
+// -ffp-contract=off, so the FP64 kernels reproduce the CPU arithmetic bit
+// for bit.
 //
 // Producers (momentum, projection) write the periodic images of the boundary
 // cells they compute, so a periodic ghost fill costs no extra pass; the ghost
@@ -26,22 +28,23 @@ constexpr int kTX = 64, kTY = 4, kThreads = kTX * kTY;
 // the last interior plane (f(N) = bc).  Every cell written here is computed
 // from interior cells that no thread writes, so one launch needs no ordering.
 // ---------------------------------------------------------------------------
-struct GhostLaunch {
-    GhostDesc d[4];
+template <class R> struct GhostLaunch {
+    GhostDesc<R> d[4];
     Box b;
     long nx_cells, ny_cells, nz_cells;   // cells per plane of each group
     int ngx[4], ngy[4], ngz[4];           // planes per group and field (0 when skipped)
 };
 
-__device__ double ghost_s1(const GhostDesc &d, const Box &b, int i, int j, int k)
+template <class R> __device__ R ghost_s1(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
 {
+    const R two = R(2.0);
     const long pj = j + (long)(b.ny + 2) * k;
     if (i == 0) {
         const int t = d.type[0];
         if (t == 0) return d.f[b.g(b.nx, j, k)];
         if (t == 1) {
-            const double bc = d.plane[0][pj];
-            return d.loc == 'x' ? bc : 2.0 * bc - d.f[b.g(1, j, k)];
+            const R bc = d.plane[0][pj];
+            return d.loc == 'x' ? bc : two * bc - d.f[b.g(1, j, k)];
         }
         if (t == 2) return d.f[b.g(1, j, k)];
         return d.f[b.g(i, j, k)];
@@ -50,8 +53,8 @@ __device__ double ghost_s1(const GhostDesc &d, const Box &b, int i, int j, int k
         const int t = d.type[1];
         if (t == 0) return d.f[b.g(1, j, k)];
         if (t == 1) {
-            const double bc = d.plane[1][pj];
-            return d.loc == 'x' ? bc : 2.0 * bc - d.f[b.g(b.nx, j, k)];
+            const R bc = d.plane[1][pj];
+            return d.loc == 'x' ? bc : two * bc - d.f[b.g(b.nx, j, k)];
         }
         if (t == 2) return d.f[b.g(b.nx, j, k)];
         return d.f[b.g(i, j, k)];
@@ -60,15 +63,16 @@ __device__ double ghost_s1(const GhostDesc &d, const Box &b, int i, int j, int k
     return d.f[b.g(i, j, k)];
 }
 
-__device__ double ghost_s2(const GhostDesc &d, const Box &b, int i, int j, int k)
+template <class R> __device__ R ghost_s2(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
 {
+    const R two = R(2.0);
     const long pi = i + (long)(b.nx + 2) * k;
     if (j == 0) {
         const int t = d.type[2];
         if (t == 0) return ghost_s1(d, b, i, b.ny, k);
         if (t == 1) {
-            const double bc = d.plane[2][pi];
-            return d.loc == 'y' ? bc : 2.0 * bc - ghost_s1(d, b, i, 1, k);
+            const R bc = d.plane[2][pi];
+            return d.loc == 'y' ? bc : two * bc - ghost_s1(d, b, i, 1, k);
         }
         if (t == 2) return ghost_s1(d, b, i, 1, k);
         return ghost_s1(d, b, i, j, k);
@@ -77,8 +81,8 @@ __device__ double ghost_s2(const GhostDesc &d, const Box &b, int i, int j, int k
         const int t = d.type[3];
         if (t == 0) return ghost_s1(d, b, i, 1, k);
         if (t == 1) {
-            const double bc = d.plane[3][pi];
-            return d.loc == 'y' ? bc : 2.0 * bc - ghost_s1(d, b, i, b.ny, k);
+            const R bc = d.plane[3][pi];
+            return d.loc == 'y' ? bc : two * bc - ghost_s1(d, b, i, b.ny, k);
         }
         if (t == 2) return ghost_s1(d, b, i, b.ny, k);
         return ghost_s1(d, b, i, j, k);
@@ -87,16 +91,17 @@ __device__ double ghost_s2(const GhostDesc &d, const Box &b, int i, int j, int k
     return ghost_s1(d, b, i, j, k);
 }
 
-__device__ double ghost_s3(const GhostDesc &d, const Box &b, int i, int j, int k)
+template <class R> __device__ R ghost_s3(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
 {
+    const R two = R(2.0);
     if (b.dim == 3) {
         const long pij = i + (long)(b.nx + 2) * j;
         if (k == 0) {
             const int t = d.type[4];
             if (t == 0) return ghost_s2(d, b, i, j, b.nz);
             if (t == 1) {
-                const double bc = d.plane[4][pij];
-                return d.loc == 'z' ? bc : 2.0 * bc - ghost_s2(d, b, i, j, 1);
+                const R bc = d.plane[4][pij];
+                return d.loc == 'z' ? bc : two * bc - ghost_s2(d, b, i, j, 1);
             }
             if (t == 2) return ghost_s2(d, b, i, j, 1);
             return ghost_s2(d, b, i, j, k);
@@ -105,8 +110,8 @@ __device__ double ghost_s3(const GhostDesc &d, const Box &b, int i, int j, int k
             const int t = d.type[5];
             if (t == 0) return ghost_s2(d, b, i, j, 1);
             if (t == 1) {
-                const double bc = d.plane[5][pij];
-                return d.loc == 'z' ? bc : 2.0 * bc - ghost_s2(d, b, i, j, b.nz);
+                const R bc = d.plane[5][pij];
+                return d.loc == 'z' ? bc : two * bc - ghost_s2(d, b, i, j, b.nz);
             }
             if (t == 2) return ghost_s2(d, b, i, j, b.nz);
             return ghost_s2(d, b, i, j, k);
@@ -119,10 +124,10 @@ __device__ double ghost_s3(const GhostDesc &d, const Box &b, int i, int j, int k
 // Group plane index p -> coordinate: 0 -> 0, 1 -> n+1, 2 -> n (normal overwrite).
 __device__ __forceinline__ int plane_coord(int p, int n) { return p == 0 ? 0 : (p == 1 ? n + 1 : n); }
 
-__global__ __launch_bounds__(256) void ghost_kernel(GhostLaunch a)
+template <class R> __global__ __launch_bounds__(256) void ghost_kernel(GhostLaunch<R> a)
 {
     const int q = blockIdx.y;
-    const GhostDesc &d = a.d[q];
+    const GhostDesc<R> &d = a.d[q];
     const Box &b = a.b;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long tx = a.nx_cells * a.ngx[q], ty = a.ny_cells * a.ngy[q], tz = a.nz_cells * a.ngz[q];
@@ -154,10 +159,10 @@ __global__ __launch_bounds__(256) void ghost_kernel(GhostLaunch a)
     d.f[b.g(i, j, k)] = ghost_s3(d, b, i, j, k);
 }
 
-void launch_ghost_desc(fen_ctx *c, const GhostDesc *descs, int n)
+template <class R> void launch_ghost_desc(fen_ctx *c, const GhostDesc<R> *descs, int n)
 {
     if (n <= 0 || n > 4) throw std::runtime_error("ghost fill of 1..4 fields");
-    GhostLaunch a{};
+    GhostLaunch<R> a{};
     const Box &b = c->box;
     a.b = b;
     a.nx_cells = (long)b.ny * b.nz;
@@ -166,7 +171,7 @@ void launch_ghost_desc(fen_ctx *c, const GhostDesc *descs, int n)
     long most = 0;
     double cells = 0.0;
     for (int q = 0; q < n; q++) {
-        const GhostDesc &d = descs[q];
+        const GhostDesc<R> &d = descs[q];
         a.d[q] = d;
         a.ngx[q] = (d.groups & 1) ? (d.type[FEN_RIGHT] == 1 && d.loc == 'x' ? 3 : 2) : 0;
         a.ngy[q] = (d.groups & 2) ? (d.type[FEN_TOP] == 1 && d.loc == 'y' ? 3 : 2) : 0;
@@ -178,24 +183,24 @@ void launch_ghost_desc(fen_ctx *c, const GhostDesc *descs, int n)
     if (most == 0) return;
     const dim3 grid((unsigned)((most + 255) / 256), (unsigned)n);
     prof_begin(c, K_GHOST);
-    hipLaunchKernelGGL(ghost_kernel, grid, dim3(256), 0, c->stream, a);
-    prof_end(c, K_GHOST, 16.0 * cells);
+    hipLaunchKernelGGL(ghost_kernel<R>, grid, dim3(256), 0, c->stream, a);
+    prof_end(c, K_GHOST, 2.0 * sizeof(R) * cells);
 }
 
 // 2-D: one cell per thread, neighbours straight from memory (the whole
 // 128^2 problem fits in cache).
-template <int SRC>
-__global__ __launch_bounds__(kThreads) void momentum_kernel_2d(MomArgs a)
+template <class R, int SRC>
+__global__ __launch_bounds__(kThreads) void momentum_kernel_2d(MomArgs<R> a)
 {
     const int i = blockIdx.x * kTX + (threadIdx.x % kTX) + 1;
     const int j = blockIdx.y * kTY + (threadIdx.x / kTX) + 1;
     if (i > a.b.nx || j > a.b.ny) return;
     const long g = a.b.g(i, j, 1), sj = a.b.gx;
-    auto at = [&](const double *f) {
+    auto at = [&](const R *f) {
         return [=](int di, int dj, int dk) { return f[g + di + sj * dj]; };
     };
     const long q = a.b.q(i, j, 1);
-    momentum_cell<0, SRC>(a, i, j, 1, at(a.u), at(a.v), at(a.w), at(a.p), a.dvo0[q], a.dvo1[q], 0.0);
+    momentum_cell<0, SRC>(a, i, j, 1, at(a.u), at(a.v), at(a.w), at(a.p), a.dvo0[q], a.dvo1[q], R(0));
 }
 
 // 3-D launch shapes (see stencil_march.hpp), chosen by a sweep of tile
@@ -224,20 +229,21 @@ long stencil_blocks(const Box &b)   // workgroups of the projection (one partial
     return (long)nbx * nby * nbz;
 }
 
-void launch_momentum(fen_ctx *c, double dt, double A, double B)
+template <class R> void launch_momentum(fen_ctx *c, double dt, double A, double B)
 {
-    MomArgs a{};
-    a.u = c->d_u[0]; a.v = c->d_u[1]; a.w = c->d_u[2]; a.p = c->d_p;
-    a.us = c->d_us[0]; a.vs = c->d_us[1]; a.ws = c->d_us[2];
-    a.dvo0 = c->d_dvo[0]; a.dvo1 = c->d_dvo[1]; a.dvo2 = c->d_dvo[2];
-    a.s0 = c->d_S[0]; a.s1 = c->d_S[1]; a.s2 = c->d_S[2];
+    MomArgs<R> a{};
+    a.u = as<R>(c->d_u[0]); a.v = as<R>(c->d_u[1]); a.w = as<R>(c->d_u[2]); a.p = as<R>(c->d_p);
+    a.us = as<R>(c->d_us[0]); a.vs = as<R>(c->d_us[1]); a.ws = as<R>(c->d_us[2]);
+    a.dvo0 = as<R>(c->d_dvo[0]); a.dvo1 = as<R>(c->d_dvo[1]); a.dvo2 = as<R>(c->d_dvo[2]);
+    a.s0 = as<R>(c->d_S[0]); a.s1 = as<R>(c->d_S[1]); a.s2 = as<R>(c->d_S[2]);
     a.b = c->box;
-    a.id = 1.0 / c->delta;
-    a.id2 = 1.0 / (c->delta * c->delta);
-    a.mu = c->viscosity;
-    a.rhof = 0.5 * (c->density + c->density);
-    a.dt = dt; a.A = A; a.B = B;
-    a.g0 = c->g[0]; a.g1 = c->g[1]; a.g2 = c->g[2];
+    // coefficients are formed in FP64 and rounded once to R
+    a.id = (R)(1.0 / c->delta);
+    a.id2 = (R)(1.0 / (c->delta * c->delta));
+    a.mu = (R)c->viscosity;
+    a.rhof = (R)(0.5 * (c->density + c->density));
+    a.dt = (R)dt; a.A = (R)A; a.B = (R)B;
+    a.g0 = (R)c->g[0]; a.g1 = (R)c->g[1]; a.g2 = (R)c->g[2];
     a.img = c->img_vel;
     int nbz;
     grid_blocks(c->box, MTY, MKB, &a.nbx, &a.nby, &nbz);
@@ -247,15 +253,15 @@ void launch_momentum(fen_ctx *c, double dt, double A, double B)
     if (d3) {
         const dim3 grid((unsigned)(a.nbx * a.nby * nbz));
         if (c->has_source)
-            hipLaunchKernelGGL((momentum_march_kernel<1, MTY, MKB, MUN>), grid, dim3(64 * MTY), 0, c->stream, a);
+            hipLaunchKernelGGL((momentum_march_kernel<R, 1, MTY, MKB, MUN>), grid, dim3(64 * MTY), 0, c->stream, a);
         else
-            hipLaunchKernelGGL((momentum_march_kernel<0, MTY, MKB, MUN>), grid, dim3(64 * MTY), 0, c->stream, a);
+            hipLaunchKernelGGL((momentum_march_kernel<R, 0, MTY, MKB, MUN>), grid, dim3(64 * MTY), 0, c->stream, a);
     } else {
         const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
-        if (c->has_source) hipLaunchKernelGGL((momentum_kernel_2d<1>), grid, dim3(kThreads), 0, c->stream, a);
-        else hipLaunchKernelGGL((momentum_kernel_2d<0>), grid, dim3(kThreads), 0, c->stream, a);
+        if (c->has_source) hipLaunchKernelGGL((momentum_kernel_2d<R, 1>), grid, dim3(kThreads), 0, c->stream, a);
+        else hipLaunchKernelGGL((momentum_kernel_2d<R, 0>), grid, dim3(kThreads), 0, c->stream, a);
     }
-    prof_end(c, K_MOMENTUM, words * 8.0 * c->n_int);
+    prof_end(c, K_MOMENTUM, words * sizeof(R) * c->n_int);
 }
 
 // add_advection alone on the resident velocity (reference test
@@ -309,16 +315,17 @@ void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz)
     const Box &b = c->box;
     const dim3 grid((unsigned)((b.nx + kTX - 1) / kTX), (unsigned)((b.ny + kTY - 1) / kTY),
                     (unsigned)(b.dim == 3 ? b.nz : 1));
+    const double *u = as<double>(c->d_u[0]), *v = as<double>(c->d_u[1]), *w = as<double>(c->d_u[2]);
     if (b.dim == 3)
-        hipLaunchKernelGGL((advection_kernel<1>), grid, dim3(kThreads), 0, c->stream, c->d_u[0], c->d_u[1],
-                           c->d_u[2], rx, ry, rz, b, 1.0 / c->delta);
+        hipLaunchKernelGGL((advection_kernel<1>), grid, dim3(kThreads), 0, c->stream, u, v, w, rx, ry, rz, b,
+                           1.0 / c->delta);
     else
-        hipLaunchKernelGGL((advection_kernel<0>), grid, dim3(kThreads), 0, c->stream, c->d_u[0], c->d_u[1],
-                           c->d_u[2], rx, ry, rz, b, 1.0 / c->delta);
+        hipLaunchKernelGGL((advection_kernel<0>), grid, dim3(kThreads), 0, c->stream, u, v, w, rx, ry, rz, b,
+                           1.0 / c->delta);
 }
 
-template <int SHIFT>
-__global__ __launch_bounds__(kThreads) void project_kernel_2d(ProjArgs a)
+template <class R, int SHIFT>
+__global__ __launch_bounds__(kThreads) void project_kernel_2d(ProjArgs<R> a)
 {
     const int i = blockIdx.x * kTX + (threadIdx.x % kTX) + 1;
     const int j = blockIdx.y * kTY + (threadIdx.x / kTX) + 1;
@@ -327,9 +334,9 @@ __global__ __launch_bounds__(kThreads) void project_kernel_2d(ProjArgs a)
     if (i <= b.nx && j <= b.ny) {
         const long g = b.g(i, j, 1), sj = b.gx;
         const double s = SHIFT ? a.shift[0] : 0.0;
-        auto F = [&](int di, int dj, int dk) {
-            if constexpr (SHIFT) return a.phi[g + di + sj * dj] - s;
-            else return a.phi[g + di + sj * dj];
+        auto F = [&](int di, int dj, int dk) -> double {
+            if constexpr (SHIFT) return (double)a.phi[g + di + sj * dj] - s;
+            else return (double)a.phi[g + di + sj * dj];
         };
         project_cell<0>(a, i, j, 1, F, a.us[g], a.vs[g], 0.0, a.p[g], a.us[g - 1], a.vs[g - sj], 0.0, md, mv, bad);
     }
@@ -358,11 +365,11 @@ __global__ __launch_bounds__(256) void checks_final_kernel(const double *part, l
     if (threadIdx.x == 0) { out[0] = sd[0]; out[1] = sv[0]; out[2] = sf[0]; }
 }
 
-void launch_project(fen_ctx *c, double dt, double *checks_out)
+template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out)
 {
-    ProjArgs a{};
-    a.us = c->d_us[0]; a.vs = c->d_us[1]; a.ws = c->d_us[2]; a.phi = c->d_phi;
-    a.u = c->d_u[0]; a.v = c->d_u[1]; a.w = c->d_u[2]; a.p = c->d_p;
+    ProjArgs<R> a{};
+    a.us = as<R>(c->d_us[0]); a.vs = as<R>(c->d_us[1]); a.ws = as<R>(c->d_us[2]); a.phi = as<R>(c->d_phi);
+    a.u = as<R>(c->d_u[0]); a.v = as<R>(c->d_u[1]); a.w = as<R>(c->d_u[2]); a.p = as<R>(c->d_p);
     a.shift = c->d_shift;
     a.part = c->d_part;
     const int lo[3] = {FEN_LEFT, FEN_BOTTOM, FEN_FRONT}, hi[3] = {FEN_RIGHT, FEN_TOP, FEN_BACK};
@@ -370,8 +377,8 @@ void launch_project(fen_ctx *c, double dt, double *checks_out)
         const int slot = FEN_U + d;
         a.lo_type[d] = c->bc_type[slot][lo[d]];
         a.hi_type[d] = c->bc_type[slot][hi[d]];
-        a.lo_plane[d] = c->d_plane[slot][lo[d]];
-        a.hi_plane[d] = c->d_plane[slot][hi[d]];
+        a.lo_plane[d] = as<R>(c->d_plane[slot][lo[d]]);
+        a.hi_plane[d] = as<R>(c->d_plane[slot][hi[d]]);
     }
     a.b = c->box;
     a.id = 1.0 / c->delta;
@@ -387,14 +394,16 @@ void launch_project(fen_ctx *c, double dt, double *checks_out)
     prof_begin(c, K_PROJECT);
     if (d3) {
         const dim3 grid((unsigned)nblk);
-        if (sh) hipLaunchKernelGGL((project_march_kernel<1, PTY, PKB, PUN>), grid, dim3(64 * PTY), 0, c->stream, a);
-        else hipLaunchKernelGGL((project_march_kernel<0, PTY, PKB, PUN>), grid, dim3(64 * PTY), 0, c->stream, a);
+        if (sh)
+            hipLaunchKernelGGL((project_march_kernel<R, 1, PTY, PKB, PUN>), grid, dim3(64 * PTY), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((project_march_kernel<R, 0, PTY, PKB, PUN>), grid, dim3(64 * PTY), 0, c->stream, a);
     } else {
         const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
-        if (sh) hipLaunchKernelGGL((project_kernel_2d<1>), grid, dim3(kThreads), 0, c->stream, a);
-        else hipLaunchKernelGGL((project_kernel_2d<0>), grid, dim3(kThreads), 0, c->stream, a);
+        if (sh) hipLaunchKernelGGL((project_kernel_2d<R, 1>), grid, dim3(kThreads), 0, c->stream, a);
+        else hipLaunchKernelGGL((project_kernel_2d<R, 0>), grid, dim3(kThreads), 0, c->stream, a);
     }
-    prof_end(c, K_PROJECT, (d3 ? 9.0 : 7.0) * 8.0 * c->n_int);
+    prof_end(c, K_PROJECT, (d3 ? 9.0 : 7.0) * sizeof(R) * c->n_int);
     prof_begin(c, K_REDUCE);
     hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(256), 0, c->stream, (const double *)c->d_part, nblk,
                        checks_out);
@@ -407,9 +416,9 @@ void launch_project(fen_ctx *c, double dt, double *checks_out)
 // ---------------------------------------------------------------------------
 constexpr int kCheckBlocks = 1024;
 
-template <int D3>
-__global__ __launch_bounds__(256) void checks_kernel(const double *u, const double *v, const double *w, Box b,
-                                                     double id, double *part)
+template <class R, int D3>
+__global__ __launch_bounds__(256) void checks_kernel(const R *u, const R *v, const R *w, Box b, double id,
+                                                     double *part)
 {
     __shared__ double sd[256], sv[256], sf[256];
     const long n = (long)b.nx * b.ny * b.nz;
@@ -419,10 +428,11 @@ __global__ __launch_bounds__(256) void checks_kernel(const double *u, const doub
         const long r = t / b.nx;
         const int j = (int)(r % b.ny) + 1, k = (int)(r / b.ny) + 1;
         const long g = b.g(i, j, k);
-        double d = (u[g] - u[g - 1]) * id + (v[g] - v[g - b.gx]) * id;
-        if constexpr (D3) d = d + (w[g] - w[g - b.gxy]) * id;
-        double vel = fabs(u[g]) + fabs(v[g]);
-        if constexpr (D3) vel = vel + fabs(w[g]);
+        const double ug = u[g], vg = v[g];
+        double d = (ug - (double)u[g - 1]) * id + (vg - (double)v[g - b.gx]) * id;
+        if constexpr (D3) d = d + ((double)w[g] - (double)w[g - b.gxy]) * id;
+        double vel = fabs(ug) + fabs(vg);
+        if constexpr (D3) vel = vel + fabs((double)w[g]);
         if (!(d == d) || !(vel == vel) || isinf(vel)) bad = 1.0;
         if (d > md) md = d;
         if (vel > mv) mv = vel;
@@ -444,7 +454,7 @@ __global__ __launch_bounds__(256) void checks_kernel(const double *u, const doub
     }
 }
 
-void launch_checks(fen_ctx *c, const double *const *vel, double *out)
+template <class R> void launch_checks(fen_ctx *c, const R *const *vel, double *out)
 {
     const Box &b = c->box;
     const double id = 1.0 / c->delta;
@@ -453,15 +463,24 @@ void launch_checks(fen_ctx *c, const double *const *vel, double *out)
     if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
     prof_begin(c, K_CHECKS);
     if (b.dim == 3)
-        hipLaunchKernelGGL((checks_kernel<1>), dim3((unsigned)nblk), dim3(256), 0, c->stream, vel[0], vel[1], vel[2],
-                           b, id, c->d_part);
+        hipLaunchKernelGGL((checks_kernel<R, 1>), dim3((unsigned)nblk), dim3(256), 0, c->stream, vel[0], vel[1],
+                           vel[2], b, id, c->d_part);
     else
-        hipLaunchKernelGGL((checks_kernel<0>), dim3((unsigned)nblk), dim3(256), 0, c->stream, vel[0], vel[1], vel[2],
-                           b, id, c->d_part);
-    prof_end(c, K_CHECKS, (b.dim == 3 ? 3.0 : 2.0) * 8.0 * c->n_int);
+        hipLaunchKernelGGL((checks_kernel<R, 0>), dim3((unsigned)nblk), dim3(256), 0, c->stream, vel[0], vel[1],
+                           vel[2], b, id, c->d_part);
+    prof_end(c, K_CHECKS, (b.dim == 3 ? 3.0 : 2.0) * sizeof(R) * c->n_int);
     prof_begin(c, K_REDUCE);
     hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(256), 0, c->stream, (const double *)c->d_part, nblk, out);
     prof_end(c, K_REDUCE, 24.0 * nblk);
 }
+
+template void launch_ghost_desc<double>(fen_ctx *, const GhostDesc<double> *, int);
+template void launch_ghost_desc<float>(fen_ctx *, const GhostDesc<float> *, int);
+template void launch_momentum<double>(fen_ctx *, double, double, double);
+template void launch_momentum<float>(fen_ctx *, double, double, double);
+template void launch_project<double>(fen_ctx *, double, double *);
+template void launch_project<float>(fen_ctx *, double, double *);
+template void launch_checks<double>(fen_ctx *, const double *const *, double *);
+template void launch_checks<float>(fen_ctx *, const float *const *, double *);
 
 }  // namespace fen

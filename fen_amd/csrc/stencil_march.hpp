@@ -41,12 +41,12 @@ template <class S> __device__ __forceinline__ TileIdx<S> tile_index(const Box &b
     return t;
 }
 
-template <class S, int NF> struct TileRegs {
-    double v[NF][S::PER];
+template <class S, int NF, class R> struct TileRegs {
+    R v[NF][S::PER];
 };
 
-template <class S, int NF>
-__device__ __forceinline__ void tile_fetch(TileRegs<S, NF> &r, const double *const *src, const TileIdx<S> &t,
+template <class S, int NF, class R>
+__device__ __forceinline__ void tile_fetch(TileRegs<S, NF, R> &r, const R *const *src, const TileIdx<S> &t,
                                            long kofs)
 {
 #pragma unroll
@@ -57,8 +57,8 @@ __device__ __forceinline__ void tile_fetch(TileRegs<S, NF> &r, const double *con
 }
 
 // LDS layout: [field][slot 0..2][S::N]
-template <class S, int NF, class SL>
-__device__ __forceinline__ void tile_commit(double *lds, SL slot, const TileRegs<S, NF> &r, const TileIdx<S> &t)
+template <class S, int NF, class R, class SL>
+__device__ __forceinline__ void tile_commit(R *lds, SL slot, const TileRegs<S, NF, R> &r, const TileIdx<S> &t)
 {
 #pragma unroll
     for (int f = 0; f < NF; f++)
@@ -70,20 +70,20 @@ __device__ __forceinline__ void tile_commit(double *lds, SL slot, const TileRegs
 template <int V> using IC = std::integral_constant<int, V>;
 
 // ----------------------------------------------------------------- momentum
-template <int SRC, int TY, int KB, int UNROLL>
+template <class R, int SRC, int TY, int KB, int UNROLL>
 struct MomentumMarch {
     using S = TileShape<TY>;
     static constexpr int NT = S::NT;
 
     template <class SM, class SC, class SN>
-    __device__ __forceinline__ static void plane(const MomArgs &a, double *lds, TileRegs<S, 4> &r,
-                                                 const TileIdx<S> &t, const double *const *src, int i, int j, int k,
-                                                 int kend, bool active, int c, double (&o)[3], SM sm, SC sc, SN sn)
+    __device__ __forceinline__ static void plane(const MomArgs<R> &a, R *lds, TileRegs<S, 4, R> &r,
+                                                 const TileIdx<S> &t, const R *const *src, int i, int j, int k,
+                                                 int kend, bool active, int c, R (&o)[3], SM sm, SC sc, SN sn)
     {
         tile_commit<S, 4>(lds, sn, r, t);   // plane k+1
         __syncthreads();
         if (k + 2 <= kend + 1) tile_fetch<S, 4>(r, src, t, (long)(k + 2) * a.b.gxy);
-        double on[3] = {0.0, 0.0, 0.0};
+        R on[3] = {R(0), R(0), R(0)};
         const long q = active ? a.b.q(i, j, k) : 0;
         const long qs = (long)a.b.nx * a.b.ny;
         if (active && k + 1 <= kend) {
@@ -104,7 +104,7 @@ struct MomentumMarch {
         __syncthreads();
     }
 
-    __device__ __forceinline__ static void run(const MomArgs &a, double *lds)
+    __device__ __forceinline__ static void run(const MomArgs<R> &a, R *lds)
     {
         const Box &b = a.b;
         const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -116,15 +116,15 @@ struct MomentumMarch {
         const int i = i0 + tx, j = j0 + ty;
         const bool active = i <= b.nx && j <= b.ny;
         const int c = (ty + 1) * S::LX + tx + 1;
-        const double *src[4] = {a.u, a.v, a.w, a.p};
+        const R *src[4] = {a.u, a.v, a.w, a.p};
         const TileIdx<S> t = tile_index<S>(b, i0, j0);
-        TileRegs<S, 4> r;
+        TileRegs<S, 4, R> r;
         tile_fetch<S, 4>(r, src, t, (long)(k0 - 1) * b.gxy);
         tile_commit<S, 4>(lds, IC<0>{}, r, t);
         tile_fetch<S, 4>(r, src, t, (long)k0 * b.gxy);
         tile_commit<S, 4>(lds, IC<1>{}, r, t);
         tile_fetch<S, 4>(r, src, t, (long)(k0 + 1) * b.gxy);
-        double o[3] = {0.0, 0.0, 0.0};
+        R o[3] = {R(0), R(0), R(0)};
         if (active) {
             const long q = b.q(i, j, k0);
             o[0] = a.dvo0[q]; o[1] = a.dvo1[q]; o[2] = a.dvo2[q];
@@ -146,25 +146,25 @@ struct MomentumMarch {
     }
 };
 
-template <int SRC, int TY, int KB, int UNROLL>
-__global__ __launch_bounds__(64 * TY) void momentum_march_kernel(MomArgs a)
+template <class R, int SRC, int TY, int KB, int UNROLL>
+__global__ __launch_bounds__(64 * TY) void momentum_march_kernel(MomArgs<R> a)
 {
-    __shared__ double lds[4 * 3 * TileShape<TY>::N];
-    MomentumMarch<SRC, TY, KB, UNROLL>::run(a, lds);
+    __shared__ R lds[4 * 3 * TileShape<TY>::N];
+    MomentumMarch<R, SRC, TY, KB, UNROLL>::run(a, lds);
 }
 
 // --------------------------------------------------------------- projection
 // Ring fields: phi, u*, v* (u* and v* need the cell below in i and j for the
 // checks); w* and p are prefetched one plane ahead, w* one plane below is
 // carried in a register.
-template <int SHIFT, int TY, int KB, int UNROLL>
+template <class R, int SHIFT, int TY, int KB, int UNROLL>
 struct ProjectMarch {
     using S = TileShape<TY>;
     static constexpr int NT = S::NT;
 
     template <class SM, class SC, class SN>
-    __device__ __forceinline__ static void plane(const ProjArgs &a, double *lds, TileRegs<S, 3> &r,
-                                                 const TileIdx<S> &t, const double *const *src, int i, int j, int k,
+    __device__ __forceinline__ static void plane(const ProjArgs<R> &a, R *lds, TileRegs<S, 3, R> &r,
+                                                 const TileIdx<S> &t, const R *const *src, int i, int j, int k,
                                                  int kend, bool active, int c, double s, double &wsm, double &wsc,
                                                  double &pc, double &md, double &mv, double &bad, SM sm, SC sc, SN sn)
     {
@@ -178,7 +178,7 @@ struct ProjectMarch {
             pnext = a.p[g + a.b.gxy];
         }
         if (active) {
-            auto L = [&](int f, int di, int dj, int dk) {
+            auto L = [&](int f, int di, int dj, int dk) -> double {
                 const int slot = dk < 0 ? (int)sm : (dk > 0 ? (int)sn : (int)sc);
                 return lds[(f * 3 + slot) * S::N + c + dj * S::LX + di];
             };
@@ -195,7 +195,7 @@ struct ProjectMarch {
         __syncthreads();
     }
 
-    __device__ __forceinline__ static void run(const ProjArgs &a, double *lds)
+    __device__ __forceinline__ static void run(const ProjArgs<R> &a, R *lds)
     {
         const Box &b = a.b;
         const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -209,9 +209,9 @@ struct ProjectMarch {
         const int c = (ty + 1) * S::LX + tx + 1;
         const double s = SHIFT ? a.shift[0] : 0.0;
         double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
-        const double *src[3] = {a.phi, a.us, a.vs};
+        const R *src[3] = {a.phi, a.us, a.vs};
         const TileIdx<S> t = tile_index<S>(b, i0, j0);
-        TileRegs<S, 3> r;
+        TileRegs<S, 3, R> r;
         tile_fetch<S, 3>(r, src, t, (long)(k0 - 1) * b.gxy);
         tile_commit<S, 3>(lds, IC<0>{}, r, t);
         tile_fetch<S, 3>(r, src, t, (long)k0 * b.gxy);
@@ -246,11 +246,11 @@ struct ProjectMarch {
     }
 };
 
-template <int SHIFT, int TY, int KB, int UNROLL>
-__global__ __launch_bounds__(64 * TY) void project_march_kernel(ProjArgs a)
+template <class R, int SHIFT, int TY, int KB, int UNROLL>
+__global__ __launch_bounds__(64 * TY) void project_march_kernel(ProjArgs<R> a)
 {
-    __shared__ double lds[3 * 3 * TileShape<TY>::N];
-    ProjectMarch<SHIFT, TY, KB, UNROLL>::run(a, lds);
+    __shared__ R lds[3 * 3 * TileShape<TY>::N];
+    ProjectMarch<R, SHIFT, TY, KB, UNROLL>::run(a, lds);
 }
 
 }  // namespace fen

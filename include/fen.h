@@ -60,8 +60,14 @@ enum { FEN_LEFT = 0, FEN_RIGHT = 1, FEN_BOTTOM = 2, FEN_TOP = 3, FEN_FRONT = 4, 
 /* Arithmetic of the step. */
 enum {
     FEN_FP64 = 0,        /* everything in FP64, the reference's real(dp) */
-    FEN_FP32_REFINE = 1  /* FP32 stencils/FFTs, FP64 elimination, residual and one
-                            FP64 defect-correction solve per step */
+    FEN_FP32_REFINE = 1  /* fields and spectrum stored in FP32; momentum stencils and
+                            FFTs computed in FP32; the divergence residual (Poisson
+                            right-hand side and checks), the projection update, the
+                            eigenvalue divides and the tridiagonal elimination in FP64,
+                            rounded once to FP32.  Each step's solve takes the FP64
+                            residual of the previous step as part of its right-hand
+                            side (div u* includes div u^n): the defect correction is
+                            carried by the next solve, not run as a separate one. */
 };
 
 /* Poisson solver kinds reported by fen_solver_kind (reference src/poisson.f90:70-111). */
