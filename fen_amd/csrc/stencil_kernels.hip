@@ -9,6 +9,7 @@
 // Producers (momentum, projection) write the periodic images of the boundary
 // cells they compute, so a periodic ghost fill costs no extra pass; the ghost
 // kernel then runs only over the axes that are not periodic.
+#include <cmath>
 #include <stdexcept>
 
 #include "fen_internal.hpp"
@@ -17,6 +18,13 @@
 namespace fen {
 
 constexpr int kTX = 64, kTY = 4, kThreads = kTX * kTY;
+
+// True when r is a power of two (its reciprocal is then exact).
+template <class T> static bool pow2_value(T r)
+{
+    int e = 0;
+    return r > T(0) && std::frexp(r, &e) == T(0.5);
+}
 
 // ---------------------------------------------------------------------------
 // Ghost fill: scalar%update_ghost_nodes (src/scalar.f90:223-396) evaluated
@@ -200,13 +208,23 @@ __global__ __launch_bounds__(kThreads) void momentum_kernel_2d(MomArgs<R> a)
         return [=](int di, int dj, int dk) { return f[g + di + sj * dj]; };
     };
     const long q = a.b.q(i, j, 1);
-    momentum_cell<0, SRC>(a, i, j, 1, at(a.u), at(a.v), at(a.w), at(a.p), a.dvo0[q], a.dvo1[q], R(0));
+    auto img = [&](R *f, R v) { write_images(f, a.b, a.img, i, j, 1, v); };
+    momentum_cell<0, SRC>(a, i, j, 1, at(a.u), at(a.v), at(a.w), at(a.p), a.dvo0[q], a.dvo1[q], R(0), img);
 }
 
-// 3-D launch shapes (see stencil_march.hpp), chosen by a sweep of tile
-// height, planes per workgroup and ring unrolling on MI355X at 256^3.
-constexpr int MTY = 4, MKB = 32, MUN = 3;   // momentum
-constexpr int PTY = 4, PKB = 16, PUN = 1;   // projection
+// 3-D launch shapes (see stencil_march.hpp), chosen per element type by a
+// sweep of tile height, planes per workgroup, ring unrolling and prefetch
+// depth on MI355X at 256^3.
+// (tile height TY, planes per workgroup KB, ring unrolling UN, prefetch depth PF)
+template <class R> struct Shape;
+template <> struct Shape<double> {
+    static constexpr int MTY = 8, MKB = 32, MUN = 1, MPF = 1;   // momentum
+    static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1;   // projection
+};
+template <> struct Shape<float> {
+    static constexpr int MTY = 4, MKB = 16, MUN = 2, MPF = 2;
+    static constexpr int PTY = 4, PKB = 32, PUN = 3, PPF = 1;
+};
 constexpr int MTX = 64;
 
 static void grid_blocks(const Box &b, int ty, int kb, int *nbx, int *nby, int *nbz)
@@ -222,11 +240,14 @@ static void grid_blocks(const Box &b, int ty, int kb, int *nbx, int *nby, int *n
     }
 }
 
-long stencil_blocks(const Box &b)   // workgroups of the projection (one partial each)
+long stencil_blocks(const Box &b)   // workgroups of the projection (one partial each), either element type
 {
     int nbx, nby, nbz;
-    grid_blocks(b, PTY, PKB, &nbx, &nby, &nbz);
-    return (long)nbx * nby * nbz;
+    grid_blocks(b, Shape<double>::PTY, Shape<double>::PKB, &nbx, &nby, &nbz);
+    const long n64 = (long)nbx * nby * nbz;
+    grid_blocks(b, Shape<float>::PTY, Shape<float>::PKB, &nbx, &nby, &nbz);
+    const long n32 = (long)nbx * nby * nbz;
+    return n64 > n32 ? n64 : n32;
 }
 
 template <class R> void launch_momentum(fen_ctx *c, double dt, double A, double B)
@@ -242,20 +263,25 @@ template <class R> void launch_momentum(fen_ctx *c, double dt, double A, double 
     a.id2 = (R)(1.0 / (c->delta * c->delta));
     a.mu = (R)c->viscosity;
     a.rhof = (R)(0.5 * (c->density + c->density));
+    a.rpow2 = pow2_value(a.rhof) ? 1 : 0;
+    a.irhof = a.rpow2 ? R(1) / a.rhof : R(0);
     a.dt = (R)dt; a.A = (R)A; a.B = (R)B;
     a.g0 = (R)c->g[0]; a.g1 = (R)c->g[1]; a.g2 = (R)c->g[2];
     a.img = c->img_vel;
     int nbz;
-    grid_blocks(c->box, MTY, MKB, &a.nbx, &a.nby, &nbz);
+    using SH = Shape<R>;
+    grid_blocks(c->box, SH::MTY, SH::MKB, &a.nbx, &a.nby, &nbz);
     const int d3 = c->box.dim == 3;
     const double words = d3 ? (c->has_source ? 16.0 : 13.0) : (c->has_source ? 11.0 : 9.0);
     prof_begin(c, K_MOMENTUM);
     if (d3) {
         const dim3 grid((unsigned)(a.nbx * a.nby * nbz));
         if (c->has_source)
-            hipLaunchKernelGGL((momentum_march_kernel<R, 1, MTY, MKB, MUN>), grid, dim3(64 * MTY), 0, c->stream, a);
+            hipLaunchKernelGGL((momentum_march_kernel<R, 1, SH::MTY, SH::MKB, SH::MUN, SH::MPF>), grid,
+                               dim3(64 * SH::MTY), 0, c->stream, a);
         else
-            hipLaunchKernelGGL((momentum_march_kernel<R, 0, MTY, MKB, MUN>), grid, dim3(64 * MTY), 0, c->stream, a);
+            hipLaunchKernelGGL((momentum_march_kernel<R, 0, SH::MTY, SH::MKB, SH::MUN, SH::MPF>), grid,
+                               dim3(64 * SH::MTY), 0, c->stream, a);
     } else {
         const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
         if (c->has_source) hipLaunchKernelGGL((momentum_kernel_2d<R, 1>), grid, dim3(kThreads), 0, c->stream, a);
@@ -338,7 +364,9 @@ __global__ __launch_bounds__(kThreads) void project_kernel_2d(ProjArgs<R> a)
             if constexpr (SHIFT) return (double)a.phi[g + di + sj * dj] - s;
             else return (double)a.phi[g + di + sj * dj];
         };
-        project_cell<0>(a, i, j, 1, F, a.us[g], a.vs[g], 0.0, a.p[g], a.us[g - 1], a.vs[g - sj], 0.0, md, mv, bad);
+        auto img = [&](R *f, R v) { write_images(f, a.b, a.img, i, j, 1, v); };
+        project_cell<0>(a, i, j, 1, F, a.us[g], a.vs[g], 0.0, a.p[g], a.us[g - 1], a.vs[g - sj], 0.0, md, mv, bad,
+                        img);
     }
     block_max3<kThreads>(md, mv, bad, a.part, (long)blockIdx.y * gridDim.x + blockIdx.x);
 }
@@ -384,9 +412,12 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     a.id = 1.0 / c->delta;
     a.dt = dt;
     a.rhof = 0.5 * (c->density + c->density);
+    a.rpow2 = pow2_value(a.rhof) ? 1 : 0;
+    a.irhof = a.rpow2 ? 1.0 / a.rhof : 0.0;
     a.img = c->img_vel & c->img_p;
     int nbz;
-    grid_blocks(c->box, PTY, PKB, &a.nbx, &a.nby, &nbz);
+    using SH = Shape<R>;
+    grid_blocks(c->box, SH::PTY, SH::PKB, &a.nbx, &a.nby, &nbz);
     const long nblk = (long)a.nbx * a.nby * nbz;
     if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
     const bool sh = c->shift_active;
@@ -395,9 +426,11 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     if (d3) {
         const dim3 grid((unsigned)nblk);
         if (sh)
-            hipLaunchKernelGGL((project_march_kernel<R, 1, PTY, PKB, PUN>), grid, dim3(64 * PTY), 0, c->stream, a);
+            hipLaunchKernelGGL((project_march_kernel<R, 1, SH::PTY, SH::PKB, SH::PUN, SH::PPF>), grid,
+                               dim3(64 * SH::PTY), 0, c->stream, a);
         else
-            hipLaunchKernelGGL((project_march_kernel<R, 0, PTY, PKB, PUN>), grid, dim3(64 * PTY), 0, c->stream, a);
+            hipLaunchKernelGGL((project_march_kernel<R, 0, SH::PTY, SH::PKB, SH::PUN, SH::PPF>), grid,
+                               dim3(64 * SH::PTY), 0, c->stream, a);
     } else {
         const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
         if (sh) hipLaunchKernelGGL((project_kernel_2d<R, 1>), grid, dim3(kThreads), 0, c->stream, a);

@@ -9,7 +9,10 @@
 // written to LDS one iteration later, so HBM latency overlaps arithmetic.
 // Inputs needed only at the cell itself (dv_o; w*, p in the projection) are
 // prefetched one plane ahead in registers.  With UNROLL == 3 the k loop is
-// unrolled by three so every ring slot is a compile-time LDS offset.
+// unrolled by three so every ring slot is a compile-time LDS offset.  With
+// PF == 2 two register sets alternate, so planes k+2 and k+3 are in flight
+// together (UNROLL == 2, ring slots at run time): twice the bytes in flight
+// per wave, which FP32 fields need to cover the same HBM latency.
 #pragma once
 
 #include <type_traits>
@@ -23,8 +26,12 @@ template <int TY> struct TileShape {
     static constexpr int PER = (N + NT - 1) / NT;
 };
 
+// Per-thread tile elements.  Loads and LDS writes are unconditional: an
+// element past the tile reads a valid address (the tile corner) and writes a
+// scratch row after the ring, and halo cells past the grid are clamped onto
+// it; neither is ever read by an active cell.
 template <class S> struct TileIdx {
-    long off[S::PER];   // offset in the ghosted array at k = 0, or -1 outside the tile/grid
+    int off[S::PER];   // element offset in the ghosted array at k = 0
 };
 
 template <class S> __device__ __forceinline__ TileIdx<S> tile_index(const Box &b, int i0, int j0)
@@ -34,9 +41,9 @@ template <class S> __device__ __forceinline__ TileIdx<S> tile_index(const Box &b
     for (int q = 0; q < S::PER; q++) {
         const int e = threadIdx.x + q * S::NT;
         const int ri = e % S::LX, rj = e / S::LX;
-        const int gi = i0 - 1 + ri, gj = j0 - 1 + rj;
-        const bool ok = e < S::N && gi <= b.nx + 1 && gj <= b.ny + 1;
-        t.off[q] = ok ? b.g(gi, gj, 0) : -1;
+        int gi = min(i0 - 1 + ri, b.nx + 1), gj = min(j0 - 1 + rj, b.ny + 1);
+        if (e >= S::N) { gi = i0 - 1; gj = j0 - 1; }
+        t.off[q] = gi + (int)b.gx * gj;
     }
     return t;
 }
@@ -50,39 +57,78 @@ __device__ __forceinline__ void tile_fetch(TileRegs<S, NF, R> &r, const R *const
                                            long kofs)
 {
 #pragma unroll
-    for (int f = 0; f < NF; f++)
+    for (int f = 0; f < NF; f++) {
+        const R *plane = src[f] + kofs;
 #pragma unroll
-        for (int q = 0; q < S::PER; q++)
-            if (t.off[q] >= 0) r.v[f][q] = src[f][t.off[q] + kofs];
+        for (int q = 0; q < S::PER; q++) r.v[f][q] = plane[t.off[q]];
+    }
 }
 
-// LDS layout: [field][slot 0..2][S::N]
+// LDS layout: [field][slot 0..2][S::N], then a scratch row of S::NT.
+template <class S, int NF> constexpr int ring_size() { return NF * 3 * S::N + S::NT; }
+
 template <class S, int NF, class R, class SL>
 __device__ __forceinline__ void tile_commit(R *lds, SL slot, const TileRegs<S, NF, R> &r, const TileIdx<S> &t)
 {
+    (void)t;
 #pragma unroll
     for (int f = 0; f < NF; f++)
 #pragma unroll
-        for (int q = 0; q < S::PER; q++)
-            if (t.off[q] >= 0) lds[(f * 3 + (int)slot) * S::N + threadIdx.x + q * S::NT] = r.v[f][q];
+        for (int q = 0; q < S::PER; q++) {
+            const int e = threadIdx.x + q * S::NT;
+            int li = (f * 3 + (int)slot) * S::N + e;
+            if (q == S::PER - 1 && e >= S::N) li = NF * 3 * S::N + (int)threadIdx.x;
+            lds[li] = r.v[f][q];
+        }
+}
+
+// Periodic images of cell (i, j, k) for the k-marching kernels: a wave is
+// one j row and k is the same for the whole workgroup, so only the x test
+// varies across lanes; `xedge` says whether the workgroup holds i == 1 or
+// i == nx.  Writes exactly the cells write_images() writes.
+template <class R>
+__device__ __forceinline__ void march_images(R *f, const Box &b, int ax, int i, int j, int k, R v, bool xedge)
+{
+    const bool xi = (ax & 1) && xedge && (i == 1 || i == b.nx);
+    const int ii = i == 1 ? b.nx + 1 : 0;
+    if (xi) f[b.g(ii, j, k)] = v;
+    const bool yj = (ax & 2) && (j == 1 || j == b.ny);
+    const bool zk = (ax & 4) && (k == 1 || k == b.nz);
+    if (yj || zk) {
+        const int jj = j == 1 ? b.ny + 1 : 0, kk = k == 1 ? b.nz + 1 : 0;
+        if (yj) {
+            f[b.g(i, jj, k)] = v;
+            if (xi) f[b.g(ii, jj, k)] = v;
+        }
+        if (zk) {
+            f[b.g(i, j, kk)] = v;
+            if (xi) f[b.g(ii, j, kk)] = v;
+            if (yj) {
+                f[b.g(i, jj, kk)] = v;
+                if (xi) f[b.g(ii, jj, kk)] = v;
+            }
+        }
+    }
 }
 
 template <int V> using IC = std::integral_constant<int, V>;
 
 // ----------------------------------------------------------------- momentum
-template <class R, int SRC, int TY, int KB, int UNROLL>
+template <class R, int SRC, int TY, int KB, int UNROLL, int PF = 1>
 struct MomentumMarch {
+    static_assert((PF == 1 && (UNROLL == 1 || UNROLL == 3)) || (PF == 2 && UNROLL == 2), "launch shape");
     using S = TileShape<TY>;
     static constexpr int NT = S::NT;
 
     template <class SM, class SC, class SN>
     __device__ __forceinline__ static void plane(const MomArgs<R> &a, R *lds, TileRegs<S, 4, R> &r,
                                                  const TileIdx<S> &t, const R *const *src, int i, int j, int k,
-                                                 int kend, bool active, int c, R (&o)[3], SM sm, SC sc, SN sn)
+                                                 int kend, bool active, int c, R (&o)[3], bool xedge, SM sm, SC sc,
+                                                 SN sn)
     {
         tile_commit<S, 4>(lds, sn, r, t);   // plane k+1
         __syncthreads();
-        if (k + 2 <= kend + 1) tile_fetch<S, 4>(r, src, t, (long)(k + 2) * a.b.gxy);
+        if (k + 1 + PF <= kend + 1) tile_fetch<S, 4>(r, src, t, (long)(k + 1 + PF) * a.b.gxy);
         R on[3] = {R(0), R(0), R(0)};
         const long q = active ? a.b.q(i, j, k) : 0;
         const long qs = (long)a.b.nx * a.b.ny;
@@ -98,7 +144,8 @@ struct MomentumMarch {
                     return lds[(f * 3 + slot) * S::N + c + dj * S::LX + di];
                 };
             };
-            momentum_cell<1, SRC>(a, i, j, k, at(0), at(1), at(2), at(3), o[0], o[1], o[2]);
+            auto img = [&](R *f, R v) { march_images(f, a.b, a.img, i, j, k, v, xedge); };
+            momentum_cell<1, SRC>(a, i, j, k, at(0), at(1), at(2), at(3), o[0], o[1], o[2], img);
         }
         o[0] = on[0]; o[1] = on[1]; o[2] = on[2];
         __syncthreads();
@@ -110,7 +157,9 @@ struct MomentumMarch {
         const int bid = xcd_remap(blockIdx.x, gridDim.x);
         const int bx = bid % a.nbx, rest = bid / a.nbx;
         const int by = rest % a.nby, bz = rest / a.nby;
-        const int tx = threadIdx.x % S::TX, ty = threadIdx.x / S::TX;
+        // a wave is one j row: ty is wave-uniform, so j-dependent branches are scalar
+        const int tx = threadIdx.x % S::TX, ty = __builtin_amdgcn_readfirstlane(threadIdx.x / S::TX);
+        const bool xedge = bx == 0 || bx == a.nbx - 1;
         const int i0 = bx * S::TX + 1, j0 = by * TY + 1, k0 = bz * KB + 1;
         const int kend = min(k0 + KB - 1, b.nz);
         const int i = i0 + tx, j = j0 + ty;
@@ -124,21 +173,37 @@ struct MomentumMarch {
         tile_fetch<S, 4>(r, src, t, (long)k0 * b.gxy);
         tile_commit<S, 4>(lds, IC<1>{}, r, t);
         tile_fetch<S, 4>(r, src, t, (long)(k0 + 1) * b.gxy);
+        TileRegs<S, 4, R> r2;
+        if constexpr (PF == 2) {
+            if (k0 + 2 <= kend + 1) tile_fetch<S, 4>(r2, src, t, (long)(k0 + 2) * b.gxy);
+        }
         R o[3] = {R(0), R(0), R(0)};
         if (active) {
             const long q = b.q(i, j, k0);
             o[0] = a.dvo0[q]; o[1] = a.dvo1[q]; o[2] = a.dvo2[q];
         }
-        if constexpr (UNROLL == 3) {
+        if constexpr (PF == 2) {
+            int sm = 0, sc = 1, sn = 2;
+            for (int k = k0; k <= kend; k += 2) {
+                plane(a, lds, r, t, src, i, j, k, kend, active, c, o, xedge, sm, sc, sn);
+                int tmp = sm;
+                sm = sc; sc = sn; sn = tmp;
+                if (k + 1 <= kend) {
+                    plane(a, lds, r2, t, src, i, j, k + 1, kend, active, c, o, xedge, sm, sc, sn);
+                    tmp = sm;
+                    sm = sc; sc = sn; sn = tmp;
+                }
+            }
+        } else if constexpr (UNROLL == 3) {
             for (int k = k0; k <= kend; k += 3) {
-                plane(a, lds, r, t, src, i, j, k, kend, active, c, o, IC<0>{}, IC<1>{}, IC<2>{});
-                if (k + 1 <= kend) plane(a, lds, r, t, src, i, j, k + 1, kend, active, c, o, IC<1>{}, IC<2>{}, IC<0>{});
-                if (k + 2 <= kend) plane(a, lds, r, t, src, i, j, k + 2, kend, active, c, o, IC<2>{}, IC<0>{}, IC<1>{});
+                plane(a, lds, r, t, src, i, j, k, kend, active, c, o, xedge, IC<0>{}, IC<1>{}, IC<2>{});
+                if (k + 1 <= kend) plane(a, lds, r, t, src, i, j, k + 1, kend, active, c, o, xedge, IC<1>{}, IC<2>{}, IC<0>{});
+                if (k + 2 <= kend) plane(a, lds, r, t, src, i, j, k + 2, kend, active, c, o, xedge, IC<2>{}, IC<0>{}, IC<1>{});
             }
         } else {
             int sm = 0, sc = 1, sn = 2;
             for (int k = k0; k <= kend; k++) {
-                plane(a, lds, r, t, src, i, j, k, kend, active, c, o, sm, sc, sn);
+                plane(a, lds, r, t, src, i, j, k, kend, active, c, o, xedge, sm, sc, sn);
                 const int tmp = sm;
                 sm = sc; sc = sn; sn = tmp;
             }
@@ -146,19 +211,20 @@ struct MomentumMarch {
     }
 };
 
-template <class R, int SRC, int TY, int KB, int UNROLL>
+template <class R, int SRC, int TY, int KB, int UNROLL, int PF = 1>
 __global__ __launch_bounds__(64 * TY) void momentum_march_kernel(MomArgs<R> a)
 {
-    __shared__ R lds[4 * 3 * TileShape<TY>::N];
-    MomentumMarch<R, SRC, TY, KB, UNROLL>::run(a, lds);
+    __shared__ R lds[ring_size<TileShape<TY>, 4>()];
+    MomentumMarch<R, SRC, TY, KB, UNROLL, PF>::run(a, lds);
 }
 
 // --------------------------------------------------------------- projection
 // Ring fields: phi, u*, v* (u* and v* need the cell below in i and j for the
 // checks); w* and p are prefetched one plane ahead, w* one plane below is
 // carried in a register.
-template <class R, int SHIFT, int TY, int KB, int UNROLL>
+template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1>
 struct ProjectMarch {
+    static_assert((PF == 1 && (UNROLL == 1 || UNROLL == 3)) || (PF == 2 && UNROLL == 2), "launch shape");
     using S = TileShape<TY>;
     static constexpr int NT = S::NT;
 
@@ -166,11 +232,12 @@ struct ProjectMarch {
     __device__ __forceinline__ static void plane(const ProjArgs<R> &a, R *lds, TileRegs<S, 3, R> &r,
                                                  const TileIdx<S> &t, const R *const *src, int i, int j, int k,
                                                  int kend, bool active, int c, double s, double &wsm, double &wsc,
-                                                 double &pc, double &md, double &mv, double &bad, SM sm, SC sc, SN sn)
+                                                 double &pc, double &md, double &mv, double &bad, bool xedge, SM sm,
+                                                 SC sc, SN sn)
     {
         tile_commit<S, 3>(lds, sn, r, t);
         __syncthreads();
-        if (k + 2 <= kend + 1) tile_fetch<S, 3>(r, src, t, (long)(k + 2) * a.b.gxy);
+        if (k + 1 + PF <= kend + 1) tile_fetch<S, 3>(r, src, t, (long)(k + 1 + PF) * a.b.gxy);
         double wnext = 0.0, pnext = 0.0;
         const long g = a.b.g(i, j, k);
         if (active && k + 1 <= kend) {
@@ -186,8 +253,9 @@ struct ProjectMarch {
                 if constexpr (SHIFT) return L(0, di, dj, dk) - s;
                 else return L(0, di, dj, dk);
             };
+            auto img = [&](R *f, R v) { march_images(f, a.b, a.img, i, j, k, v, xedge); };
             project_cell<1>(a, i, j, k, F, L(1, 0, 0, 0), L(2, 0, 0, 0), wsc, pc, L(1, -1, 0, 0), L(2, 0, -1, 0),
-                            wsm, md, mv, bad);
+                            wsm, md, mv, bad, img);
         }
         wsm = wsc;
         wsc = wnext;
@@ -201,7 +269,9 @@ struct ProjectMarch {
         const int bid = xcd_remap(blockIdx.x, gridDim.x);
         const int bx = bid % a.nbx, rest = bid / a.nbx;
         const int by = rest % a.nby, bz = rest / a.nby;
-        const int tx = threadIdx.x % S::TX, ty = threadIdx.x / S::TX;
+        // a wave is one j row: ty is wave-uniform, so j-dependent branches are scalar
+        const int tx = threadIdx.x % S::TX, ty = __builtin_amdgcn_readfirstlane(threadIdx.x / S::TX);
+        const bool xedge = bx == 0 || bx == a.nbx - 1;
         const int i0 = bx * S::TX + 1, j0 = by * TY + 1, k0 = bz * KB + 1;
         const int kend = min(k0 + KB - 1, b.nz);
         const int i = i0 + tx, j = j0 + ty;
@@ -217,27 +287,44 @@ struct ProjectMarch {
         tile_fetch<S, 3>(r, src, t, (long)k0 * b.gxy);
         tile_commit<S, 3>(lds, IC<1>{}, r, t);
         tile_fetch<S, 3>(r, src, t, (long)(k0 + 1) * b.gxy);
+        TileRegs<S, 3, R> r2;
+        if constexpr (PF == 2) {
+            if (k0 + 2 <= kend + 1) tile_fetch<S, 3>(r2, src, t, (long)(k0 + 2) * b.gxy);
+        }
         double wsm = 0.0, wsc = 0.0, pc = 0.0;
         if (active) {
             wsm = a.ws[b.g(i, j, k0 - 1)];
             wsc = a.ws[b.g(i, j, k0)];
             pc = a.p[b.g(i, j, k0)];
         }
-        if constexpr (UNROLL == 3) {
+        if constexpr (PF == 2) {
+            int sm = 0, sc = 1, sn = 2;
+            for (int k = k0; k <= kend; k += 2) {
+                plane(a, lds, r, t, src, i, j, k, kend, active, c, s, wsm, wsc, pc, md, mv, bad, xedge, sm, sc, sn);
+                int tmp = sm;
+                sm = sc; sc = sn; sn = tmp;
+                if (k + 1 <= kend) {
+                    plane(a, lds, r2, t, src, i, j, k + 1, kend, active, c, s, wsm, wsc, pc, md, mv, bad, xedge, sm, sc,
+                          sn);
+                    tmp = sm;
+                    sm = sc; sc = sn; sn = tmp;
+                }
+            }
+        } else if constexpr (UNROLL == 3) {
             for (int k = k0; k <= kend; k += 3) {
-                plane(a, lds, r, t, src, i, j, k, kend, active, c, s, wsm, wsc, pc, md, mv, bad, IC<0>{}, IC<1>{},
+                plane(a, lds, r, t, src, i, j, k, kend, active, c, s, wsm, wsc, pc, md, mv, bad, xedge, IC<0>{}, IC<1>{},
                       IC<2>{});
                 if (k + 1 <= kend)
-                    plane(a, lds, r, t, src, i, j, k + 1, kend, active, c, s, wsm, wsc, pc, md, mv, bad, IC<1>{},
+                    plane(a, lds, r, t, src, i, j, k + 1, kend, active, c, s, wsm, wsc, pc, md, mv, bad, xedge, IC<1>{},
                           IC<2>{}, IC<0>{});
                 if (k + 2 <= kend)
-                    plane(a, lds, r, t, src, i, j, k + 2, kend, active, c, s, wsm, wsc, pc, md, mv, bad, IC<2>{},
+                    plane(a, lds, r, t, src, i, j, k + 2, kend, active, c, s, wsm, wsc, pc, md, mv, bad, xedge, IC<2>{},
                           IC<0>{}, IC<1>{});
             }
         } else {
             int sm = 0, sc = 1, sn = 2;
             for (int k = k0; k <= kend; k++) {
-                plane(a, lds, r, t, src, i, j, k, kend, active, c, s, wsm, wsc, pc, md, mv, bad, sm, sc, sn);
+                plane(a, lds, r, t, src, i, j, k, kend, active, c, s, wsm, wsc, pc, md, mv, bad, xedge, sm, sc, sn);
                 const int tmp = sm;
                 sm = sc; sc = sn; sn = tmp;
             }
@@ -246,11 +333,11 @@ struct ProjectMarch {
     }
 };
 
-template <class R, int SHIFT, int TY, int KB, int UNROLL>
+template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1>
 __global__ __launch_bounds__(64 * TY) void project_march_kernel(ProjArgs<R> a)
 {
-    __shared__ R lds[3 * 3 * TileShape<TY>::N];
-    ProjectMarch<R, SHIFT, TY, KB, UNROLL>::run(a, lds);
+    __shared__ R lds[ring_size<TileShape<TY>, 3>()];
+    ProjectMarch<R, SHIFT, TY, KB, UNROLL, PF>::run(a, lds);
 }
 
 }  // namespace fen

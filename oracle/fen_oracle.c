@@ -924,3 +924,182 @@ void orc_ns_step(orc_ctx *c, int step, double *dt_io, double *maxdiv, double *ma
     if (maxdiv) *maxdiv = c->maxdiv;
     if (maxcfl) *maxcfl = c->maxcfl;
 }
+
+/* -------------------------------------------------------------------------- */
+/* Discrete adjoint of one step (no counterpart in the reference)             */
+/* -------------------------------------------------------------------------- */
+
+static int wrap(int i, int n) { return i < 1 ? i + n : (i > n ? i - n : i); }
+
+/* interior index of (i,j,k) with periodic wrap */
+static size_t IW(const orc_ctx *c, int i, int j, int k)
+{
+    return II(c, wrap(i, c->nx), wrap(j, c->ny), wrap(k, c->nz));
+}
+
+/* state value of a ghosted field at (i,j,k) with periodic wrap */
+static double SV(const orc_ctx *c, const double *f, int i, int j, int k)
+{
+    return f[GI(c, wrap(i, c->nx), wrap(j, c->ny), wrap(k, c->nz))];
+}
+
+int orc_adjoint_step(orc_ctx *c, double dt, double *ub, double *vb, double *wb, double *pb, double *dvobx,
+                     double *dvoby, double *dvobz)
+{
+    if (c->dim != 3 || c->solver != ORC_PPP) return -1;
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const size_t ni = interior_len(c);
+    const double id = 1.0 / c->delta, id2 = 1.0 / (c->delta * c->delta);
+    const double rho = c->density, rhof = 0.5 * (c->density + c->density), mu = c->viscosity;
+    const double A = 1.0 + 0.5 * dt / c->dt_o, B = -(0.5 * dt / c->dt_o);
+    double *bar[3] = {ub, vb, wb}, *dvob[3] = {dvobx, dvoby, dvobz};
+    double *psi = (double *)calloc(ni, sizeof(double));
+    double *star[3], *dvb[3], *out[3];
+    for (int d = 0; d < 3; d++) {
+        star[d] = (double *)malloc(ni * sizeof(double));
+        dvb[d] = (double *)malloc(ni * sizeof(double));
+        out[d] = (double *)calloc(ni, sizeof(double));
+    }
+
+    /* projection + pressure update, transposed:
+     *   phibar = pbar' + (dt/rhof) D ubar',  rhsbar = S phibar,
+     *   ubar* = ubar' - (rho/dt) G rhsbar  (G^T = -D on the periodic grid) */
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t q = II(c, i, j, k);
+                const double div = ((ub[q] - ub[IW(c, i - 1, j, k)]) * id + (vb[q] - vb[IW(c, i, j - 1, k)]) * id) +
+                                   (wb[q] - wb[IW(c, i, j, k - 1)]) * id;
+                psi[q] = pb[q] + (dt / rhof) * div;
+            }
+    orc_poisson_solve(c, psi, 0);   /* S is symmetric */
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t q = II(c, i, j, k);
+                const double s = rho / dt;
+                star[0][q] = ub[q] - s * (psi[IW(c, i + 1, j, k)] - psi[q]) * id;
+                star[1][q] = vb[q] - s * (psi[IW(c, i, j + 1, k)] - psi[q]) * id;
+                star[2][q] = wb[q] - s * (psi[IW(c, i, j, k + 1)] - psi[q]) * id;
+            }
+
+    /* predictor u* = u + dt(-G p/rhof + A dv + B dv_o + g), dv_o' = dv, transposed */
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t q = II(c, i, j, k);
+                const double div = ((star[0][q] - star[0][IW(c, i - 1, j, k)]) * id +
+                                    (star[1][q] - star[1][IW(c, i, j - 1, k)]) * id) +
+                                   (star[2][q] - star[2][IW(c, i, j, k - 1)]) * id;
+                pb[q] = pb[q] + (dt / rhof) * div;
+            }
+    for (int d = 0; d < 3; d++)
+        for (size_t q = 0; q < ni; q++) {
+            dvb[d][q] = dt * A * star[d][q] + dvob[d][q];
+            dvob[d][q] = dt * B * star[d][q];
+            out[d][q] = star[d][q];
+        }
+
+    /* dv = adv(u) + mu lap(u)/rhof, transposed at the state u^n: each forward
+     * term's partial derivatives times the adjoint of its output face */
+    const double *U = c->u.f, *V = c->v.f, *W = c->w.f;
+#define ACC(d, ii, jj, kk, val) (out[d][IW(c, (ii), (jj), (kk))] += (val))
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t q = II(c, i, j, k);
+                /* x-momentum at the u face (i,j,k) */
+                {
+                    const double a = dvb[0][q];
+                    const double u0 = SV(c, U, i, j, k);
+                    double t;
+                    t = -a * id * 0.5 * (SV(c, U, i + 1, j, k) + u0);          /* uuip */
+                    ACC(0, i + 1, j, k, t); ACC(0, i, j, k, t);
+                    t = a * id * 0.5 * (SV(c, U, i - 1, j, k) + u0);           /* uuim */
+                    ACC(0, i - 1, j, k, t); ACC(0, i, j, k, t);
+                    t = -a * id * 0.25 * (SV(c, V, i + 1, j, k) + SV(c, V, i, j, k));   /* uvjp: d/du */
+                    ACC(0, i, j + 1, k, t); ACC(0, i, j, k, t);
+                    t = -a * id * 0.25 * (SV(c, U, i, j + 1, k) + u0);                  /* uvjp: d/dv */
+                    ACC(1, i + 1, j, k, t); ACC(1, i, j, k, t);
+                    t = a * id * 0.25 * (SV(c, V, i + 1, j - 1, k) + SV(c, V, i, j - 1, k));   /* uvjm */
+                    ACC(0, i, j, k, t); ACC(0, i, j - 1, k, t);
+                    t = a * id * 0.25 * (u0 + SV(c, U, i, j - 1, k));
+                    ACC(1, i + 1, j - 1, k, t); ACC(1, i, j - 1, k, t);
+                    t = -a * id * 0.25 * (SV(c, W, i + 1, j, k) + SV(c, W, i, j, k));   /* uwkp */
+                    ACC(0, i, j, k + 1, t); ACC(0, i, j, k, t);
+                    t = -a * id * 0.25 * (SV(c, U, i, j, k + 1) + u0);
+                    ACC(2, i + 1, j, k, t); ACC(2, i, j, k, t);
+                    t = a * id * 0.25 * (SV(c, W, i + 1, j, k - 1) + SV(c, W, i, j, k - 1));   /* uwkm */
+                    ACC(0, i, j, k, t); ACC(0, i, j, k - 1, t);
+                    t = a * id * 0.25 * (u0 + SV(c, U, i, j, k - 1));
+                    ACC(2, i + 1, j, k - 1, t); ACC(2, i, j, k - 1, t);
+                }
+                /* y-momentum at the v face (i,j,k) */
+                {
+                    const double b = dvb[1][q];
+                    const double v0 = SV(c, V, i, j, k);
+                    double t;
+                    t = -b * id * 0.25 * (SV(c, U, i, j + 1, k) + SV(c, U, i, j, k));   /* vuip: d/dv */
+                    ACC(1, i + 1, j, k, t); ACC(1, i, j, k, t);
+                    t = -b * id * 0.25 * (SV(c, V, i + 1, j, k) + v0);                  /* vuip: d/du */
+                    ACC(0, i, j + 1, k, t); ACC(0, i, j, k, t);
+                    t = b * id * 0.25 * (SV(c, U, i - 1, j + 1, k) + SV(c, U, i - 1, j, k));   /* vuim */
+                    ACC(1, i, j, k, t); ACC(1, i - 1, j, k, t);
+                    t = b * id * 0.25 * (v0 + SV(c, V, i - 1, j, k));
+                    ACC(0, i - 1, j + 1, k, t); ACC(0, i - 1, j, k, t);
+                    t = -b * id * 0.5 * (SV(c, V, i, j + 1, k) + v0);                   /* vvjp */
+                    ACC(1, i, j + 1, k, t); ACC(1, i, j, k, t);
+                    t = b * id * 0.5 * (SV(c, V, i, j - 1, k) + v0);                    /* vvjm */
+                    ACC(1, i, j - 1, k, t); ACC(1, i, j, k, t);
+                    t = -b * id * 0.25 * (SV(c, W, i, j + 1, k) + SV(c, W, i, j, k));   /* vwkp */
+                    ACC(1, i, j, k + 1, t); ACC(1, i, j, k, t);
+                    t = -b * id * 0.25 * (SV(c, V, i, j, k + 1) + v0);
+                    ACC(2, i, j + 1, k, t); ACC(2, i, j, k, t);
+                    t = b * id * 0.25 * (SV(c, W, i, j + 1, k - 1) + SV(c, W, i, j, k - 1));   /* vwkm */
+                    ACC(1, i, j, k, t); ACC(1, i, j, k - 1, t);
+                    t = b * id * 0.25 * (v0 + SV(c, V, i, j, k - 1));
+                    ACC(2, i, j + 1, k - 1, t); ACC(2, i, j, k - 1, t);
+                }
+                /* z-momentum at the w face (i,j,k) */
+                {
+                    const double e = dvb[2][q];
+                    const double w0 = SV(c, W, i, j, k);
+                    double t;
+                    t = -e * id * 0.25 * (SV(c, U, i, j, k) + SV(c, U, i, j, k + 1));   /* wuip: d/dw */
+                    ACC(2, i, j, k, t); ACC(2, i + 1, j, k, t);
+                    t = -e * id * 0.25 * (w0 + SV(c, W, i + 1, j, k));                  /* wuip: d/du */
+                    ACC(0, i, j, k, t); ACC(0, i, j, k + 1, t);
+                    t = e * id * 0.25 * (SV(c, U, i - 1, j, k) + SV(c, U, i - 1, j, k + 1));   /* wuim */
+                    ACC(2, i, j, k, t); ACC(2, i - 1, j, k, t);
+                    t = e * id * 0.25 * (w0 + SV(c, W, i - 1, j, k));
+                    ACC(0, i - 1, j, k, t); ACC(0, i - 1, j, k + 1, t);
+                    t = -e * id * 0.25 * (SV(c, V, i, j, k) + SV(c, V, i, j, k + 1));   /* wvjp */
+                    ACC(2, i, j, k, t); ACC(2, i, j + 1, k, t);
+                    t = -e * id * 0.25 * (w0 + SV(c, W, i, j + 1, k));
+                    ACC(1, i, j, k, t); ACC(1, i, j, k + 1, t);
+                    t = e * id * 0.25 * (SV(c, V, i, j - 1, k) + SV(c, V, i, j - 1, k + 1));   /* wvjm */
+                    ACC(2, i, j, k, t); ACC(2, i, j - 1, k, t);
+                    t = e * id * 0.25 * (w0 + SV(c, W, i, j - 1, k));
+                    ACC(1, i, j - 1, k, t); ACC(1, i, j - 1, k + 1, t);
+                    t = -e * id * 0.5 * (w0 + SV(c, W, i, j, k + 1));                   /* wwkp */
+                    ACC(2, i, j, k, t); ACC(2, i, j, k + 1, t);
+                    t = e * id * 0.5 * (w0 + SV(c, W, i, j, k - 1));                    /* wwkm */
+                    ACC(2, i, j, k, t); ACC(2, i, j, k - 1, t);
+                }
+                /* diffusion: mu/rhof times the (symmetric) periodic Laplacian */
+                for (int d = 0; d < 3; d++) {
+                    const double t = mu * dvb[d][q] * id2 / rhof;
+                    ACC(d, i + 1, j, k, t); ACC(d, i - 1, j, k, t);
+                    ACC(d, i, j + 1, k, t); ACC(d, i, j - 1, k, t);
+                    ACC(d, i, j, k + 1, t); ACC(d, i, j, k - 1, t);
+                    ACC(d, i, j, k, -6.0 * t);
+                }
+            }
+#undef ACC
+    for (int d = 0; d < 3; d++) {
+        memcpy(bar[d], out[d], ni * sizeof(double));
+        free(star[d]); free(dvb[d]); free(out[d]);
+    }
+    free(psi);
+    return 0;
+}

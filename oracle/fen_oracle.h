@@ -80,6 +80,17 @@ void orc_thomas(int n, const double *a, const double *b, const double *cc, doubl
 
 int orc_num_threads(void);
 
+/* Discrete adjoint of one orc_ns_step (3-D, triply periodic, ppp, no source,
+ * fixed dt; no counterpart in the reference).  The context's u, v, w hold the
+ * state u^n the step started from.  In: ub/vb/wb = adjoint of the new
+ * velocity, pb = adjoint of the new pressure, dvob[3] = adjoint of the new
+ * dv_o; out (in place): the adjoints of the step's inputs u^n, p^n, dv_o^n.
+ * All arrays interior-sized (nx*ny*nz).  Returns 0, or -1 when the context
+ * is not of that kind.  Written as the transposed partial derivatives of
+ * each forward expression, scattered term by term, serially. */
+int orc_adjoint_step(orc_ctx *c, double dt, double *ub, double *vb, double *wb, double *pb, double *dvobx,
+                     double *dvoby, double *dvobz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,8 @@ def lib():
         L.orc_fft_c2r.argtypes = [ctypes.c_int, dp, dp, dp]
         L.orc_thomas.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_double, ctypes.c_double, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_adjoint_step.argtypes = [ctypes.c_void_p, ctypes.c_double] + [dp] * 7
+        L.orc_adjoint_step.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -172,6 +174,16 @@ class Oracle:
         r = [np.zeros(self.n, order="F") for _ in range(3)]
         lib().orc_gradient(self.h, _p(s), *[_p(a) for a in r])
         return r
+
+    def adjoint_step(self, dt, ub, vb, wb, pb, dvob):
+        """Adjoint of one step, in place on Fortran-ordered interior arrays;
+        the context's u, v, w must hold the state the step started from."""
+        arrs = [ub, vb, wb, pb] + list(dvob)
+        for a in arrs:
+            assert a.flags["F_CONTIGUOUS"] and a.dtype == np.float64 and a.shape == tuple(self.n)
+        rc = lib().orc_adjoint_step(self.h, dt, *[_p(a) for a in arrs])
+        if rc != 0:
+            raise ValueError("adjoint step needs a 3-D triply periodic context")
 
     def poisson_solve(self, f, gl):
         """solve_poisson in place on a Fortran-ordered array with ghost level gl."""
