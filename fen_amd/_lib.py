@@ -75,6 +75,8 @@ def load():
         "fen_profile_name": (ctypes.c_char_p, [c_int]),
         "fen_profile_read": (c_int, [vp, c_int, dp, ctypes.POINTER(ctypes.c_longlong), dp]),
         "fen_profile_predecessor": (c_int, [vp, c_int, ip]),
+        "fen_adjoint_record": (c_int, [vp, c_int, c_int, c_int, dp]),
+        "fen_adjoint_backward": (c_int, [vp, dp, dp, dp, dp, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

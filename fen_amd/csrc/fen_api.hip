@@ -222,6 +222,7 @@ void *field_ptr(fen_ctx *c, int field)
 
 void free_all(fen_ctx *c)
 {
+    fen::adjoint_free(c);
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     for (int d = 0; d < 3; d++) { f(c->d_u[d]); f(c->d_us[d]); f(c->d_dvo[d]); f(c->d_S[d]); }
     f(c->d_p); f(c->d_phi); f(c->d_spec); f(c->d_c1); f(c->d_dd);
@@ -323,6 +324,8 @@ template <class R> void advance(fen_ctx *c, double dt)
 }
 
 }  // namespace
+
+void fen::advance_fp64(fen_ctx *c, double dt) { advance<double>(c, dt); }
 
 #define FEN_TRY(ctx, ...)                                              \
     try {                                                              \
@@ -835,6 +838,26 @@ int fen_profile_predecessor(fen_ctx *c, int k, int *prev)
         const fen::ProfMark &cur = c->prof_marks[m], &pm = c->prof_marks[m - 1];
         if (cur.kernel == k && pm.seq == cur.seq - 1) { *prev = pm.kernel; break; }
     }
+    return FEN_OK;
+}
+
+int fen_adjoint_record(fen_ctx *c, int step0, int nsteps, int every, double *dt)
+{
+    if (!c || !dt || nsteps < 1 || every < 1) return FEN_ERR_ARG;
+    if (const char *why = fen::adjoint_unsupported(c)) return fail(c, FEN_ERR_UNSUPPORTED, why);
+    if (!c->have_dt_o) return fail(c, FEN_ERR_STATE, "dt_o is undefined: call fen_set_timestep or fen_set_dt_o");
+    FEN_TRY(c, fen::adjoint_record(c, step0, nsteps, every, *dt));
+    return FEN_OK;
+}
+
+int fen_adjoint_backward(fen_ctx *c, const double *seed_u, const double *seed_v, const double *seed_w,
+                         double *grad_u, double *grad_v, double *grad_w, double *grad_p)
+{
+    if (!c || !seed_u || !seed_v || !seed_w || !grad_u || !grad_v || !grad_w) return FEN_ERR_ARG;
+    if (!c->tape) return fail(c, FEN_ERR_STATE, "no recorded run: call fen_adjoint_record first");
+    const double *seed[3] = {seed_u, seed_v, seed_w};
+    double *grad[3] = {grad_u, grad_v, grad_w};
+    FEN_TRY(c, fen::adjoint_backward(c, seed, grad, grad_p));
     return FEN_OK;
 }
 

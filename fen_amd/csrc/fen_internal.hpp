@@ -122,6 +122,8 @@ struct ProfSlot {
     double bytes = 0.0;   // algorithmic HBM bytes per launch (last launch)
 };
 
+struct AdjointTape;
+
 }  // namespace fen
 
 struct fen_ctx {
@@ -178,6 +180,8 @@ struct fen_ctx {
     hipStream_t stream = nullptr;
     std::string err;
 
+    fen::AdjointTape *tape = nullptr;       // recorded run of the discrete adjoint
+
     unsigned prof_mask = 0;
     long long prof_seq = 0;                 // launches issued on the stream
     std::vector<fen::ProfMark> prof_marks;  // chronological
@@ -208,6 +212,15 @@ template <class R> void launch_checks(fen_ctx *c, const R *const *vel, double *o
 long stencil_blocks(const Box &b);   // workgroups of the momentum / projection grids
 template <class R> void launch_apply_shift(fen_ctx *c, R *f, size_t n);
 void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz);   // FP64 contexts
+
+// One FP64 time step (the step fen_ns_step runs, without the CFL update).
+void advance_fp64(fen_ctx *c, double dt);
+
+// Discrete adjoint (adjoint.hip).
+const char *adjoint_unsupported(const fen_ctx *c);   // nullptr when the context is supported
+void adjoint_record(fen_ctx *c, int step0, int nsteps, int every, double dt);
+void adjoint_backward(fen_ctx *c, const double *seed[3], double *grad[3], double *grad_p);
+void adjoint_free(fen_ctx *c);
 
 // Profiling helpers.
 void prof_begin(fen_ctx *c, int k);

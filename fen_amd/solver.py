@@ -10,6 +10,7 @@ tests read like FEN's own programs:
 * ``update_ghost_nodes(field)``   scalar%update_ghost_nodes (src/scalar.f90:223)
 * ``add_advection(R)``            add_advection (src/navier_stokes.f90:261)
 * ``status_line(step, time, dt)`` print_navier_stokes_solver_status (src/navier_stokes.f90:734)
+* ``adjoint_record`` / ``adjoint_backward``  the discrete adjoint (no reference counterpart)
 
 Arrays are numpy float64 in Fortran order with FEN's ghosted shapes.
 """
@@ -151,6 +152,24 @@ class Solver:
         d = ctypes.c_double(dt)
         self._check(self.lib.fen_ns_run(self.h, step0, nsteps, ctypes.byref(d)))
         return d.value
+
+    def adjoint_record(self, step0, nsteps, every, dt):
+        """Run nsteps steps (as run()) keeping checkpoints every `every` steps."""
+        d = ctypes.c_double(dt)
+        self._check(self.lib.fen_adjoint_record(self.h, step0, nsteps, every, ctypes.byref(d)))
+        return d.value
+
+    def adjoint_backward(self, seed, want_p=False):
+        """seed: dL/d(final u, v, w) as interior arrays; returns dL/d(initial u, v, w)
+        (and dL/d(initial p) when want_p)."""
+        s = [np.asfortranarray(a, dtype=np.float64) for a in seed]
+        for a in s:
+            if a.shape != self.n:
+                raise ValueError(f"seed shape {a.shape} != {self.n}")
+        g = [np.zeros(self.n, order="F") for _ in range(4 if want_p else 3)]
+        self._check(self.lib.fen_adjoint_backward(self.h, _dp(s[0]), _dp(s[1]), _dp(s[2]), _dp(g[0]), _dp(g[1]),
+                                                  _dp(g[2]), _dp(g[3]) if want_p else None))
+        return g
 
     def synchronize(self):
         self._check(self.lib.fen_synchronize(self.h))

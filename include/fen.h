@@ -180,6 +180,24 @@ const char *fen_profile_name(int kernel);
 int fen_profile_read(fen_ctx *ctx, int kernel, double *total_ms, long long *launches, double *bytes);
 int fen_profile_predecessor(fen_ctx *ctx, int kernel, int *prev);
 
+/* Discrete adjoint of the time step -- no counterpart in the reference; for
+ * callers that differentiate a run (sensitivities, optimisation).  FP64, 3-D
+ * triply periodic (ppp), no source term, fixed dt (constant_CFL off).
+ *
+ * fen_adjoint_record runs nsteps steps from the resident state, exactly as
+ * fen_ns_run(ctx, step0, nsteps, dt) would, keeping a copy of the state every
+ * `every` steps in device memory (ceil(nsteps/every) + 1 copies of u, v, w, p
+ * and dv_o).  Returns FEN_ERR_UNSUPPORTED for other contexts. */
+int fen_adjoint_record(fen_ctx *ctx, int step0, int nsteps, int every, double *dt);
+
+/* Adjoint of the recorded run: given seed = dL/du(final) (interior arrays of
+ * nx*ny*nz doubles for u, v, w), returns dL/du(initial) in grad_u/v/w and,
+ * if grad_p is not NULL, dL/dp(initial) (interior arrays).  Each segment is
+ * recomputed from its copy; the resident state is left at the end of the
+ * recorded run.  FEN_ERR_STATE if nothing was recorded. */
+int fen_adjoint_backward(fen_ctx *ctx, const double *seed_u, const double *seed_v, const double *seed_w,
+                         double *grad_u, double *grad_v, double *grad_w, double *grad_p);
+
 #ifdef __cplusplus
 }
 #endif
