@@ -77,7 +77,11 @@ template <int R, int SIGN, typename C> __device__ __forceinline__ void dft_any(C
 
 // LDS index with one pad element every 16 to break power-of-two strides.
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
-template <int N> constexpr int lds_line() { return N + N / 16; }
+// Stride between lines in LDS: the padded length plus one element, so that
+// the same position in consecutive lines falls in different banks (N + N/16
+// complex values would be a multiple of 32 banks apart for N >= 32, and the
+// staging loops that walk across lines would conflict).
+template <int N> constexpr int lds_line() { return N + N / 16 + 1; }
 
 // One Stockham stage and the recursion over the remaining ones.  `line`
 // points at this thread's line in LDS (padded indexing), t in [0, N/8).
