@@ -5,13 +5,15 @@ program taylor_green_2d
     ! pattern of the reference's test/small_test/navier_stokes/taylor_green_vortex.
     ! Doubly periodic [0,2pi]^2, u = sin x cos y, v = -cos x sin y,
     ! p = -(cos 2x + cos 2y)/4, nu = 0.01, dt = 1e-3.
-    ! Usage: taylor_green_2d [N] [nsteps]   -> u.raw v.raw p.raw
+    ! Usage: taylor_green_2d [N] [nsteps]   -> u.raw v.raw p.raw, and
+    ! save_fields' data/vx_<nsteps>.raw, vy_, p_ when a data/ folder exists.
 
     use IO_mod
     use precision_mod     , only : dp
     use global_mod        , only : pi
     use grid_mod
-    use solver_mod        , only : init_solver, advance_solution, destroy_solver, print_solver_status
+    use solver_mod        , only : init_solver, advance_solution, destroy_solver, print_solver_status, &
+                                   save_fields
     use navier_stokes_mod , only : v, p, viscosity, dt_o
 
     implicit none
@@ -21,6 +23,7 @@ program taylor_green_2d
     type(grid)       :: comp_grid
     type(bc_type)    :: bc(4)
     character(len=32):: arg
+    logical          :: have_data
 
     Nx = 128
     nsteps = 1000
@@ -61,6 +64,8 @@ program taylor_green_2d
     call v%x%write('u.raw')
     call v%y%write('v.raw')
     call p%write('p.raw')
+    inquire(file = 'data/.', exist = have_data)
+    if (have_data) call save_fields(nsteps)
 
     call destroy_solver()
     call comp_grid%destroy

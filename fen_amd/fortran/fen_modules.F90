@@ -548,6 +548,28 @@ contains
             end do
         end do
     end subroutine divergence_of_vector
+
+    !> Face value averaged to cell centres, 0.5*(f(i) + f(i-1)) along `face`
+    !> (reference src/fields.f90:210-250).
+    subroutine face_to_center(sf, sc, face)
+        type(scalar), intent(in) :: sf
+        type(scalar), intent(inout) :: sc
+        character(len=1), intent(in) :: face
+        integer :: i, j, k, di, dj, dk
+        di = 0; dj = 0; dk = 0
+        select case(face)
+            case('x'); di = 1
+            case('y'); dj = 1
+            case('z'); dk = 1
+        end select
+        do k = sc%G%lo(3), sc%G%hi(3)
+            do j = sc%G%lo(2), sc%G%hi(2)
+                do i = sc%G%lo(1), sc%G%hi(1)
+                    sc%f(i,j,k) = 0.5_dp*(sf%f(i,j,k) + sf%f(i-di,j-dj,k-dk))
+                end do
+            end do
+        end do
+    end subroutine face_to_center
 end module fields_mod
 
 !===================================================================================================
@@ -968,6 +990,30 @@ contains
         advance_solution => navier_stokes_solver
         print_solver_status => print_navier_stokes_solver_status
     end subroutine init_solver
+
+    !> Cell-centred fields data/vx_<step>.raw, vy_, [vz_,] p_ (raw float64
+    !> interiors; reference src/solver.f90:103-156).
+    subroutine save_fields(step)
+        use scalar_mod, only : scalar
+        use fields_mod, only : face_to_center
+        use navier_stokes_mod, only : v, p, fen_sync_host
+        integer, intent(in) :: step
+        type(scalar) :: temp
+        character(len=7) :: sn
+        call fen_sync_host()
+        call temp%allocate(v%G, v%x%gl)
+        write(sn, '(I0.7)') step
+        call face_to_center(v%x, temp, 'x')
+        call temp%write('data/vx_'//sn//'.raw')
+        call face_to_center(v%y, temp, 'y')
+        call temp%write('data/vy_'//sn//'.raw')
+#if DIM==3
+        call face_to_center(v%z, temp, 'z')
+        call temp%write('data/vz_'//sn//'.raw')
+#endif
+        call p%write('data/p_'//sn//'.raw')
+        call temp%destroy()
+    end subroutine save_fields
 
     !> Restart file data/state_<step>.raw: p, u, v, dv_o%x, dv_o%y [, w, dv_o%z],
     !> each the raw float64 interior in Fortran order (src/solver.f90:160-240).

@@ -36,6 +36,7 @@ def _bin(name):
 @pytest.mark.gpu
 def test_fortran_taylor_green_2d_matches_oracle(tmp_path):
     N, steps = 128, 1000
+    (tmp_path / "data").mkdir()
     out = subprocess.run([_bin("taylor_green_2d"), str(N), str(steps)], cwd=tmp_path, check=True,
                          capture_output=True, text=True, timeout=120).stdout
     # print_navier_stokes_solver_status format (src/navier_stokes.f90:746)
@@ -53,6 +54,13 @@ def test_fortran_taylor_green_2d_matches_oracle(tmp_path):
     for name, fid in (("u", U), ("v", V), ("p", P)):
         got = np.fromfile(tmp_path / f"{name}.raw").reshape((N, N), order="F")
         ref = o.field(fid)[1:-1, 1:-1, 1]
+        assert rel_err(got, ref) < 1e-12, name
+    # save_fields: face values averaged to centres, 0.5*(f(i) + f(i-1)) (src/fields.f90:210-250)
+    u, v = o.field(U)[:, :, 1], o.field(V)[:, :, 1]
+    centred = {"vx": 0.5 * (u[1:-1, 1:-1] + u[:-2, 1:-1]), "vy": 0.5 * (v[1:-1, 1:-1] + v[1:-1, :-2]),
+               "p": o.field(P)[1:-1, 1:-1, 1]}
+    for name, ref in centred.items():
+        got = np.fromfile(tmp_path / "data" / f"{name}_{steps:07d}.raw").reshape((N, N), order="F")
         assert rel_err(got, ref) < 1e-12, name
 
 
