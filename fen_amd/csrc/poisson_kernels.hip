@@ -34,9 +34,10 @@ template <class R> struct DivR2CArgs {
     int direct;   // 1: `u` holds phi itself (interior values, strides sx/sy, ghost gl)
     long sx, sy;
     int gl;
+    int ny_shift;   // log2(ny)
 };
 
-template <class R, int D3, int N>
+template <class R, int D3, int N, int DIRECT>
 __global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs<R> a)
 {
     using C = typename Cplx<R>::type;
@@ -45,26 +46,41 @@ __global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs<R> a)
     __shared__ C tw[N];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
-    const int ny = a.b.ny;
-    for (int e = threadIdx.x; e < TJ * N; e += kBlock) {
-        const int row = e / N, i = e % N + 1;
-        const long r = row0 + row;
-        double val = 0.0;
-        if (r < a.nrows) {
-            const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
-            if (a.direct) {
-                val = a.u[(i - 1 + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
+    // Staging: every thread owns PER elements; their global loads are issued
+    // in batches before any LDS write, so a batch costs one memory latency.
+    constexpr int PER = TJ * N / kBlock, BATCH = 8;
+    static_assert(PER % BATCH == 0, "staging batches");
+#pragma unroll
+    for (int b0 = 0; b0 < PER; b0 += BATCH) {
+        double val[BATCH];
+#pragma unroll
+        for (int m = 0; m < BATCH; m++) {
+            const int e = threadIdx.x + (b0 + m) * kBlock;
+            const int row = e / N, i = e % N + 1;
+            const long r = row0 + row;
+            const bool ok = r < a.nrows;
+            const long rr = ok ? r : row0;   // a valid row: the load is kept, its value dropped
+            const int j = (int)(rr & (a.b.ny - 1)) + 1, k = (int)(rr >> a.ny_shift) + 1;
+            double v;
+            if constexpr (DIRECT) {
+                v = a.u[(i - 1 + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
             } else {
                 const long g = a.b.g(i, j, k);
                 double d = ((double)a.u[g] - (double)a.u[g - 1]) * a.id +
                            ((double)a.v[g] - (double)a.v[g - a.b.gx]) * a.id;
                 if constexpr (D3) d = d + ((double)a.w[g] - (double)a.w[g - a.b.gxy]) * a.id;
-                val = d * a.rho / a.dt;
+                v = d * a.rho / a.dt;
             }
+            val[m] = ok ? v : 0.0;
         }
-        C *line = buf + (row >> 1) * LL;
-        if (row & 1) line[lds_pad(i - 1)].y = (R)val;
-        else line[lds_pad(i - 1)].x = (R)val;
+#pragma unroll
+        for (int m = 0; m < BATCH; m++) {
+            const int e = threadIdx.x + (b0 + m) * kBlock;
+            const int row = e / N, i = e % N + 1;
+            C *line = buf + (row >> 1) * LL;
+            if (row & 1) line[lds_pad(i - 1)].y = (R)val[m];
+            else line[lds_pad(i - 1)].x = (R)val[m];
+        }
     }
     __syncthreads();
     fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
@@ -94,6 +110,7 @@ template <class R> struct C2RArgs {
     R *out;
     long nrows, kxs, sx, sy;
     int ny, gl, want_sum;
+    int ny_shift;   // log2(ny)
     double *part;
     Box b;     // ghosted box of `out` when img != 0
     int img;   // periodic image axes to write (phi in the time step)
@@ -109,18 +126,32 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs<R> a)
     __shared__ double red[kBlock];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
-    for (int e = threadIdx.x; e < L * (N / 2 + 1); e += kBlock) {
-        const int l = e % L, kx = e / L;
+    // Staging: all of a thread's spectrum loads are issued before its LDS writes.
+    constexpr int NE = L * (N / 2 + 1), PER = (NE + kBlock - 1) / kBlock;
+    C X1[PER], X2[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kBlock;
+        const int ee = e < NE ? e : 0;
+        const int l = ee % L, kx = ee / L;
         const long r = row0 + 2 * l;
-        C X1 = C{R(0), R(0)}, X2 = C{R(0), R(0)};
-        if (r < a.nrows) X1 = a.spec[kx * a.kxs + r];
-        if (r + 1 < a.nrows) X2 = a.spec[kx * a.kxs + r + 1];
+        const bool ok1 = e < NE && r < a.nrows, ok2 = e < NE && r + 1 < a.nrows;
+        const C v1 = a.spec[kx * a.kxs + (ok1 ? r : row0)];
+        const C v2 = a.spec[kx * a.kxs + (ok2 ? r + 1 : row0)];
+        X1[m] = ok1 ? v1 : C{R(0), R(0)};
+        X2[m] = ok2 ? v2 : C{R(0), R(0)};
+    }
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kBlock;
+        if (e >= NE) continue;
+        const int l = e % L, kx = e / L;
         C *line = buf + l * LL;
         if (kx == 0 || kx == N / 2) {
-            line[lds_pad(kx)] = C{X1.x, X2.x};   // Im of bins 0 and n/2 ignored
+            line[lds_pad(kx)] = C{X1[m].x, X2[m].x};   // Im of bins 0 and n/2 ignored
         } else {
-            line[lds_pad(kx)] = C{X1.x - X2.y, X1.y + X2.x};
-            line[lds_pad(N - kx)] = C{X1.x + X2.y, X2.x - X1.y};
+            line[lds_pad(kx)] = C{X1[m].x - X2[m].y, X1[m].y + X2[m].x};
+            line[lds_pad(N - kx)] = C{X1[m].x + X2[m].y, X2[m].x - X1[m].y};
         }
     }
     __syncthreads();
@@ -132,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs<R> a)
         if (r >= a.nrows) continue;
         const C z = buf[(row >> 1) * LL + lds_pad(i)];
         const R val = (row & 1) ? z.y : z.x;
-        const long j = r % a.ny, k = r / a.ny;
+        const long j = r & (a.ny - 1), k = r >> a.ny_shift;
         a.out[(i + a.gl) + a.sx * (j + a.gl) + a.sy * (k + a.gl)] = val;
         if (a.img) write_images(a.out, a.b, a.img, i + 1, (int)j + 1, (int)k + 1, val);
         acc += (double)val;
@@ -183,10 +214,20 @@ __global__ __launch_bounds__(kBlock) void y_kernel(YArgs<R> a)
     __shared__ C tw[N];
     load_twiddles<N>(tw, a.tw);
     const long line0 = (long)blockIdx.x * L;
-    for (int e = threadIdx.x; e < L * N; e += kBlock) {
+    constexpr int PER = L * N / kBlock;
+    C v[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kBlock;
         const int l = e / N, j = e % N;
         const long ln = line0 + l;
-        buf[l * LL + lds_pad(j)] = ln < a.nlines ? a.spec[ln * N + j] : C{R(0), R(0)};
+        const C x = a.spec[(ln < a.nlines ? ln : line0) * N + j];
+        v[m] = ln < a.nlines ? x : C{R(0), R(0)};
+    }
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kBlock;
+        buf[(e / N) * LL + lds_pad(e % N)] = v[m];
     }
     __syncthreads();
     C *line = buf + (threadIdx.x / T) * LL;
@@ -242,9 +283,19 @@ __global__ __launch_bounds__(kBlock) void z_solve_kernel(ZArgs<R> a)
     const int kx = blockIdx.y;
     const int j0 = blockIdx.x * L;
     C *base = a.spec + kx * a.kxs + j0;
-    for (int e = threadIdx.x; e < L * N; e += kBlock) {
+    constexpr int PER = L * N / kBlock;
+    C v[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kBlock;
         const int l = e % L, k = e / L;
-        buf[l * LL + lds_pad(k)] = (j0 + l < a.ny) ? base[(long)k * a.ny + l] : C{R(0), R(0)};
+        const C x = base[(long)k * a.ny + (j0 + l < a.ny ? l : 0)];
+        v[m] = (j0 + l < a.ny) ? x : C{R(0), R(0)};
+    }
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kBlock;
+        buf[(e % L) * LL + lds_pad(e / L)] = v[m];
     }
     __syncthreads();
     C *line = buf + (threadIdx.x / T) * LL;
@@ -380,6 +431,13 @@ template <class R> __global__ void apply_shift_kernel(R *f, long n, const double
     default: throw std::runtime_error("unsupported transform length"); \
     }
 
+static int ilog2(int n)   // n a power of two
+{
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return l;
+}
+
 template <class R> static double spec_bytes(const fen_ctx *c)
 {
     return (double)c->hx * c->ny * c->nz * 2.0 * sizeof(R);
@@ -404,13 +462,15 @@ void launch_div_r2c_impl(fen_ctx *c, const R *const *vel, const R *direct_src, l
     a.post = post;
     a.nrows = (long)c->ny * c->nz;
     a.kxs = (long)c->ny * c->nz;
+    a.ny_shift = ilog2(c->ny);
     const int tj = 2 * (kBlock / (c->nx / 8));
     const dim3 grid((unsigned)((a.nrows + tj - 1) / tj));
     const double bytes = (vel ? (c->dim == 3 ? 3.0 : 2.0) : 1.0) * sizeof(R) * c->n_int + spec_bytes<R>(c);
     prof_begin(c, K_DIV_R2C);
-#define L_DIV(NN)                                                                                     \
-    if (c->dim == 3) hipLaunchKernelGGL((div_r2c_kernel<R, 1, NN>), grid, dim3(kBlock), 0, c->stream, a); \
-    else hipLaunchKernelGGL((div_r2c_kernel<R, 0, NN>), grid, dim3(kBlock), 0, c->stream, a);
+#define L_DIV(NN)                                                                                        \
+    if (a.direct) hipLaunchKernelGGL((div_r2c_kernel<R, 0, NN, 1>), grid, dim3(kBlock), 0, c->stream, a);    \
+    else if (c->dim == 3) hipLaunchKernelGGL((div_r2c_kernel<R, 1, NN, 0>), grid, dim3(kBlock), 0, c->stream, a); \
+    else hipLaunchKernelGGL((div_r2c_kernel<R, 0, NN, 0>), grid, dim3(kBlock), 0, c->stream, a);
     FEN_DISPATCH_N(c->nx, L_DIV)
 #undef L_DIV
     prof_end(c, K_DIV_R2C, bytes);
@@ -514,6 +574,7 @@ template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long
     a.sy = sy;
     a.ny = c->ny;
     a.gl = gl_out;
+    a.ny_shift = ilog2(c->ny);
     a.want_sum = want_sum ? 1 : 0;
     a.part = c->d_part;
     a.b = c->box;
