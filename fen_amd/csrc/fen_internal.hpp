@@ -82,6 +82,35 @@ __device__ __forceinline__ void write_images(R *f, const Box &b, int ax, int i, 
     if (ii >= 0 && jj >= 0 && kk >= 0) f[b.g(ii, jj, kk)] = v;
 }
 
+// write_images() for callers whose j and k are the same across a wave (a
+// k-marching tile row, an x line of the c2r): only the x test varies across
+// lanes; `xedge` says whether the wave can hold i == 1 or i == nx.  Writes
+// exactly the cells write_images() writes.
+template <class R>
+__device__ __forceinline__ void march_images(R *f, const Box &b, int ax, int i, int j, int k, R v, bool xedge)
+{
+    const bool xi = (ax & 1) && xedge && (i == 1 || i == b.nx);
+    const int ii = i == 1 ? b.nx + 1 : 0;
+    if (xi) f[b.g(ii, j, k)] = v;
+    const bool yj = (ax & 2) && (j == 1 || j == b.ny);
+    const bool zk = (ax & 4) && (k == 1 || k == b.nz);
+    if (yj || zk) {
+        const int jj = j == 1 ? b.ny + 1 : 0, kk = k == 1 ? b.nz + 1 : 0;
+        if (yj) {
+            f[b.g(i, jj, k)] = v;
+            if (xi) f[b.g(ii, jj, k)] = v;
+        }
+        if (zk) {
+            f[b.g(i, j, kk)] = v;
+            if (xi) f[b.g(ii, j, kk)] = v;
+            if (yj) {
+                f[b.g(i, jj, kk)] = v;
+                if (xi) f[b.g(ii, jj, kk)] = v;
+            }
+        }
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; this
 // renumbering gives every XCD one contiguous range of the grid (a k-slab),
 // so the stencil neighbours a workgroup needs are in its own XCD's L2.

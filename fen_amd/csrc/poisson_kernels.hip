@@ -121,9 +121,9 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs<R> a)
 {
     using C = typename Cplx<R>::type;
     constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), TJ = 2 * L, LL = lds_line<N>();
+    static_assert(L * LL * sizeof(C) >= kBlock * sizeof(double), "the reduction reuses buf");
     __shared__ C buf[L * LL];
     __shared__ C tw[N];
-    __shared__ double red[kBlock];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
     // Staging: all of a thread's spectrum loads are issued before its LDS writes.
@@ -158,17 +158,23 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs<R> a)
     fft_lds<N, 1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
     double acc = 0.0;
     for (int e = threadIdx.x; e < TJ * N; e += kBlock) {
-        const int row = e / N, i = e % N;
+        // for N >= 64 a wave writes part of one x line: its row is wave-uniform
+        const int row = N >= 64 ? __builtin_amdgcn_readfirstlane(e / N) : e / N, i = e % N;
         const long r = row0 + row;
         if (r >= a.nrows) continue;
         const C z = buf[(row >> 1) * LL + lds_pad(i)];
         const R val = (row & 1) ? z.y : z.x;
-        const long j = r & (a.ny - 1), k = r >> a.ny_shift;
+        const int j = (int)(r & (a.ny - 1)), k = (int)(r >> a.ny_shift);
         a.out[(i + a.gl) + a.sx * (j + a.gl) + a.sy * (k + a.gl)] = val;
-        if (a.img) write_images(a.out, a.b, a.img, i + 1, (int)j + 1, (int)k + 1, val);
+        if (a.img) {
+            if constexpr (N >= 64) march_images(a.out, a.b, a.img, i + 1, j + 1, k + 1, val, true);
+            else write_images(a.out, a.b, a.img, i + 1, j + 1, k + 1, val);
+        }
         acc += (double)val;
     }
     if (a.want_sum) {
+        __syncthreads();                     // every read of buf is done: reuse it
+        double *red = reinterpret_cast<double *>(buf);
         red[threadIdx.x] = acc;
         __syncthreads();
         for (int s = kBlock / 2; s > 0; s >>= 1) {

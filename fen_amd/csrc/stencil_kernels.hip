@@ -215,15 +215,16 @@ __global__ __launch_bounds__(kThreads) void momentum_kernel_2d(MomArgs<R> a)
 // 3-D launch shapes (see stencil_march.hpp), chosen per element type by a
 // sweep of tile height, planes per workgroup, ring unrolling and prefetch
 // depth on MI355X at 256^3.
-// (tile height TY, planes per workgroup KB, ring unrolling UN, prefetch depth PF)
+// (tile height TY, planes per workgroup KB, ring unrolling UN, prefetch depth
+// PF, minimum waves per SIMD for the register allocation OCC)
 template <class R> struct Shape;
 template <> struct Shape<double> {
-    static constexpr int MTY = 8, MKB = 32, MUN = 1, MPF = 1;   // momentum
-    static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1;   // projection
+    static constexpr int MTY = 8, MKB = 32, MUN = 1, MPF = 1, MOCC = 1;   // momentum
+    static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 1;   // projection
 };
 template <> struct Shape<float> {
-    static constexpr int MTY = 4, MKB = 16, MUN = 2, MPF = 2;
-    static constexpr int PTY = 4, PKB = 32, PUN = 3, PPF = 1;
+    static constexpr int MTY = 4, MKB = 16, MUN = 2, MPF = 2, MOCC = 1;
+    static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 4;
 };
 constexpr int MTX = 64;
 
@@ -277,10 +278,10 @@ template <class R> void launch_momentum(fen_ctx *c, double dt, double A, double 
     if (d3) {
         const dim3 grid((unsigned)(a.nbx * a.nby * nbz));
         if (c->has_source)
-            hipLaunchKernelGGL((momentum_march_kernel<R, 1, SH::MTY, SH::MKB, SH::MUN, SH::MPF>), grid,
+            hipLaunchKernelGGL((momentum_march_kernel<R, 1, SH::MTY, SH::MKB, SH::MUN, SH::MPF, SH::MOCC>), grid,
                                dim3(64 * SH::MTY), 0, c->stream, a);
         else
-            hipLaunchKernelGGL((momentum_march_kernel<R, 0, SH::MTY, SH::MKB, SH::MUN, SH::MPF>), grid,
+            hipLaunchKernelGGL((momentum_march_kernel<R, 0, SH::MTY, SH::MKB, SH::MUN, SH::MPF, SH::MOCC>), grid,
                                dim3(64 * SH::MTY), 0, c->stream, a);
     } else {
         const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
@@ -426,10 +427,10 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     if (d3) {
         const dim3 grid((unsigned)nblk);
         if (sh)
-            hipLaunchKernelGGL((project_march_kernel<R, 1, SH::PTY, SH::PKB, SH::PUN, SH::PPF>), grid,
+            hipLaunchKernelGGL((project_march_kernel<R, 1, SH::PTY, SH::PKB, SH::PUN, SH::PPF, SH::POCC>), grid,
                                dim3(64 * SH::PTY), 0, c->stream, a);
         else
-            hipLaunchKernelGGL((project_march_kernel<R, 0, SH::PTY, SH::PKB, SH::PUN, SH::PPF>), grid,
+            hipLaunchKernelGGL((project_march_kernel<R, 0, SH::PTY, SH::PKB, SH::PUN, SH::PPF, SH::POCC>), grid,
                                dim3(64 * SH::PTY), 0, c->stream, a);
     } else {
         const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);

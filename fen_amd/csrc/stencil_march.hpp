@@ -82,35 +82,6 @@ __device__ __forceinline__ void tile_commit(R *lds, SL slot, const TileRegs<S, N
         }
 }
 
-// Periodic images of cell (i, j, k) for the k-marching kernels: a wave is
-// one j row and k is the same for the whole workgroup, so only the x test
-// varies across lanes; `xedge` says whether the workgroup holds i == 1 or
-// i == nx.  Writes exactly the cells write_images() writes.
-template <class R>
-__device__ __forceinline__ void march_images(R *f, const Box &b, int ax, int i, int j, int k, R v, bool xedge)
-{
-    const bool xi = (ax & 1) && xedge && (i == 1 || i == b.nx);
-    const int ii = i == 1 ? b.nx + 1 : 0;
-    if (xi) f[b.g(ii, j, k)] = v;
-    const bool yj = (ax & 2) && (j == 1 || j == b.ny);
-    const bool zk = (ax & 4) && (k == 1 || k == b.nz);
-    if (yj || zk) {
-        const int jj = j == 1 ? b.ny + 1 : 0, kk = k == 1 ? b.nz + 1 : 0;
-        if (yj) {
-            f[b.g(i, jj, k)] = v;
-            if (xi) f[b.g(ii, jj, k)] = v;
-        }
-        if (zk) {
-            f[b.g(i, j, kk)] = v;
-            if (xi) f[b.g(ii, j, kk)] = v;
-            if (yj) {
-                f[b.g(i, jj, kk)] = v;
-                if (xi) f[b.g(ii, jj, kk)] = v;
-            }
-        }
-    }
-}
-
 template <int V> using IC = std::integral_constant<int, V>;
 
 // ----------------------------------------------------------------- momentum
@@ -211,8 +182,9 @@ struct MomentumMarch {
     }
 };
 
-template <class R, int SRC, int TY, int KB, int UNROLL, int PF = 1>
-__global__ __launch_bounds__(64 * TY) void momentum_march_kernel(MomArgs<R> a)
+// OCC: minimum waves per SIMD the register allocation must allow (1: no limit).
+template <class R, int SRC, int TY, int KB, int UNROLL, int PF = 1, int OCC = 1>
+__global__ __launch_bounds__(64 * TY, OCC) void momentum_march_kernel(MomArgs<R> a)
 {
     __shared__ R lds[ring_size<TileShape<TY>, 4>()];
     MomentumMarch<R, SRC, TY, KB, UNROLL, PF>::run(a, lds);
@@ -333,8 +305,8 @@ struct ProjectMarch {
     }
 };
 
-template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1>
-__global__ __launch_bounds__(64 * TY) void project_march_kernel(ProjArgs<R> a)
+template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1, int OCC = 1>
+__global__ __launch_bounds__(64 * TY, OCC) void project_march_kernel(ProjArgs<R> a)
 {
     __shared__ R lds[ring_size<TileShape<TY>, 3>()];
     ProjectMarch<R, SHIFT, TY, KB, UNROLL, PF>::run(a, lds);
