@@ -10,7 +10,7 @@ Prints one JSON line.  `value` is cell-updates per second (interior cells
 advanced one full step per second) with all fields resident in HBM.
 `roofline` is the dominant kernel's algorithmic HBM bytes per launch over
 its average duration, timed with HIP events recorded on the library's own
-stream inside the timed region; `step_roofline` applies the same
+stream around that kernel on every tenth step of the timed region; `step_roofline` applies the same
 algorithmic model (37 words per cell per step) to the whole step.
 `fp32_refine` repeats the measurement in the FP32 form (FP32 fields,
 stencils and FFTs; FP64 divergence residual, projection update and
@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 WORDS_3D = 37           # algorithmic words per cell per step (3-D, ppp)
+SAMPLE = 10             # the dominant kernel is timed on every SAMPLE-th step of the timed region
 
 
 def read_traffic(kernel, workload):
@@ -119,13 +120,14 @@ def measure(N, steps, warmup, precision):
     # the longest launch (each stencil/FFT kernel runs once per step)
     dominant = max(prof, key=lambda k: prof[k]["ms"] / prof[k]["launches"])
     breakdown = {k: round(1e3 * v["ms"] / v["launches"], 2) for k, v in prof.items()}   # us per launch
-    pred = g.profile_predecessor(dominant)
     g.profile([])
 
     step0 = 3 + warmup
-    # timed region: one event after the dominant kernel and one after the
-    # launch preceding it, so its duration is measured on the library's stream
-    g.profile([dominant] + ([pred] if pred and pred != dominant else []))
+    # timed region: every SAMPLE-th step, one event just before and one just
+    # after the dominant kernel on the library's stream (an event drains the
+    # stream, so the other steps run without them)
+    g.profile([dominant])
+    g.profile_sample(SAMPLE)
     g.profile_reset()
     g.synchronize()
     t0 = time.perf_counter()
@@ -133,6 +135,7 @@ def measure(N, steps, warmup, precision):
     g.synchronize()
     t1 = time.perf_counter()
     kp = g.profile_read()[dominant]
+    g.profile_sample(1)
     g.profile([])
     sc = g.scalars()
 
@@ -184,7 +187,7 @@ def fp32_accuracy(N, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="grid size N (N^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -70,6 +70,7 @@ def load():
         "fen_grid_delta": (c_int, [vp, dp]),
         "fen_synchronize": (c_int, [vp]),
         "fen_profile_enable": (c_int, [vp, c_int]),
+        "fen_profile_sample": (c_int, [vp, c_int]),
         "fen_profile_reset": (c_int, [vp]),
         "fen_profile_count": (c_int, []),
         "fen_profile_name": (ctypes.c_char_p, [c_int]),

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -23,20 +24,35 @@ using fen::as;
 
 namespace fen {
 
-void prof_begin(fen_ctx *c, int k) { (void)c; (void)k; }
-
-void prof_end(fen_ctx *c, int k, double bytes)
+static hipEvent_t prof_event(fen_ctx *c)
 {
-    const long long seq = c->prof_seq++;
-    if (!(c->prof_mask & (1u << k))) return;
     if (c->prof_used == c->prof_pool.size()) {
         hipEvent_t e;
         FEN_HIP_CHECK(hipEventCreate(&e));
         c->prof_pool.push_back(e);
     }
-    hipEvent_t e = c->prof_pool[c->prof_used++];
+    return c->prof_pool[c->prof_used++];
+}
+
+static bool prof_sampled(const fen_ctx *c) { return c->step_count % c->prof_every == 0; }
+
+// Sampled mode only: an event just before the launch of a selected kernel.
+void prof_begin(fen_ctx *c, int k)
+{
+    if (c->prof_every <= 1 || !(c->prof_mask & (1u << k)) || !prof_sampled(c)) return;
+    hipEvent_t e = prof_event(c);
     FEN_HIP_CHECK(hipEventRecord(e, c->stream));
-    c->prof_marks.push_back(ProfMark{k, seq, e});
+    c->prof_marks.push_back(ProfMark{k, c->prof_seq, e, true});
+}
+
+void prof_end(fen_ctx *c, int k, double bytes)
+{
+    const long long seq = c->prof_seq++;
+    if (!(c->prof_mask & (1u << k))) return;
+    if (c->prof_every > 1 && !prof_sampled(c)) return;
+    hipEvent_t e = prof_event(c);
+    FEN_HIP_CHECK(hipEventRecord(e, c->stream));
+    c->prof_marks.push_back(ProfMark{k, seq, e, false});
     c->prof[k].bytes = bytes;
 }
 }  // namespace fen
@@ -223,6 +239,7 @@ void *field_ptr(fen_ctx *c, int field)
 void free_all(fen_ctx *c)
 {
     fen::adjoint_free(c);
+    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     for (int d = 0; d < 3; d++) { f(c->d_u[d]); f(c->d_us[d]); f(c->d_dvo[d]); f(c->d_S[d]); }
     f(c->d_p); f(c->d_phi); f(c->d_spec); f(c->d_c1); f(c->d_dd);
@@ -269,6 +286,7 @@ template <class R> void advance(fen_ctx *c, double dt)
     const double A = 1.0 + 0.5 * dt / c->dt_o;          // src/navier_stokes.f90:157-158
     const double B = -(0.5 * dt / c->dt_o);
     c->last_dt = dt;
+    c->step_count++;
 
     launch_momentum<R>(c, dt, A, B);
     if (const int rest = remaining_groups(c, c->img_vel)) {
@@ -327,6 +345,49 @@ template <class R> void advance(fen_ctx *c, double dt)
 
 void fen::advance_fp64(fen_ctx *c, double dt) { advance<double>(c, dt); }
 
+namespace {
+
+// One step through the cached graph, capturing it first if dt, dt_o or the
+// configuration changed.  The host-side effects of a step (last_dt, the
+// pending-shift flag, checks_valid) are those advance() leaves, which do not
+// depend on the state before the step.
+void advance_graph(fen_ctx *c, double dt)
+{
+    if (!c->step_graph || c->graph_dt != dt || c->graph_dt_o != c->dt_o || c->graph_version != c->config_version) {
+        if (c->step_graph) {
+            FEN_HIP_CHECK(hipGraphExecDestroy(c->step_graph));
+            c->step_graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        FEN_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            with_type(c, [&](auto tag) { advance<decltype(tag)>(c, dt); });
+        } catch (...) {
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        FEN_HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FEN_HIP_CHECK(e);
+        c->graph_dt = dt;
+        c->graph_dt_o = c->dt_o;
+        c->graph_version = c->config_version;
+    } else {
+        c->last_dt = dt;
+        c->step_count++;
+        const bool mean = c->solver == FEN_SOLVER_PP || c->solver == FEN_SOLVER_PPN;
+        bool dirichlet = false;
+        for (int f = 0; f < 6; f++) dirichlet = dirichlet || c->bc_type[FEN_PHI][f] == 1;
+        c->shift_active = mean && !dirichlet;
+        c->checks_valid = true;
+    }
+    FEN_HIP_CHECK(hipGraphLaunch(c->step_graph, c->stream));
+}
+
+}  // namespace
+
 #define FEN_TRY(ctx, ...)                                              \
     try {                                                              \
         __VA_ARGS__;                                                   \
@@ -351,6 +412,7 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
     for (int d = 0; d < 3; d++) c->x0[d] = x0 ? x0[d] : 0.0;
     for (int s = 0; s < 6; s++) c->bc_phys[s] = (dim == 3 || s < 4) ? bc[s] : FEN_BC_PERIODIC;
     c->precision = precision;
+    if (const char *g = std::getenv("FEN_STEP_GRAPH")) c->use_graph = std::atoi(g) != 0;
     c->delta = lx / (double)(float)nx;                                   // src/grid.f90:140
     c->periodic[0] = c->bc_phys[0] == FEN_BC_PERIODIC && c->bc_phys[1] == FEN_BC_PERIODIC;
     c->periodic[1] = c->bc_phys[2] == FEN_BC_PERIODIC && c->bc_phys[3] == FEN_BC_PERIODIC;
@@ -457,6 +519,7 @@ int fen_set_params(fen_ctx *c, double density, double viscosity, const double *g
     for (int d = 0; d < 3; d++) c->g[d] = g ? g[d] : 0.0;
     c->cfl = cfl;
     c->constant_cfl = constant_cfl;
+    c->config_version++;
     return FEN_OK;
 }
 
@@ -522,7 +585,10 @@ int fen_upload(fen_ctx *c, int field, const double *host, size_t count)
         if (field >= FEN_S_X && !c->d_S[0]) {
             for (int d = 0; d < 3; d++) c->d_S[d] = dev_alloc_bytes(c->n_int * elem(c));
         }
-        if (field >= FEN_S_X) c->has_source = true;
+        if (field >= FEN_S_X && !c->has_source) {
+            c->has_source = true;
+            c->config_version++;
+        }
         copy_in(c, field_ptr(c, field), host, n);
         if (field == FEN_PHI) c->shift_active = false;
         if (field >= FEN_U && field <= FEN_W) c->checks_valid = false;
@@ -574,6 +640,7 @@ int fen_set_bc_type(fen_ctx *c, int field, int face, int type)
     if (s < 0 || face < 0 || face > 5 || type < 0 || type > 2) return fail(c, FEN_ERR_ARG, "bad field, face or type");
     c->bc_type[s][face] = type;
     refresh_images(c);
+    c->config_version++;
     return FEN_OK;
 }
 
@@ -631,7 +698,8 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
             if (*dt > 1.1 * c->dt_o) *dt = 1.1 * c->dt_o;
         }
         const double dtv = *dt;
-        with_type(c, [&](auto tag) { advance<decltype(tag)>(c, dtv); });
+        if (c->use_graph && !c->constant_cfl && !c->prof_mask) advance_graph(c, dtv);
+        else with_type(c, [&](auto tag) { advance<decltype(tag)>(c, dtv); });
         FEN_HIP_CHECK(hipGetLastError());
     });
     if (maxdiv || maxcfl) return fen_get_scalars(c, maxdiv, maxcfl, nullptr, nullptr, nullptr);
@@ -793,6 +861,13 @@ int fen_profile_enable(fen_ctx *c, int mask)
     return FEN_OK;
 }
 
+int fen_profile_sample(fen_ctx *c, int every)
+{
+    if (!c || every < 1) return FEN_ERR_ARG;
+    c->prof_every = every;
+    return FEN_OK;
+}
+
 int fen_profile_reset(fen_ctx *c)
 {
     if (!c) return FEN_ERR_ARG;
@@ -817,7 +892,10 @@ int fen_profile_read(fen_ctx *c, int k, double *total_ms, long long *launches, d
         long long n = 0;
         for (size_t m = 1; m < c->prof_marks.size(); m++) {
             const fen::ProfMark &cur = c->prof_marks[m], &prev = c->prof_marks[m - 1];
-            if (cur.kernel != k || prev.seq != cur.seq - 1) continue;
+            if (cur.kernel != k || cur.begin) continue;
+            const bool pair = prev.begin && prev.seq == cur.seq;             // sampled mode
+            const bool chain = !prev.begin && prev.seq == cur.seq - 1;       // every launch
+            if (!pair && !chain) continue;
             float e = 0.0f;
             FEN_HIP_CHECK(hipEventElapsedTime(&e, prev.ev, cur.ev));
             ms += e;
@@ -836,7 +914,7 @@ int fen_profile_predecessor(fen_ctx *c, int k, int *prev)
     *prev = -1;
     for (size_t m = 1; m < c->prof_marks.size(); m++) {
         const fen::ProfMark &cur = c->prof_marks[m], &pm = c->prof_marks[m - 1];
-        if (cur.kernel == k && pm.seq == cur.seq - 1) { *prev = pm.kernel; break; }
+        if (cur.kernel == k && !cur.begin && !pm.begin && pm.seq == cur.seq - 1) { *prev = pm.kernel; break; }
     }
     return FEN_OK;
 }

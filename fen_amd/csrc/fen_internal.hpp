@@ -145,6 +145,7 @@ struct ProfMark {
     int kernel;
     long long seq;
     hipEvent_t ev;
+    bool begin;   // recorded just before the launch (sampled mode)
 };
 
 struct ProfSlot {
@@ -211,7 +212,19 @@ struct fen_ctx {
 
     fen::AdjointTape *tape = nullptr;       // recorded run of the discrete adjoint
 
+    // One fixed-dt step captured as a HIP graph and replayed (fen_ns_step /
+    // fen_ns_run without constant_CFL or profiling).  The graph bakes in the
+    // kernel arguments: dt, dt_o and everything `config_version` counts
+    // (parameters, bc types, the source term); a change of any rebuilds it.
+    bool use_graph = true;                  // FEN_STEP_GRAPH=0 at create turns it off
+    hipGraphExec_t step_graph = nullptr;
+    double graph_dt = 0.0, graph_dt_o = 0.0;
+    int graph_version = -1;
+    int config_version = 0;
+
     unsigned prof_mask = 0;
+    int prof_every = 1;                     // > 1: time only every prof_every-th step, begin/end pairs
+    long long step_count = 0;               // steps issued (advance calls)
     long long prof_seq = 0;                 // launches issued on the stream
     std::vector<fen::ProfMark> prof_marks;  // chronological
     std::vector<hipEvent_t> prof_pool;      // events owned by the context

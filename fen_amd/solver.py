@@ -209,6 +209,10 @@ class Solver:
                 mask |= 1 << names.index(kname)
         self._check(self.lib.fen_profile_enable(self.h, mask))
 
+    def profile_sample(self, every):
+        """Time only every `every`-th step, with an event pair around each selected launch."""
+        self._check(self.lib.fen_profile_sample(self.h, every))
+
     def profile_predecessor(self, kernel):
         names = [self.lib.fen_profile_name(k).decode() for k in range(self.lib.fen_profile_count())]
         k = ctypes.c_int()

@@ -172,8 +172,13 @@ int fen_synchronize(fen_ctx *ctx);
  * launch just before it on the stream (which must be selected too) to its own.
  * fen_profile_read returns the summed milliseconds and count of the timed
  * launches and the kernel's algorithmic HBM bytes per launch;
- * fen_profile_predecessor names the kernel launched just before kernel k. */
+ * fen_profile_predecessor names the kernel launched just before kernel k.
+ * fen_profile_sample(ctx, n), n > 1, switches to sampling: only every n-th
+ * time step is timed, with one event just before and one just after each
+ * launch of a selected kernel (the events drain the stream; sampling keeps
+ * that cost off the other steps).  n = 1 restores the default. */
 int fen_profile_enable(fen_ctx *ctx, int mask);
+int fen_profile_sample(fen_ctx *ctx, int every);
 int fen_profile_reset(fen_ctx *ctx);
 int fen_profile_count(void);
 const char *fen_profile_name(int kernel);
