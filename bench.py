@@ -213,7 +213,7 @@ def main():
 
     N = args.n
     g64, r64 = measure(N, args.steps, args.warmup, 0)
-    workload = f"tgv3d_{N}_fp64"
+    workload = f"tgv3d_{N}"
     traffic = read_traffic(r64["dominant"], workload)
     step_gbs = WORDS_3D * 8.0 * N ** 3 / (r64["ms"] * 1e-3) / 1e9
 
@@ -225,7 +225,7 @@ def main():
                 "speedup_vs_fp64": r64["ms"] / r32["ms"],
                 "roofline": {"bound": "hbm", "kernel": r32["dominant"], "achieved": r32["achieved"],
                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r32["achieved"] / HBM_PEAK_GBS,
-                             "traffic": read_traffic(r32["dominant"], f"tgv3d_{N}_fp32"),
+                             "traffic": read_traffic(r32["dominant"] + ".f32", workload),
                              "bytes_per_launch": r32["bytes"], "avg_launch_ms": r32["avg_ms"]},
                 "step_roofline": {"words_per_cell": WORDS_3D, "bytes_per_word": 4, "achieved": step32,
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step32 / HBM_PEAK_GBS},

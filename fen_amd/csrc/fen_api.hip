@@ -34,7 +34,8 @@ static hipEvent_t prof_event(fen_ctx *c)
     return c->prof_pool[c->prof_used++];
 }
 
-static bool prof_sampled(const fen_ctx *c) { return c->step_count % c->prof_every == 0; }
+// Timed steps are the 1st, (n+1)-th, ... after fen_profile_sample / fen_profile_reset.
+static bool prof_sampled(const fen_ctx *c) { return (c->step_count - 1 - c->prof_base) % c->prof_every == 0; }
 
 // Sampled mode only: an event just before the launch of a selected kernel.
 void prof_begin(fen_ctx *c, int k)
@@ -865,6 +866,7 @@ int fen_profile_sample(fen_ctx *c, int every)
 {
     if (!c || every < 1) return FEN_ERR_ARG;
     c->prof_every = every;
+    c->prof_base = c->step_count;
     return FEN_OK;
 }
 
@@ -875,6 +877,7 @@ int fen_profile_reset(fen_ctx *c)
         sync(c);
         c->prof_marks.clear();
         c->prof_used = 0;
+        c->prof_base = c->step_count;
     });
     return FEN_OK;
 }

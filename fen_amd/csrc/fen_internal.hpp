@@ -225,6 +225,7 @@ struct fen_ctx {
     unsigned prof_mask = 0;
     int prof_every = 1;                     // > 1: time only every prof_every-th step, begin/end pairs
     long long step_count = 0;               // steps issued (advance calls)
+    long long prof_base = 0;                // step_count when sampling was (re)started
     long long prof_seq = 0;                 // launches issued on the stream
     std::vector<fen::ProfMark> prof_marks;  // chronological
     std::vector<hipEvent_t> prof_pool;      // events owned by the context

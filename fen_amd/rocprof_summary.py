@@ -27,9 +27,9 @@ _PATTERNS = [
     (r"momentum_march_kernel|momentum_kernel_2d", "momentum"),
     (r"project_march_kernel|project_kernel_2d", "project"),
     (r"div_r2c_kernel", "div_r2c"),
-    (r"y_kernel<\d+, -1, 0>", "y_fwd"),
-    (r"y_kernel<\d+, 1, 0>", "y_inv"),
-    (r"y_kernel<\d+, -?1, 1>", "y_solve"),
+    (r"y_kernel<(?:double|float), \d+, -1, 0>", "y_fwd"),
+    (r"y_kernel<(?:double|float), \d+, 1, 0>", "y_inv"),
+    (r"y_kernel<(?:double|float), \d+, -?1, 1>", "y_solve"),
     (r"z_solve_kernel", "z_solve"),
     (r"thomas_kernel", "thomas"),
     (r"c2r_kernel", "c2r"),
@@ -41,9 +41,10 @@ _PATTERNS = [
 
 
 def short_name(kernel):
+    """Profiler name of a demangled kernel name; FP32 instantiations get '.f32'."""
     for rx, name in _PATTERNS:
         if re.search(rx, kernel):
-            return name
+            return name + (".f32" if "<float" in kernel else "")
     return kernel.split("(")[0][:60]
 
 
