@@ -76,28 +76,73 @@ def copy_probe(nbytes, reps=3):
 
 
 def cpu_baseline(N, budget_s):
-    """Time the oracle on the same workload: 1 warm-up step, then as many
-    steps as fit the budget (at least 1, at most 5)."""
+    """Time the oracle on the same workload on this machine's host cores:
+    OpenMP over the cores this process may use, then one thread; 1 warm-up
+    step and then 5 timed steps each (fewer if a step exceeds the budget),
+    median step time."""
+    import statistics
     from oracle import oracle as O
     from fen_amd import cases
     from tests.common import make_oracle
 
+    cores = len(os.sched_getaffinity(0))
     case = cases.tgv3d(N, steps=8)
-    o = make_oracle(case)
-    t0 = time.perf_counter()
-    o.step(1, case.dt)
-    first = time.perf_counter() - t0
-    nsteps = max(1, min(5, int(budget_s / max(first, 1e-3))))
-    t0 = time.perf_counter()
-    for s in range(2, 2 + nsteps):
-        o.step(s, case.dt)
-    el = time.perf_counter() - t0
-    ms = 1e3 * el / nsteps
-    cores = O.num_threads()
-    o.close()
+    out = {}
+    for label, threads in (("all", cores), ("single", 1)):
+        O.set_num_threads(threads)
+        o = make_oracle(case)
+        t0 = time.perf_counter()
+        o.step(1, case.dt)
+        first = time.perf_counter() - t0
+        nsteps = max(1, min(5, int(budget_s / max(first, 1e-3))))
+        times = []
+        for s in range(2, 2 + nsteps):
+            t0 = time.perf_counter()
+            o.step(s, case.dt)
+            times.append(time.perf_counter() - t0)
+        o.close()
+        out[label] = (1e3 * statistics.median(times), nsteps)
+    O.set_num_threads(cores)
+    ms, n_all = out["all"]
+    ms1, n1 = out["single"]
     return {"value": N ** 3 / (ms * 1e-3), "unit": "cell-updates/s", "cores": cores, "kind": "port",
             "ms_per_step": ms,
-            "sample": f"oracle (C, OpenMP {cores} threads) on tgv3d {N}^3 FP64: 1 warm-up + {nsteps} timed steps"}
+            "single_thread": {"value": N ** 3 / (ms1 * 1e-3), "ms_per_step": ms1, "cores": 1},
+            "sample": f"oracle (C, OpenMP) on tgv3d {N}^3 FP64: 1 warm-up + {n_all} timed steps on {cores} threads "
+                      f"and 1 warm-up + {n1} on 1 thread, median step"}
+
+
+def tgv2d_time(steps=200):
+    """2-D 128^2 case: launch-bound, reported as time per step."""
+    from fen_amd import cases
+    from tests.common import make_gpu
+    case = cases.tgv2d(128, steps=steps)
+    g = make_gpu(case)
+    g.run(0, 10, case.dt)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(10, steps, case.dt)
+    g.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    g.close()
+    return {"workload": "2-D Taylor-Green vortex 128^2 (pp), FP64", "ms_per_step": ms, "steps": steps}
+
+
+def channel_rate(steps=20, precision=0):
+    """Channel 512x512x256 (x, y periodic, walls at z; ppn: FFT x, y + Thomas z)."""
+    from fen_amd import cases
+    from tests.common import make_gpu
+    case = cases.channel(steps=steps)
+    g = make_gpu(case, precision=precision)
+    g.run(0, 3, case.dt)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(3, steps, case.dt)
+    g.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    g.close()
+    cells = case.n[0] * case.n[1] * case.n[2]
+    return {"ms_per_step": ms, "value": cells / (ms * 1e-3), "steps": steps}
 
 
 def measure(N, steps, warmup, precision):
@@ -192,6 +237,7 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid size N (N^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+refine form")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 512^3, channel and 2-D measurements")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle steps")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="BYTES",
                     help="after the run, launch a plain device copy over BYTES bytes three times: "
@@ -234,6 +280,21 @@ def main():
                                       maxdiv_fp64=r64["maxdiv"], maxdiv_fp32_refine=r32["maxdiv"])}
         g32.close()
 
+    extra = None
+    if not args.no_extra:
+        # the other configurations, fewer steps each (same warm-up and timing as above)
+        g64.close()
+        extra = {}
+        g, r512 = measure(512, 20, 3, 0)
+        g.close()
+        g, r512f = measure(512, 20, 3, FP32_REFINE)
+        g.close()
+        extra["tgv3d_512"] = {"fp64": {"value": r512["value"], "ms_per_step": r512["ms"]},
+                              "fp32_refine": {"value": r512f["value"], "ms_per_step": r512f["ms"]},
+                              "speedup_vs_fp64": r512["ms"] / r512f["ms"], "steps": 20}
+        extra["channel_512x512x256"] = {"fp64": channel_rate(20, 0), "fp32_refine": channel_rate(20, FP32_REFINE)}
+        extra["tgv2d_128"] = tgv2d_time()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(N, args.cpu_budget)
@@ -261,11 +322,13 @@ def main():
         "kernel_us_per_launch": r64["breakdown"],
         "maxdiv": r64["maxdiv"],
         "fp32_refine": fp32,
+        "other_configs": extra,
         "cpu_baseline": cpu,
     }
     if cpu:
         out["gpu_over_cpu"] = r64["value"] / cpu["value"]
-    g64.close()
+    if args.no_extra:
+        g64.close()
     print(json.dumps(out), flush=True)
     if args.copy_probe > 0:
         copy_probe(int(args.copy_probe))

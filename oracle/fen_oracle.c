@@ -280,6 +280,15 @@ int orc_num_threads(void)
 #endif
 }
 
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* -------------------------------------------------------------------------- */
 /* Poisson fast direct solvers (src/poisson.f90)                              */
 /* -------------------------------------------------------------------------- */

@@ -79,6 +79,7 @@ void orc_thomas(int n, const double *a, const double *b, const double *cc, doubl
                 double *re, double *im);
 
 int orc_num_threads(void);
+void orc_set_num_threads(int n);   /* OpenMP threads of the following calls */
 
 /* Discrete adjoint of one orc_ns_step (3-D, triply periodic, ppp, no source,
  * fixed dt; no counterpart in the reference).  The context's u, v, w hold the

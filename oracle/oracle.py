@@ -68,6 +68,7 @@ def lib():
         L.orc_fft_c2r.argtypes = [ctypes.c_int, dp, dp, dp]
         L.orc_thomas.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_double, ctypes.c_double, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
         L.orc_adjoint_step.argtypes = [ctypes.c_void_p, ctypes.c_double] + [dp] * 7
         L.orc_adjoint_step.restype = ctypes.c_int
         _lib = L
@@ -227,3 +228,7 @@ def thomas(a, b, c, lam_x, lam_y, r):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_num_threads(n):
+    lib().orc_set_num_threads(n)
