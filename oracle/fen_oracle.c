@@ -59,7 +59,8 @@ struct orc_ctx {
     double *mwx, *mwy, *mwz;
     double *ta, *tb, *tc;
     offt fx, fy, fz;
-    double *sre, *sim;  /* half spectrum (nx/2+1)*ny*nz, kx fastest */
+    double *sre, *sim;  /* half spectrum (nx/2+1)*ny*nz, kx fastest (real spectra use sre) */
+    double *cosx, *cosy; /* cos(pi m / (2n)), m < 4n, for the DCTs along x and y */
 };
 
 #define GX(c) ((size_t)(c)->nx + 2)
@@ -510,12 +511,214 @@ static void poisson_ppn(orc_ctx *c, double *f, int gl)
     for (size_t q = 0; q < n; q++) f[q] = f[q] - sub;                           /* :1171 */
 }
 
+/* cos(pi m / (2n)) for m < 4n, in long double. */
+static double *dct_table(int n)
+{
+    double *t = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+    const long double pi = acosl(-1.0L);
+    for (int m = 0; m < 4 * n; m++) t[m] = (double)cosl(pi * (long double)m / (2.0L * (long double)n));
+    return t;
+}
+
+/* FFTW_REDFT10 (DCT-II), unnormalised, by its definition:
+ * y_k = 2 sum_j x_j cos(pi k (2j+1) / (2n)). */
+static void dct2_line(int n, const double *T, const double *x, size_t xs, double *y, size_t ys)
+{
+    for (int k = 0; k < n; k++) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s += x[(size_t)j * xs] * T[((size_t)k * (2 * j + 1)) % (4 * (size_t)n)];
+        y[(size_t)k * ys] = 2.0 * s;
+    }
+}
+
+/* FFTW_REDFT01 (DCT-III), unnormalised: y_j = x_0 + 2 sum_{k>=1} x_k cos(pi k (2j+1) / (2n)). */
+static void dct3_line(int n, const double *T, const double *x, size_t xs, double *y, size_t ys)
+{
+    for (int j = 0; j < n; j++) {
+        double s = 0.0;
+        for (int k = 1; k < n; k++) s += x[(size_t)k * xs] * T[((size_t)k * (2 * j + 1)) % (4 * (size_t)n)];
+        y[(size_t)j * ys] = x[0] + 2.0 * s;
+    }
+}
+
+/* Thomas sweep of the 2-D solvers, src/poisson.f90:346-384: divisions,
+ * and the c1 and d1 denominators associated differently. */
+static void thomas_2d(int n, const double *a, const double *b, const double *cc, double mx, double *re, double *im,
+                      size_t s, double *c1, double *dr, double *di)
+{
+    c1[0] = cc[0] / (b[0] + mx);
+    dr[0] = re[0] / (b[0] + mx);
+    if (im) di[0] = im[0] / (b[0] + mx);
+    for (int j = 1; j < n - 1; j++) {
+        c1[j] = cc[j] / ((b[j] - a[j] * c1[j - 1]) + mx);
+        const double den = (b[j] + mx) - a[j] * c1[j - 1];
+        dr[j] = (re[(size_t)j * s] - a[j] * dr[j - 1]) / den;
+        if (im) di[j] = (im[(size_t)j * s] - a[j] * di[j - 1]) / den;
+    }
+    const double frac = (b[n - 1] + mx) - a[n - 1] * c1[n - 2];
+    if (frac != 0.0) {
+        dr[n - 1] = (re[(size_t)(n - 1) * s] - a[n - 1] * dr[n - 2]) / frac;
+        if (im) di[n - 1] = (im[(size_t)(n - 1) * s] - a[n - 1] * di[n - 2]) / frac;
+    } else {
+        dr[n - 1] = 0.0;
+        if (im) di[n - 1] = 0.0;
+    }
+    re[(size_t)(n - 1) * s] = dr[n - 1];
+    if (im) im[(size_t)(n - 1) * s] = di[n - 1];
+    for (int j = n - 2; j >= 0; j--) {
+        re[(size_t)j * s] = dr[j] - c1[j] * re[(size_t)(j + 1) * s];
+        if (im) im[(size_t)j * s] = di[j] - c1[j] * im[(size_t)(j + 1) * s];
+    }
+}
+
+/* Thomas sweep of the 3-D solvers on a real column (nnn), as thomas_column. */
+static void thomas_real(int n, const double *a, const double *b, const double *cc, double mx, double my, double *x,
+                        size_t s, double *c1, double *d)
+{
+    double factor = 1.0 / ((b[0] + mx) + my);
+    c1[0] = cc[0] * factor;
+    d[0] = x[0] * factor;
+    for (int k = 1; k < n - 1; k++) {
+        factor = 1.0 / (((b[k] + mx) + my) - a[k] * c1[k - 1]);
+        c1[k] = cc[k] * factor;
+        d[k] = (x[(size_t)k * s] - a[k] * d[k - 1]) * factor;
+    }
+    factor = ((b[n - 1] + mx) + my) - a[n - 1] * c1[n - 2];
+    d[n - 1] = factor != 0.0 ? (x[(size_t)(n - 1) * s] - a[n - 1] * d[n - 2]) / factor : 0.0;
+    x[(size_t)(n - 1) * s] = d[n - 1];
+    for (int k = n - 2; k >= 0; k--) x[(size_t)k * s] = d[k] - c1[k] * x[(size_t)(k + 1) * s];
+}
+
+/* Pointer to interior line (j,k) of f with ghost level gl. */
+static double *row(const orc_ctx *c, double *f, int gl, int j, int k)
+{
+    const size_t sx = gl ? GX(c) : (size_t)c->nx, sy = gl ? GY(c) : (size_t)c->ny;
+    return f + (size_t)gl + sx * ((size_t)(j - 1 + gl) + sy * (size_t)(k - 1 + gl));
+}
+
+/* poisson_solver_pn, src/poisson.f90:306-412 (2-D: x periodic, y Neumann). */
+static void poisson_pn(orc_ctx *c, double *f, int gl)
+{
+    const int hx = c->nx / 2 + 1, ny = c->ny;
+    spectrum_x_forward(c, f, gl);
+    spectrum_scale(c, (double)(float)c->nx);                                    /* :338 */
+    double *w = (double *)malloc(sizeof(double) * 3 * (size_t)ny);
+    for (int i = 0; i < hx; i++)                                                /* :344-384 */
+        thomas_2d(ny, c->ta, c->tb, c->tc, c->mwx[i], c->sre + i, c->sim + i, (size_t)hx, w, w + ny, w + 2 * ny);
+    free(w);
+    spectrum_x_backward(c, f, gl);
+    const double mean = interior_sum(c, f, gl);                                 /* :397-404 */
+    const double sub = mean / (double)(float)(c->nx * c->ny * c->nz);
+    for (int k = 1; k <= c->nz; k++)                                            /* :408: interior only */
+        for (int j = 1; j <= ny; j++) {
+            double *r = row(c, f, gl, j, k);
+            for (int i = 0; i < c->nx; i++) r[i] = r[i] - sub;
+        }
+}
+
+/* poisson_solver_nn, src/poisson.f90:509-596 (2-D: x and y Neumann). */
+static void poisson_nn(orc_ctx *c, double *f, int gl)
+{
+    const int nx = c->nx, ny = c->ny;
+    double *R = c->sre;                                                         /* outr_x, [i][j] */
+    for (int j = 1; j <= ny; j++) dct2_line(nx, c->cosx, row(c, f, gl, j, 1), 1, R + (size_t)nx * (j - 1), 1);
+    double *w = (double *)malloc(sizeof(double) * 3 * (size_t)ny);
+    for (int i = 0; i < nx; i++)                                                /* :540-580 */
+        thomas_2d(ny, c->ta, c->tb, c->tc, c->mwx[i], R + i, NULL, (size_t)nx, w, w + ny, w + 2 * ny);
+    free(w);
+    for (int j = 1; j <= ny; j++) dct3_line(nx, c->cosx, R + (size_t)nx * (j - 1), 1, row(c, f, gl, j, 1), 1);
+    const size_t n = array_len(c, gl);
+    const double s = (double)(float)(nx * 2);
+    for (size_t q = 0; q < n; q++) f[q] = f[q] / s;                             /* :594: whole array */
+}
+
+/* poisson_solver_npn, src/poisson.f90:1177-1312 (x Neumann, y periodic, z Neumann). */
+static void poisson_npn(orc_ctx *c, double *f, int gl)
+{
+    const int nx = c->nx, ny = c->ny, nz = c->nz, hy = ny / 2 + 1;
+    double *R = (double *)malloc(sizeof(double) * (size_t)nx * ny * nz);      /* outr_x, [i][j][k] */
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            dct2_line(nx, c->cosx, row(c, f, gl, j, k), 1, R + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1)), 1);
+    const size_t nr = (size_t)nx * ny * nz;
+    for (size_t q = 0; q < nr; q++) R[q] = R[q] / (double)(nx * 2);            /* :1215 real(nx*2, dp) */
+    double *t = (double *)malloc(sizeof(double) * 2 * (size_t)ny);
+    for (int k = 0; k < nz; k++)                                                /* :1221-1228: y r2c */
+        for (int i = 0; i < nx; i++) {
+            const size_t o = (size_t)i + (size_t)nx * (size_t)hy * k;
+            fft_r2c_line(&c->fy, R + (size_t)i + (size_t)nx * (size_t)ny * k, (size_t)nx, c->sre + o, c->sim + o,
+                         (size_t)nx, t, t + ny);
+        }
+    const size_t ns = (size_t)nx * hy * nz;
+    for (size_t q = 0; q < ns; q++) { c->sre[q] = c->sre[q] / (double)ny; c->sim[q] = c->sim[q] / (double)ny; }
+    double *w = (double *)malloc(sizeof(double) * 3 * (size_t)nz);
+    for (int jy = 0; jy < hy; jy++)                                             /* :1236-1280 */
+        for (int i = 0; i < nx; i++) {
+            const size_t q = (size_t)i + (size_t)nx * jy;
+            thomas_column(nz, c->ta, c->tb, c->tc, c->mwx[i], c->mwy[jy], c->sre + q, c->sim + q,
+                          (size_t)nx * hy, w, w + nz, w + 2 * nz);
+        }
+    free(w);
+    for (int k = 0; k < nz; k++)                                                /* :1286-1291: y c2r */
+        for (int i = 0; i < nx; i++) {
+            const size_t o = (size_t)i + (size_t)nx * (size_t)hy * k;
+            fft_c2r_line(&c->fy, c->sre + o, c->sim + o, (size_t)nx, R + (size_t)i + (size_t)nx * (size_t)ny * k,
+                         (size_t)nx, t, t + ny);
+        }
+    free(t);
+    memset(f, 0, array_len(c, gl) * sizeof(double));                           /* :1297 */
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            dct3_line(nx, c->cosx, R + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1)), 1, row(c, f, gl, j, k), 1);
+    free(R);                                                                    /* no mean removal (:1310) */
+}
+
+/* poisson_solver_nnn, src/poisson.f90:1316-1451 (x, y, z Neumann). */
+static void poisson_nnn(orc_ctx *c, double *f, int gl)
+{
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    double *R = (double *)malloc(sizeof(double) * (size_t)nx * ny * nz);
+    double *Y = (double *)malloc(sizeof(double) * (size_t)nx * ny * nz);
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            dct2_line(nx, c->cosx, row(c, f, gl, j, k), 1, R + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1)), 1);
+    const size_t nr = (size_t)nx * ny * nz;
+    for (size_t q = 0; q < nr; q++) R[q] = R[q] / (double)(nx * 2);            /* :1354 */
+    for (int k = 0; k < nz; k++)                                                /* :1360-1365: y DCT-II */
+        for (int i = 0; i < nx; i++) {
+            const size_t o = (size_t)i + (size_t)nx * (size_t)ny * k;
+            dct2_line(ny, c->cosy, R + o, (size_t)nx, Y + o, (size_t)nx);
+        }
+    for (size_t q = 0; q < nr; q++) Y[q] = Y[q] / (double)(ny * 2);            /* :1368 */
+    double *w = (double *)malloc(sizeof(double) * 2 * (size_t)nz);
+    for (int j = 0; j < ny; j++)                                                /* :1375-1419 */
+        for (int i = 0; i < nx; i++)
+            thomas_real(nz, c->ta, c->tb, c->tc, c->mwx[i], c->mwy[j], Y + (size_t)i + (size_t)nx * j,
+                        (size_t)nx * ny, w, w + nz);
+    free(w);
+    for (int k = 0; k < nz; k++)                                                /* :1425-1430: y DCT-III */
+        for (int i = 0; i < nx; i++) {
+            const size_t o = (size_t)i + (size_t)nx * (size_t)ny * k;
+            dct3_line(ny, c->cosy, Y + o, (size_t)nx, R + o, (size_t)nx);
+        }
+    memset(f, 0, array_len(c, gl) * sizeof(double));                           /* :1436 */
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            dct3_line(nx, c->cosx, R + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1)), 1, row(c, f, gl, j, k), 1);
+    free(R);
+    free(Y);                                                                    /* no mean removal (:1449) */
+}
+
 void orc_poisson_solve(orc_ctx *c, double *f, int gl)
 {
     switch (c->solver) {
     case ORC_PP: poisson_pp(c, f, gl); break;
     case ORC_PPP: poisson_ppp(c, f, gl); break;
     case ORC_PPN: poisson_ppn(c, f, gl); break;
+    case ORC_PN: poisson_pn(c, f, gl); break;
+    case ORC_NN: poisson_nn(c, f, gl); break;
+    case ORC_NPN: poisson_npn(c, f, gl); break;
+    case ORC_NNN: poisson_nnn(c, f, gl); break;
     default: break;
     }
 }
@@ -565,9 +768,13 @@ orc_ctx *orc_create(int dim, int nx, int ny, int nz, double lx, double ly, doubl
     if (dim == 3) {
         if (pb[0] && pb[1] && pb[2]) c->solver = ORC_PPP;
         else if (pb[0] && pb[1] && !pb[2]) c->solver = ORC_PPN;
-        else { free(c); return NULL; }   /* npn / nnn: not restated yet */
+        else if (!pb[0] && pb[1] && !pb[2]) c->solver = ORC_NPN;
+        else if (!pb[0] && !pb[1] && !pb[2]) c->solver = ORC_NNN;
+        else { free(c); return NULL; }   /* the reference stops (:91-94) */
     } else {
         if (pb[0] && pb[1]) c->solver = ORC_PP;
+        else if (pb[0] && !pb[1]) c->solver = ORC_PN;
+        else if (!pb[0] && !pb[1]) c->solver = ORC_NN;
         else { free(c); return NULL; }
     }
 
@@ -596,12 +803,19 @@ orc_ctx *orc_create(int dim, int nx, int ny, int nz, double lx, double ly, doubl
     fft_init(&c->fx, nx);
     fft_init(&c->fy, ny);
     fft_init(&c->fz, nz);
-    c->mwx = modified_wavenumbers(nx, c->delta, 2.0);
-    c->mwy = modified_wavenumbers(ny, c->delta, 2.0);
+    /* Neumann directions use DCT wavenumbers (factor pi), :265,794,881,899 */
+    c->mwx = modified_wavenumbers(nx, c->delta, pb[0] ? 2.0 : 1.0);
+    c->mwy = modified_wavenumbers(ny, c->delta, pb[1] ? 2.0 : 1.0);
     if (c->solver == ORC_PPP) c->mwz = modified_wavenumbers(nz, c->delta, 2.0);
-    if (c->solver == ORC_PPN)
+    if (c->solver == ORC_PPN || c->solver == ORC_NPN || c->solver == ORC_NNN)
         tridiag_coefficients(c, nz, c->bc[4] == ORC_BC_INFLOW && c->bc[5] == ORC_BC_OUTFLOW);
-    const size_t ns = (size_t)(nx / 2 + 1) * ny * nz;
+    if (c->solver == ORC_PN || c->solver == ORC_NN)                             /* along y, :214-226 */
+        tridiag_coefficients(c, ny, c->bc[2] == ORC_BC_INFLOW && c->bc[3] == ORC_BC_OUTFLOW);
+    if (!pb[0]) c->cosx = dct_table(nx);
+    if (!pb[1]) c->cosy = dct_table(ny);
+    size_t ns = (size_t)(nx / 2 + 1) * ny * nz;
+    if ((size_t)nx * (ny / 2 + 1) * nz > ns) ns = (size_t)nx * (ny / 2 + 1) * nz;
+    if ((size_t)nx * ny * nz > ns) ns = (size_t)nx * ny * nz;
     c->sre = (double *)calloc(ns, sizeof(double));
     c->sim = (double *)calloc(ns, sizeof(double));
     return c;
@@ -615,7 +829,7 @@ void orc_destroy(orc_ctx *c)
     fft_free(&c->fx); fft_free(&c->fy); fft_free(&c->fz);
     free(c->mwx); free(c->mwy); free(c->mwz);
     free(c->ta); free(c->tb); free(c->tc);
-    free(c->sre); free(c->sim);
+    free(c->sre); free(c->sim); free(c->cosx); free(c->cosy);
     free(c);
 }
 

@@ -169,6 +169,65 @@ def test_poisson_inverts_fen_laplacian(kind, dim, n, bc):
         assert abs(phi.mean()) < 1e-12 * np.abs(phi).max()
 
 
+def _laplacian_axes(phi, delta, neumann):
+    """FEN's Laplacian with, per axis, periodic wrap or Neumann ends (ghost =
+    first/last cell): the operator the Neumann solvers invert (tridiagonal
+    b1 += a1, bn += cn, src/poisson.f90:214-226; DCT-II modes, :265)."""
+    out = np.zeros_like(phi)
+    for ax, neu in enumerate(neumann):
+        if phi.shape[ax] == 1:
+            continue
+        lo = np.take(phi, [0] if neu else [-1], axis=ax)
+        hi = np.take(phi, [-1] if neu else [0], axis=ax)
+        g = np.concatenate([lo, phi, hi], axis=ax)
+        sl = [slice(None)] * 3
+        sl[ax] = slice(2, None)
+        up = g[tuple(sl)]
+        sl[ax] = slice(0, -2)
+        out += (up - 2.0 * phi + g[tuple(sl)])
+    return out / (delta * delta)
+
+
+@pytest.mark.parametrize("kind,dim,n,bc", [
+    ("pn", 2, (32, 16, 1), [BC_PERIODIC] * 2 + [BC_WALL] * 2 + [BC_PERIODIC] * 2),
+    ("nn", 2, (16, 32, 1), [BC_WALL] * 4 + [BC_PERIODIC] * 2),
+    ("npn", 3, (16, 8, 12), [BC_WALL] * 2 + [BC_PERIODIC] * 2 + [BC_WALL] * 2),
+    ("nnn", 3, (8, 16, 12), [BC_WALL] * 6),
+])
+def test_neumann_poisson_solvers_invert_fen_laplacian(kind, dim, n, bc):
+    """pn, nn, npn, nnn (src/poisson.f90:306-596, 1177-1451): DCT-II/III along
+    Neumann axes, FFT along periodic ones, Thomas along the last; the result
+    satisfies FEN's Laplacian with Neumann ends to round-off for a zero-mean
+    right-hand side (the constant mode is singular)."""
+    rng = np.random.default_rng(9)
+    L = (1.0, n[1] / n[0], n[2] / n[0])
+    o = O.Oracle(dim, n, L, bc=bc)
+    assert o.solver == kind
+    f = np.asfortranarray(rng.standard_normal(o.n))
+    f -= f.mean()
+    phi = o.poisson_solve(f.copy(order="F"), 0)
+    neumann = [kind[0] == "n", kind[1] == "n", dim == 3 and kind[2] == "n"]
+    lap = _laplacian_axes(phi, o.delta, neumann)
+    assert np.abs(lap - f).max() < 1e-10 * np.abs(f).max(), np.abs(lap - f).max()
+    if kind == "pn":   # pn removes the interior mean (:408)
+        assert abs(phi.mean()) < 1e-12 * np.abs(phi).max()
+
+
+def test_neumann_solver_dct_conventions():
+    """The x transform pair of nn is FFTW's REDFT10/REDFT01: a single cosine
+    mode cos(pi k (i - 1/2)/n) is an eigenvector of the Neumann Laplacian with
+    eigenvalue 2(cos(pi k/n) - 1)/delta^2, and the solve divides by it."""
+    n = (32, 16, 1)
+    o = O.Oracle(2, n, (1.0, 0.5, 1.0), bc=[BC_WALL] * 4 + [BC_PERIODIC] * 2)
+    i = np.arange(1, n[0] + 1)[:, None, None]
+    j = np.arange(1, n[1] + 1)[None, :, None]
+    for kx, ky in ((1, 0), (5, 3), (31, 15)):
+        mode = np.cos(np.pi * kx * (i - 0.5) / n[0]) * np.cos(np.pi * ky * (j - 0.5) / n[1]) * np.ones(n)
+        lam = 2 * (np.cos(np.pi * kx / n[0]) - 1) / o.delta ** 2 + 2 * (np.cos(np.pi * ky / n[1]) - 1) / o.delta ** 2
+        phi = o.poisson_solve(np.asfortranarray(mode), 0)
+        assert np.abs(phi - mode / lam).max() < 1e-11 * np.abs(mode / lam).max(), (kx, ky)
+
+
 @pytest.mark.parametrize("case", ["ppp", "ppn", "pp"])
 def test_poisson_convergence_rate(case):
     """test/small_test/poisson/convergence_rate/convergence_rate.f90 (order > 1.8)."""
