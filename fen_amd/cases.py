@@ -11,7 +11,7 @@ import math
 
 import numpy as np
 
-from ._lib import BC_PERIODIC, BC_WALL, P, U, V, W
+from ._lib import BC_INFLOW, BC_OUTFLOW, BC_PERIODIC, BC_WALL, P, U, V, W
 
 GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 SEED = 20261202
@@ -45,10 +45,11 @@ def gaussian(component, n, seed=SEED):
 class Case:
     """Grid, physics and initial interior fields of one run."""
 
-    def __init__(self, name, dim, n, L, bc, density, viscosity, g, dt, dt_o, steps, fields, cfl=1.0):
+    def __init__(self, name, dim, n, L, bc, density, viscosity, g, dt, dt_o, steps, fields, cfl=1.0, planes=None):
         self.name, self.dim, self.n, self.L, self.bc = name, dim, n, L, bc
         self.density, self.viscosity, self.g = density, viscosity, g
         self.dt, self.dt_o, self.steps, self.fields, self.cfl = dt, dt_o, steps, fields, cfl
+        self.planes = planes or {}   # {(field, face): boundary values on the ghosted face plane}
 
     def apply(self, solver, upload):
         """Configure a solver (oracle or GPU) and write the initial fields.
@@ -58,6 +59,11 @@ class Case:
         """
         solver.set_params(density=self.density, viscosity=self.viscosity, g=self.g, cfl=self.cfl)
         solver.set_dt_o(self.dt_o)
+        for (fid, face), plane in self.planes.items():
+            if hasattr(solver, "set_bc_plane"):
+                solver.set_bc_plane(fid, face, plane)
+            else:
+                solver.bc_plane(fid, face)[...] = plane
         for fid, arr in self.fields.items():
             upload(fid, arr)
         for fid in self.fields:
@@ -144,6 +150,114 @@ def channel(nx=512, ny=512, nz=256, Ub=15.7, noise=0.05, nu=1.0 / 180.0, steps=1
     bc = [BC_PERIODIC, BC_PERIODIC, BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL]
     return Case("channel", 3, shp, (Lx, Ly, Lz), bc, 1.0, nu, (1.0, 0.0, 0.0), dt, dt, steps,
                 {U: u, V: v, W: w, P: p}, cfl=cfl)
+
+
+def _noise(shp, amp, seed, first=0):
+    n = int(np.prod(shp))
+    return [amp * gaussian(first + c, n, seed).reshape(shp, order="F") for c in range(3)]
+
+
+def _dt(d, Uref, nu, cfl):
+    """Convective limit, and a viscous one inside the stability region of
+    explicit AB2 diffusion in 3-D (nu dt / d^2 <= 1/12)."""
+    return min(cfl * d / Uref, 0.05 * d * d / nu)
+
+
+def cavity2d(nx=64, ny=64, Re=100.0, steps=50, noise=0.01, cfl=0.5, seed=SEED):
+    """Lid-driven cavity, walls on the four sides (nn): the top wall moves
+    with u = 1.  Initial u, v: seeded noise, zero on the wall faces."""
+    d = 1.0 / nx
+    shp = (nx, ny, 1)
+    u, v, _ = _noise(shp, noise, seed)
+    u[-1, :, :] = 0.0   # the right face of the last cell is the wall
+    v[:, -1, :] = 0.0
+    p = np.zeros(shp, order="F")
+    lid = np.ones((nx + 2, 3))
+    bc = [BC_WALL] * 4 + [BC_PERIODIC] * 2
+    return Case("cavity2d", 2, shp, (1.0, ny * d, d), bc, 1.0, 1.0 / Re, (0.0, 0.0, 0.0), _dt(d, 1.0, 1.0 / Re, cfl),
+                _dt(d, 1.0, 1.0 / Re, cfl), steps, {U: u, V: v, P: p}, cfl=cfl, planes={(U, 3): lid})
+
+
+def channel2d(nx=128, ny=64, Ub=1.0, Re=100.0, steps=50, noise=0.05, cfl=0.5, inflow=False, seed=SEED):
+    """2-D channel, x periodic (pn).  Walls at y with the forcing g = (1, 0);
+    or, with `inflow`, a parabolic v profile entering at the bottom face and
+    leaving through an outflow face at the top (FEN's Inflow/Outflow pair)."""
+    d = 1.0 / ny
+    shp = (nx, ny, 1)
+    yc = (np.arange(1, ny + 1, dtype=np.float64) - 0.5) * d
+    xc = (np.arange(1, nx + 1, dtype=np.float64) - 0.5) * d
+    u, v, _ = _noise(shp, noise * Ub, seed)
+    if inflow:
+        prof = 1.5 * Ub * 4.0 * (xc * (nx * d - xc)) / (nx * d) ** 2
+        v += prof[:, None, None]
+        plane = np.zeros((nx + 2, 3))
+        plane[1:-1, :] = prof[:, None]
+        bc = [BC_PERIODIC, BC_PERIODIC, BC_INFLOW, BC_OUTFLOW, BC_PERIODIC, BC_PERIODIC]
+        planes, g = {(V, 2): plane}, (0.0, 0.0, 0.0)
+    else:
+        u += (1.5 * Ub * 4.0 * yc * (1.0 - yc))[None, :, None]
+        v[:, -1, :] = 0.0
+        bc = [BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC]
+        planes, g = {}, (1.0, 0.0, 0.0)
+    p = np.zeros(shp, order="F")
+    nu = Ub / Re
+    dt = _dt(d, 1.5 * Ub, nu, cfl)
+    return Case("channel2d_io" if inflow else "channel2d", 2, shp, (nx * d, 1.0, d), bc, 1.0, nu, g, dt, dt, steps,
+                {U: u, V: v, P: p}, cfl=cfl, planes=planes)
+
+
+def duct3d(nx=64, ny=32, nz=32, Ub=1.0, Re=100.0, steps=30, noise=0.05, cfl=0.5, z_io=False, seed=SEED):
+    """3-D duct, walls at x, y periodic, walls at z (npn): flow along y driven
+    by the forcing g = (0, 1, 0) from a seeded perturbation of the laminar
+    profile.  With `z_io` the z faces are FEN's Inflow/Outflow pair instead:
+    a parabolic w profile enters at the bottom and leaves at the top.  (x
+    must stay a wall pair: npn's cosine transform in x fixes a zero normal
+    gradient of phi there, which an Outflow face, phi = 0, contradicts.)"""
+    d = 1.0 / nz
+    shp = (nx, ny, nz)
+    zc = (np.arange(1, nz + 1, dtype=np.float64) - 0.5) * d
+    xc = (np.arange(1, nx + 1, dtype=np.float64) - 0.5) * d
+    Lx = nx * d
+    px = 4.0 * xc * (Lx - xc) / Lx ** 2
+    u, v, w = _noise(shp, noise * Ub, seed)
+    u[-1, :, :] = 0.0
+    gy = ny + 2
+    if z_io:
+        prof = 1.5 * Ub * px
+        w += prof[:, None, None]
+        plane = np.zeros((nx + 2, gy))
+        plane[1:-1, :] = prof[:, None]
+        bc = [BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC, BC_INFLOW, BC_OUTFLOW]
+        planes, g = {(W, 4): plane}, (0.0, 0.0, 0.0)
+    else:
+        v += (2.0 * Ub * px[:, None] * (4.0 * zc * (1.0 - zc))[None, :])[:, None, :]
+        w[:, :, -1] = 0.0
+        bc = [BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL]
+        planes, g = {}, (0.0, 1.0, 0.0)
+    p = np.zeros(shp, order="F")
+    nu = Ub / Re
+    dt = _dt(d, 2.0 * Ub, nu, cfl)
+    return Case("duct3d_zio" if z_io else "duct3d", 3, shp, (Lx, ny * d, 1.0), bc, 1.0, nu, g, dt,
+                dt, steps, {U: np.asfortranarray(u), V: np.asfortranarray(v), W: np.asfortranarray(w), P: p},
+                cfl=cfl, planes=planes)
+
+
+def cavity3d(nx=32, ny=32, nz=32, Re=100.0, steps=30, noise=0.01, cfl=0.5, seed=SEED):
+    """Lid-driven cubic cavity, walls on all six faces (nnn); the top z wall
+    moves with u = 1."""
+    d = 1.0 / nx
+    shp = (nx, ny, nz)
+    u, v, w = _noise(shp, noise, seed)
+    u[-1, :, :] = 0.0
+    v[:, -1, :] = 0.0
+    w[:, :, -1] = 0.0
+    p = np.zeros(shp, order="F")
+    lid = np.ones((nx + 2, ny + 2))
+    nu = 1.0 / Re
+    dt = _dt(d, 1.0, nu, cfl)
+    return Case("cavity3d", 3, shp, (1.0, ny * d, nz * d), [BC_WALL] * 6, 1.0, nu, (0.0, 0.0, 0.0), dt, dt, steps,
+                {U: np.asfortranarray(u), V: np.asfortranarray(v), W: np.asfortranarray(w), P: p}, cfl=cfl,
+                planes={(U, 5): lid})
 
 
 def interior(shape_ghosted):

@@ -3,8 +3,9 @@
 // Orchestrates one navier_stokes_solver step (src/navier_stokes.f90:50-136)
 // as a fixed sequence of kernels on the context's stream:
 //   K1 momentum (u -> u*, dv_o)        ghost(u*)
-//   K2 div(u*)*rho/dt + x r2c          K3..K5 y / z passes (ppp, ppn, pp)
-//   K6 x c2r -> phi (+ mean, pp/ppn)   ghost(phi)
+//   K2 div(u*)*rho/dt + x r2c or DCT-II
+//   K3..K5 y / z passes: FFT, eigenvalue divide or Thomas per solver
+//   K6 x c2r or DCT-III -> phi (+ mean, pp/ppn/pn)   ghost(phi)
 //   K7 projection + p update           ghost(u, p)
 //   K8 checks
 #include <algorithm>
@@ -14,6 +15,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fen_internal.hpp"
@@ -125,6 +127,18 @@ template <class R> std::vector<typename fen::Cplx<R>::type> twiddles(int n)
     for (int m = 0; m < n; m++) {
         const long double a = tpi * (long double)m / (long double)n;
         w[m] = {(R)std::cos(a), (R)(-std::sin(a))};
+    }
+    return w;
+}
+
+// (cos, sin)(pi k / (2n)), k < n, for Makhoul's DCT post/pre-twiddle, in long double.
+std::vector<double2> dct_twiddles(int n)
+{
+    std::vector<double2> w(n);
+    const long double pi = std::acos(-1.0L);
+    for (int k = 0; k < n; k++) {
+        const long double a = pi * (long double)k / (2.0L * (long double)n);
+        w[k] = double2{(double)std::cos(a), (double)std::sin(a)};
     }
     return w;
 }
@@ -245,7 +259,7 @@ void free_all(fen_ctx *c)
     for (int d = 0; d < 3; d++) { f(c->d_u[d]); f(c->d_us[d]); f(c->d_dvo[d]); f(c->d_S[d]); }
     f(c->d_p); f(c->d_phi); f(c->d_spec); f(c->d_c1); f(c->d_dd);
     f(c->d_mwx); f(c->d_mwy); f(c->d_mwz); f(c->d_ta); f(c->d_tb); f(c->d_tc);
-    f(c->d_twx); f(c->d_twy); f(c->d_twz); f(c->d_part); f(c->d_scal);
+    f(c->d_twx); f(c->d_twy); f(c->d_twz); f(c->d_spec2); f(c->d_mkx); f(c->d_mky); f(c->d_part); f(c->d_scal);
     for (int s = 0; s < 5; s++) for (int p = 0; p < 6; p++) f(c->d_plane[s][p]);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     for (auto e : c->prof_pool) (void)hipEventDestroy(e);
@@ -279,6 +293,92 @@ std::string fortran_e(double x, int w, int d)
     return buf;
 }
 
+// Whether the solver subtracts the mean of phi: pp, ppn and pn
+// (src/poisson.f90:397-408, 495-503, 1159-1171); npn, nnn and nn do not.
+bool removes_mean(const fen_ctx *c)
+{
+    return c->solver == FEN_SOLVER_PP || c->solver == FEN_SOLVER_PPN || c->solver == FEN_SOLVER_PN;
+}
+
+// Whether x is a cosine (Neumann) direction: nn, npn, nnn.
+bool x_cosine(const fen_ctx *c)
+{
+    return c->solver == FEN_SOLVER_NN || c->solver == FEN_SOLVER_NPN || c->solver == FEN_SOLVER_NNN;
+}
+
+// The Poisson solve up to the x backward transform: the x transform of
+// div(u*) rho/dt (vel) or of `direct` (strides sx, sy, ghost level gl), the
+// scalings of the reference solver, and the solve in the other directions.
+// Leaves phi's x spectrum in the spectrum buffer.
+template <class R>
+void solve_to_x_spectrum(fen_ctx *c, const R *const *vel, const R *direct, long sx, long sy, int gl, double rho,
+                         double dt)
+{
+    using namespace fen;
+    const double nx = (double)c->nx, ny = (double)c->ny;
+    switch (c->solver) {
+    case FEN_SOLVER_PPP:
+        launch_div_r2c_impl<R>(c, vel, direct, sx, sy, gl, rho, dt, 1.0);
+        launch_y_fft<R>(c, -1, 1.0);
+        launch_z_solve<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny * c->nz));
+        launch_y_fft<R>(c, +1, 1.0);
+        return;
+    case FEN_SOLVER_PPN:
+        launch_div_r2c_impl<R>(c, vel, direct, sx, sy, gl, rho, dt, 1.0 / nx);
+        launch_y_fft<R>(c, -1, 1.0 / ny);
+        launch_thomas<R>(c);
+        launch_y_fft<R>(c, +1, 1.0);
+        return;
+    case FEN_SOLVER_PP:
+        launch_div_r2c_impl<R>(c, vel, direct, sx, sy, gl, rho, dt, 1.0);
+        launch_y_solve_2d<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny));
+        return;
+    default: break;
+    }
+    if constexpr (std::is_same_v<R, double>) {
+        // scalings are powers of two: multiplying by the reciprocal equals the
+        // reference's division
+        switch (c->solver) {
+        case FEN_SOLVER_PN:   // src/poisson.f90:330-384
+            launch_div_r2c_impl<double>(c, vel, direct, sx, sy, gl, rho, dt, 1.0 / nx);
+            launch_thomas_neumann(c);
+            return;
+        case FEN_SOLVER_NN:   // :530-580; the 1/(2nx) follows the x DCT-III
+            launch_dct2_x(c, vel, direct, sx, sy, gl, rho, dt, 1.0);
+            launch_thomas_neumann(c);
+            return;
+        case FEN_SOLVER_NPN:  // :1205-1291
+            launch_dct2_x(c, vel, direct, sx, sy, gl, rho, dt, 1.0 / (2.0 * nx));
+            launch_rfft_y(c, 0, 1.0 / ny);
+            launch_thomas_neumann(c);
+            launch_rfft_y(c, 1, 1.0);
+            return;
+        case FEN_SOLVER_NNN:  // :1344-1430
+            launch_dct2_x(c, vel, direct, sx, sy, gl, rho, dt, 1.0 / (2.0 * nx));
+            launch_dct_y(c, 0, 1.0 / (2.0 * ny));
+            launch_thomas_neumann(c);
+            launch_dct_y(c, 1, 1.0);
+            return;
+        default: break;
+        }
+    }
+    throw std::runtime_error("solver not available for this precision");
+}
+
+// x backward transform of the spectrum into the interior of phi; sums the
+// interior for the mean when `mean`; writes the periodic images in `img`.
+template <class R> void x_backward(fen_ctx *c, R *phi, int gl, long sx, long sy, bool mean, int img)
+{
+    if constexpr (std::is_same_v<R, double>) {
+        if (x_cosine(c)) {
+            // nn divides by 2nx after the DCT-III (src/poisson.f90:594)
+            fen::launch_dct3_x(c, phi, gl, sx, sy, c->solver == FEN_SOLVER_NN ? 1.0 / (2.0 * (double)c->nx) : 1.0);
+            return;
+        }
+    }
+    fen::launch_c2r<R>(c, phi, gl, sx, sy, mean, img);
+}
+
 // One time step; see the kernel sequence at the top of this file.
 template <class R> void advance(fen_ctx *c, double dt)
 {
@@ -297,30 +397,14 @@ template <class R> void advance(fen_ctx *c, double dt)
         launch_ghost_desc<R>(c, g, d3 ? 3 : 2);
     }
     const R *vs[3] = {as<R>(c->d_us[0]), as<R>(c->d_us[1]), as<R>(c->d_us[2])};
-    const double post_x = c->solver == FEN_SOLVER_PPN ? 1.0 / (double)c->nx : 1.0;
-    launch_div_r2c_impl<R>(c, vs, nullptr, 0, 0, 0, c->density, dt, post_x);
-    switch (c->solver) {
-    case FEN_SOLVER_PPP:
-        launch_y_fft<R>(c, -1, 1.0);
-        launch_z_solve<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny * c->nz));
-        launch_y_fft<R>(c, +1, 1.0);
-        break;
-    case FEN_SOLVER_PPN:
-        launch_y_fft<R>(c, -1, 1.0 / (double)c->ny);
-        launch_thomas<R>(c);
-        launch_y_fft<R>(c, +1, 1.0);
-        break;
-    case FEN_SOLVER_PP:
-        launch_y_solve_2d<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny));
-        break;
-    default: throw std::runtime_error("solver not available");
-    }
-    const bool mean = c->solver == FEN_SOLVER_PP || c->solver == FEN_SOLVER_PPN;
+    solve_to_x_spectrum<R>(c, vs, nullptr, 0, 0, 0, c->density, dt);
+    const bool mean = removes_mean(c);
     bool dirichlet = false;
     for (int f = 0; f < 6; f++) dirichlet = dirichlet || c->bc_type[FEN_PHI][f] == 1;
-    // images written by the c2r only when no shift has to be applied first
-    const int img_phi = (mean && dirichlet) ? 0 : c->img_phi;
-    launch_c2r<R>(c, as<R>(c->d_phi), 1, c->box.gx, c->box.gxy, mean, img_phi);
+    // images written by the c2r only when no shift has to be applied first;
+    // the DCT-III writes none
+    const int img_phi = ((mean && dirichlet) || x_cosine(c)) ? 0 : c->img_phi;
+    x_backward<R>(c, as<R>(c->d_phi), 1, c->box.gx, c->box.gxy, mean, img_phi);
     c->shift_active = mean;
     if (mean && dirichlet) {   // the shift does not commute with 2*bc - f: apply it first
         launch_apply_shift<R>(c, as<R>(c->d_phi), c->n_ghost);
@@ -378,7 +462,7 @@ void advance_graph(fen_ctx *c, double dt)
     } else {
         c->last_dt = dt;
         c->step_count++;
-        const bool mean = c->solver == FEN_SOLVER_PP || c->solver == FEN_SOLVER_PPN;
+        const bool mean = removes_mean(c);
         bool dirichlet = false;
         for (int f = 0; f < 6; f++) dirichlet = dirichlet || c->bc_type[FEN_PHI][f] == 1;
         c->shift_active = mean && !dirichlet;
@@ -419,16 +503,27 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
     c->periodic[1] = c->bc_phys[2] == FEN_BC_PERIODIC && c->bc_phys[3] == FEN_BC_PERIODIC;
     c->periodic[2] = dim == 3 ? (c->bc_phys[4] == FEN_BC_PERIODIC && c->bc_phys[5] == FEN_BC_PERIODIC) : 1;
     const int *pb = c->periodic;
+    // src/poisson.f90:70-111; other combinations stop in the reference (:91-94)
     if (dim == 3) {
         if (pb[0] && pb[1] && pb[2]) c->solver = FEN_SOLVER_PPP;
         else if (pb[0] && pb[1] && !pb[2]) c->solver = FEN_SOLVER_PPN;
+        else if (!pb[0] && pb[1] && !pb[2]) c->solver = FEN_SOLVER_NPN;
+        else if (!pb[0] && !pb[1] && !pb[2]) c->solver = FEN_SOLVER_NNN;
         else { delete c; return FEN_ERR_UNSUPPORTED; }
     } else {
         if (pb[0] && pb[1]) c->solver = FEN_SOLVER_PP;
+        else if (pb[0] && !pb[1]) c->solver = FEN_SOLVER_PN;
+        else if (!pb[0] && !pb[1]) c->solver = FEN_SOLVER_NN;
         else { delete c; return FEN_ERR_UNSUPPORTED; }
     }
-    if (!pow2(nx) || !pow2(ny) || (c->solver == FEN_SOLVER_PPP && !pow2(nz)) ||
-        (dim == 3 && nz < 2)) {
+    const int sv = c->solver;
+    const bool neumann_solver = sv == FEN_SOLVER_PN || sv == FEN_SOLVER_NN || sv == FEN_SOLVER_NPN ||
+                                sv == FEN_SOLVER_NNN;
+    // transformed directions need power-of-two lengths, and so do the rows of
+    // the x r2c / c2r (pn: its Thomas direction y); Thomas directions >= 2
+    const bool y_transformed = !(sv == FEN_SOLVER_PN || sv == FEN_SOLVER_NN);
+    if (!pow2(nx) || ((y_transformed || sv == FEN_SOLVER_PN) && !pow2(ny)) || (sv == FEN_SOLVER_PPP && !pow2(nz)) || (dim == 3 && nz < 2) ||
+        ny < 2 || (neumann_solver && precision != FEN_FP64)) {
         delete c;
         return FEN_ERR_UNSUPPORTED;
     }
@@ -460,28 +555,42 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
             for (int f = 0; f < 6; f++) c->d_plane[s][f] = dev_alloc_bytes(plane_count(c, f) * es);
         const size_t nspec = (size_t)c->hx * ny * nz;
         c->d_spec = dev_alloc_bytes(nspec * 2 * es);
+        const size_t hy = (size_t)ny / 2 + 1;
         if (c->solver == FEN_SOLVER_PPN) {
             c->d_c1 = dev_alloc<double>(nspec);
             if (precision != FEN_FP64) c->d_dd = dev_alloc<double2>(nspec);
         }
-        c->d_mwx = dev_copy(modified_wavenumbers(nx, c->delta, 2.0));
-        c->d_mwy = dev_copy(modified_wavenumbers(ny, c->delta, 2.0));
+        if (c->solver == FEN_SOLVER_PN) c->d_c1 = dev_alloc<double>((size_t)c->hx * ny);
+        if (c->solver == FEN_SOLVER_NN || c->solver == FEN_SOLVER_NNN) c->d_c1 = dev_alloc<double>(c->n_int);
+        if (c->solver == FEN_SOLVER_NPN) {
+            c->d_spec2 = dev_alloc<double2>((size_t)nx * nz * hy);
+            c->d_c1 = dev_alloc<double>((size_t)nx * nz * hy);
+        }
+        // DCT directions use the pi-based wavenumbers (src/poisson.f90:265,794,881,899)
+        c->d_mwx = dev_copy(modified_wavenumbers(nx, c->delta, pb[0] ? 2.0 : 1.0));
+        c->d_mwy = dev_copy(modified_wavenumbers(ny, c->delta, pb[1] ? 2.0 : 1.0));
         if (c->solver == FEN_SOLVER_PPP) c->d_mwz = dev_copy(modified_wavenumbers(nz, c->delta, 2.0));
-        if (c->solver == FEN_SOLVER_PPN) {
-            // src/poisson.f90:741-757
+        // tridiagonal coefficients along z (ppn, npn, nnn) or y (pn, nn): src/poisson.f90:214-226,741-757
+        const bool thomas_y = c->solver == FEN_SOLVER_PN || c->solver == FEN_SOLVER_NN;
+        if (thomas_y || c->solver == FEN_SOLVER_PPN || c->solver == FEN_SOLVER_NPN || c->solver == FEN_SOLVER_NNN) {
+            const int n = thomas_y ? ny : nz;
+            const int lo_face = thomas_y ? FEN_BOTTOM : FEN_FRONT, hi_face = thomas_y ? FEN_TOP : FEN_BACK;
             const double d2 = c->delta * c->delta;
-            std::vector<double> a(nz, 1.0 / d2), bb(nz, -(2.0 / d2)), cc(nz, 1.0 / d2);
+            std::vector<double> a(n, 1.0 / d2), bb(n, -(2.0 / d2)), cc(n, 1.0 / d2);
             bb[0] = bb[0] + a[0];
-            if (c->bc_phys[4] == FEN_BC_INFLOW && c->bc_phys[5] == FEN_BC_OUTFLOW) bb[nz - 1] = bb[nz - 1] - cc[nz - 1];
-            else bb[nz - 1] = bb[nz - 1] + cc[nz - 1];
+            if (c->bc_phys[lo_face] == FEN_BC_INFLOW && c->bc_phys[hi_face] == FEN_BC_OUTFLOW)
+                bb[n - 1] = bb[n - 1] - cc[n - 1];
+            else bb[n - 1] = bb[n - 1] + cc[n - 1];
             a[0] = 0.0;
-            cc[nz - 1] = 0.0;
+            cc[n - 1] = 0.0;
             c->d_ta = dev_copy(a); c->d_tb = dev_copy(bb); c->d_tc = dev_copy(cc);
         }
+        if (!pb[0]) c->d_mkx = dev_copy(dct_twiddles(nx));
+        if (c->solver == FEN_SOLVER_NNN) c->d_mky = dev_copy(dct_twiddles(ny));
         with_type(c, [&](auto tag) {
             using R = decltype(tag);
             c->d_twx = dev_copy(twiddles<R>(nx));
-            c->d_twy = dev_copy(twiddles<R>(ny));
+            if (y_transformed) c->d_twy = dev_copy(twiddles<R>(ny));
             if (c->solver == FEN_SOLVER_PPP) c->d_twz = dev_copy(twiddles<R>(nz));
         });
         c->n_part = std::max<size_t>(std::max<size_t>(3 * 1024, (size_t)ny * nz / 2 + 1),
@@ -726,28 +835,11 @@ int fen_poisson_solve(fen_ctx *c, double *f, int gl)
     FEN_TRY(c, {
         void *buf = c->d_us[0];   // scratch between steps
         copy_in(c, buf, f, n);
-        const double post_x = c->solver == FEN_SOLVER_PPN ? 1.0 / (double)c->nx : 1.0;
-        const bool mean = c->solver != FEN_SOLVER_PPP;
+        const bool mean = removes_mean(c);
         with_type(c, [&](auto tag) {
             using R = decltype(tag);
-            fen::launch_div_r2c_impl<R>(c, nullptr, as<R>(buf), sx, sy, gl, 1.0, 1.0, post_x);
-            switch (c->solver) {
-            case FEN_SOLVER_PPP:
-                fen::launch_y_fft<R>(c, -1, 1.0);
-                fen::launch_z_solve<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny * c->nz));
-                fen::launch_y_fft<R>(c, +1, 1.0);
-                break;
-            case FEN_SOLVER_PPN:
-                fen::launch_y_fft<R>(c, -1, 1.0 / (double)c->ny);
-                fen::launch_thomas<R>(c);
-                fen::launch_y_fft<R>(c, +1, 1.0);
-                break;
-            case FEN_SOLVER_PP:
-                fen::launch_y_solve_2d<R>(c, 1.0 / (double)(float)((long)c->nx * c->ny));
-                break;
-            default: throw std::runtime_error("solver not available");
-            }
-            fen::launch_c2r<R>(c, as<R>(buf), gl, sx, sy, mean, 0);
+            solve_to_x_spectrum<R>(c, nullptr, as<R>(buf), sx, sy, gl, 1.0, 1.0);
+            x_backward<R>(c, as<R>(buf), gl, sx, sy, mean, 0);
         });
         std::vector<double> host(n);
         copy_out(c, host.data(), buf, n);
@@ -755,19 +847,32 @@ int fen_poisson_solve(fen_ctx *c, double *f, int gl)
             FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal + 3, c->d_shift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
             sync(c);
         }
-        if (c->solver == FEN_SOLVER_PPN && gl) {
-            // phi%f = 0 over the whole array before the c2r (src/poisson.f90:1151)
-            const size_t gx = c->nx + 2, gy = c->ny + 2;
-            for (size_t q = 0; q < n; q++) {
-                const size_t i = q % gx, j = (q / gx) % gy, k = q / (gx * gy);
-                const bool interior = i >= 1 && i <= (size_t)c->nx && j >= 1 && j <= (size_t)c->ny && k >= 1 &&
-                                      k <= (size_t)c->nz;
-                if (!interior) host[q] = 0.0;
-            }
+        // ghosts: the x backward transform writes the interior only
+        const size_t gx = c->nx + 2, gy = c->ny + 2;
+        auto interior = [&](size_t q) {
+            const size_t i = q % gx, j = (q / gx) % gy, k = q / (gx * gy);
+            return i >= 1 && i <= (size_t)c->nx && j >= 1 && j <= (size_t)c->ny && k >= 1 && k <= (size_t)c->nz;
+        };
+        const int sv = c->solver;
+        if (gl && (sv == FEN_SOLVER_PPN || sv == FEN_SOLVER_NPN || sv == FEN_SOLVER_NNN)) {
+            // phi%f = 0 over the whole array before the x backward transform
+            // (src/poisson.f90:1151, 1297, 1436)
+            for (size_t q = 0; q < n; q++)
+                if (!interior(q)) host[q] = 0.0;
+        }
+        if (gl && sv == FEN_SOLVER_NN) {   // whole array / float(nx*2) (:594)
+            const double s2 = (double)(float)(2 * c->nx);
+            for (size_t q = 0; q < n; q++)
+                if (!interior(q)) host[q] = host[q] / s2;
         }
         if (mean) {
             const double s = c->h_scal[3];
-            for (size_t q = 0; q < n; q++) host[q] = host[q] - s;   // whole array (:503, :1171)
+            if (sv == FEN_SOLVER_PN) {   // interior only (:408)
+                for (size_t q = 0; q < n; q++)
+                    if (!gl || interior(q)) host[q] = host[q] - s;
+            } else {
+                for (size_t q = 0; q < n; q++) host[q] = host[q] - s;   // whole array (:503, :1171)
+            }
         }
         std::memcpy(f, host.data(), n * sizeof(double));
     });

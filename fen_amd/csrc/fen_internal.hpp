@@ -190,6 +190,8 @@ struct fen_ctx {
     void *d_spec = nullptr;      // complex R, [kx][k][j]
     double *d_c1 = nullptr;      // Thomas scratch (ppn), [kx][k][j]
     double2 *d_dd = nullptr;     // FP64 Thomas right-hand side (ppn, FP32 form), [kx][k][j]
+    double2 *d_spec2 = nullptr;  // npn: y half spectrum of the real x-DCT lines, [kx][k][jy]
+    double2 *d_mkx = nullptr, *d_mky = nullptr;   // (cos, sin)(pi k/(2n)), k < n: DCTs along x, y
     double *d_mwx = nullptr, *d_mwy = nullptr, *d_mwz = nullptr;
     double *d_ta = nullptr, *d_tb = nullptr, *d_tc = nullptr;
     void *d_twx = nullptr, *d_twy = nullptr, *d_twz = nullptr;   // complex R, W_n^m, m < n
@@ -253,6 +255,13 @@ template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long
 template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out);
 template <class R> void launch_checks(fen_ctx *c, const R *const *vel, double *out2);
 long stencil_blocks(const Box &b);   // workgroups of the momentum / projection grids
+// Neumann-direction solver passes (neumann_kernels.hip, FP64).
+void launch_dct2_x(fen_ctx *c, const double *const *vel, const double *direct, long sx, long sy, int gl, double rho,
+                   double dt, double post);
+void launch_dct3_x(fen_ctx *c, double *phi, int gl, long sx, long sy, double post);
+void launch_dct_y(fen_ctx *c, int inverse, double post);
+void launch_rfft_y(fen_ctx *c, int inverse, double post);
+void launch_thomas_neumann(fen_ctx *c);
 template <class R> void launch_apply_shift(fen_ctx *c, R *f, size_t n);
 void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz);   // FP64 contexts
 

@@ -1,0 +1,481 @@
+// neumann_kernels.hip -- FP64 passes of the Poisson solvers with Neumann
+// (non-periodic) directions: pn and nn in 2-D, npn and nnn in 3-D
+// (reference src/poisson.f90:306-596, 1177-1451).
+//
+// DCT-II (FFTW_REDFT10, y_k = 2 sum_j x_j cos(pi k (2j+1)/(2n))) and DCT-III
+// (FFTW_REDFT01, y_j = x_0 + 2 sum_{k>=1} x_k cos(pi k (2j+1)/(2n))) use
+// Makhoul's length-n complex FFT form, two real lines packed in one complex
+// line:
+//   forward: v = (x_0, x_2, ..., x_{n-2}, x_{n-1}, ..., x_3, x_1), V = FFT(v),
+//            y_k = 2 Re(exp(-i pi k/(2n)) V_k);
+//   inverse: U_k = (x_k - i x_{n-k}) exp(i pi k/(2n)) (x_n = 0), u = IFFT(U)
+//            (unnormalised, real), y_{2m} = u_m, y_{2m+1} = u_{n-1-m}.
+// The transforms run in LDS with the Stockham kernels of fft_device.hpp.
+// Real spectra are laid out [kx][k][j] (j fastest), as the complex one.
+#include <stdexcept>
+
+#include "fen_internal.hpp"
+#include "fft_device.hpp"
+
+namespace fen {
+
+namespace {
+
+constexpr int kNB = 256;
+
+template <int N> constexpr int nb_tpl() { return N / 8; }           // threads per complex line
+template <int N> constexpr int nb_lines() { return kNB / (N / 8); } // complex lines per block
+
+// Position of x_i in Makhoul's reordered sequence v.
+__device__ __forceinline__ int mk_pos(int i, int n) { return (i & 1) ? n - 1 - (i >> 1) : (i >> 1); }
+
+// ---------------------------------------------------------------------------
+// DCT-II of 2L rows per block.  Source rows: SRC 0 = div(u*)*rho/dt on the
+// ghosted velocity (x pass), 1 = interior rows of phi with strides (x pass,
+// direct solve), 2 = contiguous real lines of length N (y pass).  Output
+// element k of row r goes to out[k*ok + r*orr], times `post`.
+// ---------------------------------------------------------------------------
+struct DctArgs {
+    const double *u, *v, *w;      // SRC 0: velocity; SRC 1/2: u is the source
+    double *out;
+    const double2 *tw;            // W_N^m, m < N
+    const double2 *mk;            // (cos, sin)(pi k/(2N)), k < N
+    Box b;
+    double id, rho, dt, post;
+    long nrows, ok, orr;          // rows, output strides
+    long sx, sy;                  // SRC 1: strides of the phi array
+    int gl, d3;
+};
+
+template <int N, int SRC>
+__global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
+{
+    constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), TJ = 2 * L, LL = lds_line<N>();
+    __shared__ double2 buf[L * LL];
+    __shared__ double2 tw[N];
+    load_twiddles<N>(tw, a.tw);
+    const long row0 = (long)blockIdx.x * TJ;
+    for (int e = threadIdx.x; e < TJ * N; e += kNB) {
+        const int row = e / N, i = e % N;
+        const long r = row0 + row;
+        double val = 0.0;
+        if (r < a.nrows) {
+            if constexpr (SRC == 2) {
+                val = a.u[r * N + i];
+            } else {
+                const int j = (int)(r % a.b.ny) + 1, k = (int)(r / a.b.ny) + 1;
+                if constexpr (SRC == 1) {
+                    val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
+                } else {
+                    const long g = a.b.g(i + 1, j, k);
+                    double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
+                    if (a.d3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
+                    val = d * a.rho / a.dt;
+                }
+            }
+        }
+        double2 *line = buf + (row >> 1) * LL;
+        if (row & 1) line[lds_pad(mk_pos(i, N))].y = val;
+        else line[lds_pad(mk_pos(i, N))].x = val;
+    }
+    __syncthreads();
+    fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
+    for (int e = threadIdx.x; e < TJ * N; e += kNB) {
+        const int row = e % TJ, k = e / TJ;
+        const long r = row0 + row;
+        if (r >= a.nrows) continue;
+        const double2 *line = buf + (row >> 1) * LL;
+        const double2 zk = line[lds_pad(k)], zn = line[lds_pad((N - k) & (N - 1))];
+        const double c = a.mk[k].x, s = a.mk[k].y;
+        double y;
+        if ((row & 1) == 0) y = c * (zk.x + zn.x) + s * (zk.y - zn.y);
+        else y = c * (zk.y + zn.y) + s * (zn.x - zk.x);
+        a.out[k * a.ok + r * a.orr] = y * a.post;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DCT-III of 2L rows per block.  Input element k of row r at in[k*ik + r*ir].
+// DST 0: interior rows of phi (strides sx, sy, ghost level gl), times post;
+// DST 1: contiguous real lines of length N, times post.
+// ---------------------------------------------------------------------------
+struct Dct3Args {
+    const double *in;
+    double *out;
+    const double2 *tw, *mk;
+    long nrows, ik, ir;
+    long sx, sy;
+    int gl, ny;
+    double post;
+};
+
+template <int N, int DST>
+__global__ __launch_bounds__(kNB) void dct3_rows_kernel(Dct3Args a)
+{
+    constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), TJ = 2 * L, LL = lds_line<N>();
+    __shared__ double2 buf[L * LL];
+    __shared__ double2 tw[N];
+    load_twiddles<N>(tw, a.tw);
+    const long row0 = (long)blockIdx.x * TJ;
+    for (int e = threadIdx.x; e < L * N; e += kNB) {
+        const int l = e % L, k = e / L;
+        const long r = row0 + 2 * l;
+        double a1 = 0.0, b1 = 0.0, a2 = 0.0, b2 = 0.0;   // x_k and x_{N-k} of both rows
+        if (r < a.nrows) {
+            a1 = a.in[k * a.ik + r * a.ir];
+            if (k > 0) b1 = a.in[(N - k) * a.ik + r * a.ir];
+        }
+        if (r + 1 < a.nrows) {
+            a2 = a.in[k * a.ik + (r + 1) * a.ir];
+            if (k > 0) b2 = a.in[(N - k) * a.ik + (r + 1) * a.ir];
+        }
+        const double c = a.mk[k].x, s = a.mk[k].y;
+        const double2 U1 = double2{a1 * c + b1 * s, a1 * s - b1 * c};
+        const double2 U2 = double2{a2 * c + b2 * s, a2 * s - b2 * c};
+        buf[l * LL + lds_pad(k)] = double2{U1.x - U2.y, U1.y + U2.x};
+    }
+    __syncthreads();
+    fft_lds<N, 1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
+    for (int e = threadIdx.x; e < TJ * N; e += kNB) {
+        const int row = e / N, i = e % N;
+        const long r = row0 + row;
+        if (r >= a.nrows) continue;
+        const double2 z = buf[(row >> 1) * LL + lds_pad(mk_pos(i, N))];
+        const double val = ((row & 1) ? z.y : z.x) * a.post;
+        if constexpr (DST == 0) {
+            const long j = r % a.ny, k = r / a.ny;
+            a.out[(i + a.gl) + a.sx * (j + a.gl) + a.sy * (k + a.gl)] = val;
+        } else {
+            a.out[r * N + i] = val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Real FFT along contiguous lines of length N: A[r*N + j] -> B[r*H + jy],
+// H = N/2 + 1, times post (npn, y direction), and back (c2r, the imaginary
+// parts of bins 0 and N/2 ignored).
+// ---------------------------------------------------------------------------
+struct RfftArgs {
+    double *re;        // real lines
+    double2 *cx;       // half-spectrum lines
+    const double2 *tw;
+    long nrows;
+    double post;
+};
+
+template <int N>
+__global__ __launch_bounds__(kNB) void rfft_rows_kernel(RfftArgs a)
+{
+    constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), TJ = 2 * L, LL = lds_line<N>(), H = N / 2 + 1;
+    __shared__ double2 buf[L * LL];
+    __shared__ double2 tw[N];
+    load_twiddles<N>(tw, a.tw);
+    const long row0 = (long)blockIdx.x * TJ;
+    for (int e = threadIdx.x; e < TJ * N; e += kNB) {
+        const int row = e / N, j = e % N;
+        const long r = row0 + row;
+        const double val = r < a.nrows ? a.re[r * N + j] : 0.0;
+        double2 *line = buf + (row >> 1) * LL;
+        if (row & 1) line[lds_pad(j)].y = val;
+        else line[lds_pad(j)].x = val;
+    }
+    __syncthreads();
+    fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
+    for (int e = threadIdx.x; e < TJ * H; e += kNB) {
+        const int row = e / H, k = e % H;
+        const long r = row0 + row;
+        if (r >= a.nrows) continue;
+        const double2 *line = buf + (row >> 1) * LL;
+        const double2 zk = line[lds_pad(k)], zn = line[lds_pad((N - k) & (N - 1))];
+        double2 X;
+        if ((row & 1) == 0) X = double2{(zk.x + zn.x) * 0.5, (zk.y - zn.y) * 0.5};
+        else X = double2{(zk.y + zn.y) * 0.5, (zn.x - zk.x) * 0.5};
+        a.cx[r * H + k] = double2{X.x * a.post, X.y * a.post};
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(kNB) void irfft_rows_kernel(RfftArgs a)
+{
+    constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), LL = lds_line<N>(), H = N / 2 + 1;
+    __shared__ double2 buf[L * LL];
+    __shared__ double2 tw[N];
+    load_twiddles<N>(tw, a.tw);
+    const long row0 = (long)blockIdx.x * 2 * L;
+    for (int e = threadIdx.x; e < L * H; e += kNB) {
+        const int l = e / H, k = e % H;
+        const long r = row0 + 2 * l;
+        const double2 X1 = r < a.nrows ? a.cx[r * H + k] : double2{0.0, 0.0};
+        const double2 X2 = r + 1 < a.nrows ? a.cx[(r + 1) * H + k] : double2{0.0, 0.0};
+        double2 *line = buf + l * LL;
+        if (k == 0 || k == N / 2) {
+            line[lds_pad(k)] = double2{X1.x, X2.x};
+        } else {
+            line[lds_pad(k)] = double2{X1.x - X2.y, X1.y + X2.x};
+            line[lds_pad(N - k)] = double2{X1.x + X2.y, X2.x - X1.y};
+        }
+    }
+    __syncthreads();
+    fft_lds<N, 1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
+    for (int e = threadIdx.x; e < 2 * L * N; e += kNB) {
+        const int row = e / N, j = e % N;
+        const long r = row0 + row;
+        if (r >= a.nrows) continue;
+        const double2 z = buf[(row >> 1) * LL + lds_pad(j)];
+        a.re[r * N + j] = (row & 1) ? z.y : z.x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched Thomas sweeps over strided columns.  Column t = o*ni + q starts at
+// o*os + q and has stride s and length n; its diagonal shift is mwx[o]
+// (+ mwy[q] in 3-D).  E is double (real spectra) or double2 (complex).
+// ASSOC2D: the 2-D solvers' form (src/poisson.f90:346-384): divisions, c1
+// denominator (b - a c1) + mx, d denominator (b + mx) - a c1.  Otherwise
+// the 3-D form (:1094-1135, 1236-1280): reciprocal pivots, last row divides.
+// c1 lives in an FP64 scratch array laid out like the column elements.
+// ---------------------------------------------------------------------------
+struct ThomasGArgs {
+    void *x;
+    double *c1;
+    long ncol, ni, os, s;
+    int n;
+    const double *mwx, *mwy, *ta, *tb, *tc;
+};
+
+__device__ __forceinline__ double sub_(double a, double b) { return a - b; }
+__device__ __forceinline__ double2 sub_(double2 a, double2 b) { return double2{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ double mul_(double a, double f) { return a * f; }
+__device__ __forceinline__ double2 mul_(double2 a, double f) { return double2{a.x * f, a.y * f}; }
+__device__ __forceinline__ double div_(double a, double f) { return a / f; }
+__device__ __forceinline__ double2 div_(double2 a, double f) { return double2{a.x / f, a.y / f}; }
+template <class E> __device__ __forceinline__ E zero_();
+template <> __device__ __forceinline__ double zero_<double>() { return 0.0; }
+template <> __device__ __forceinline__ double2 zero_<double2>() { return double2{0.0, 0.0}; }
+
+template <class E, int ASSOC2D>
+__global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
+{
+    const long t = (long)blockIdx.x * kNB + threadIdx.x;
+    if (t >= a.ncol) return;
+    const long o = t / a.ni, q = t % a.ni;
+    const long base = o * a.os + q, s = a.s;
+    E *x = static_cast<E *>(a.x) + base;
+    double *c1 = a.c1 + base;
+    const double mx = a.mwx[o], my = ASSOC2D ? 0.0 : a.mwy[q];
+    const int n = a.n;
+    const double *A = a.ta, *B = a.tb, *C = a.tc;
+    E d;
+    double c1p;
+    if constexpr (ASSOC2D) {
+        c1p = C[0] / (B[0] + mx);
+        d = div_(x[0], B[0] + mx);
+        c1[0] = c1p;
+        x[0] = d;
+        for (int k = 1; k < n - 1; k++) {
+            const double cn = C[k] / ((B[k] - A[k] * c1p) + mx);
+            d = div_(sub_(x[k * s], mul_(d, A[k])), (B[k] + mx) - A[k] * c1p);
+            c1p = cn;
+            c1[k * s] = c1p;
+            x[k * s] = d;
+        }
+        const double frac = (B[n - 1] + mx) - A[n - 1] * c1p;
+        d = frac != 0.0 ? div_(sub_(x[(n - 1) * s], mul_(d, A[n - 1])), frac) : zero_<E>();
+    } else {
+        double factor = 1.0 / ((B[0] + mx) + my);
+        c1p = C[0] * factor;
+        c1[0] = c1p;
+        d = mul_(x[0], factor);
+        x[0] = d;
+        for (int k = 1; k < n - 1; k++) {
+            factor = 1.0 / (((B[k] + mx) + my) - A[k] * c1p);
+            c1p = C[k] * factor;
+            c1[k * s] = c1p;
+            d = mul_(sub_(x[k * s], mul_(d, A[k])), factor);
+            x[k * s] = d;
+        }
+        factor = ((B[n - 1] + mx) + my) - A[n - 1] * c1p;
+        d = factor != 0.0 ? div_(sub_(x[(n - 1) * s], mul_(d, A[n - 1])), factor) : zero_<E>();
+    }
+    x[(n - 1) * s] = d;
+    E xn = d;
+    for (int k = n - 2; k >= 0; k--) {
+        xn = sub_(x[k * s], mul_(xn, c1[k * s]));
+        x[k * s] = xn;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+#define FEN_NB_DISPATCH(n, MACRO)                                        \
+    switch (n) {                                                         \
+    case 8: MACRO(8); break;                                             \
+    case 16: MACRO(16); break;                                           \
+    case 32: MACRO(32); break;                                           \
+    case 64: MACRO(64); break;                                           \
+    case 128: MACRO(128); break;                                         \
+    case 256: MACRO(256); break;                                         \
+    case 512: MACRO(512); break;                                         \
+    case 1024: MACRO(1024); break;                                       \
+    default: throw std::runtime_error("unsupported transform length");   \
+    }
+
+static unsigned row_blocks(long nrows, int N) { return (unsigned)((nrows + 2 * (kNB / (N / 8)) - 1) / (2 * (kNB / (N / 8)))); }
+
+// x DCT-II of the rows of div(u*)*rho/dt (vel != nullptr) or of phi (direct)
+// into the real spectrum A[kx*nrows + r], times post.
+void launch_dct2_x(fen_ctx *c, const double *const *vel, const double *direct, long sx, long sy, int gl, double rho,
+                   double dt, double post)
+{
+    DctArgs a{};
+    if (vel) { a.u = vel[0]; a.v = vel[1]; a.w = vel[2]; }
+    else { a.u = direct; a.sx = sx; a.sy = sy; a.gl = gl; }
+    a.out = as<double>(c->d_spec);
+    a.tw = as<double2>(c->d_twx);
+    a.mk = c->d_mkx;
+    a.b = c->box;
+    a.id = 1.0 / c->delta;
+    a.rho = rho;
+    a.dt = dt;
+    a.post = post;
+    a.nrows = (long)c->ny * c->nz;
+    a.ok = a.nrows;
+    a.orr = 1;
+    a.d3 = c->dim == 3;
+    const unsigned grid = row_blocks(a.nrows, c->nx);
+    prof_begin(c, K_DIV_R2C);
+#define L_D2(NN)                                                                          \
+    if (vel) hipLaunchKernelGGL((dct2_rows_kernel<NN, 0>), dim3(grid), dim3(kNB), 0, c->stream, a); \
+    else hipLaunchKernelGGL((dct2_rows_kernel<NN, 1>), dim3(grid), dim3(kNB), 0, c->stream, a);
+    FEN_NB_DISPATCH(c->nx, L_D2)
+#undef L_D2
+    prof_end(c, K_DIV_R2C, (vel ? (c->dim == 3 ? 3.0 : 2.0) : 1.0) * 8.0 * c->n_int + 8.0 * c->n_int);
+}
+
+// x DCT-III of A[kx*nrows + r] into the interior rows of phi, times post.
+void launch_dct3_x(fen_ctx *c, double *phi, int gl, long sx, long sy, double post)
+{
+    Dct3Args a{};
+    a.in = as<double>(c->d_spec);
+    a.out = phi;
+    a.tw = as<double2>(c->d_twx);
+    a.mk = c->d_mkx;
+    a.nrows = (long)c->ny * c->nz;
+    a.ik = a.nrows;
+    a.ir = 1;
+    a.sx = sx;
+    a.sy = sy;
+    a.gl = gl;
+    a.ny = c->ny;
+    a.post = post;
+    const unsigned grid = row_blocks(a.nrows, c->nx);
+    prof_begin(c, K_C2R);
+#define L_D3(NN) hipLaunchKernelGGL((dct3_rows_kernel<NN, 0>), dim3(grid), dim3(kNB), 0, c->stream, a);
+    FEN_NB_DISPATCH(c->nx, L_D3)
+#undef L_D3
+    prof_end(c, K_C2R, 16.0 * c->n_int);
+}
+
+// y DCT-II / DCT-III in place on the real spectrum lines A[(kx*nz + k)*ny + j] (nnn).
+void launch_dct_y(fen_ctx *c, int inverse, double post)
+{
+    const long nrows = (long)c->nx * c->nz;
+    const unsigned grid = row_blocks(nrows, c->ny);
+    const int kid = inverse ? K_Y_INV : K_Y_FWD;
+    prof_begin(c, kid);
+    if (!inverse) {
+        DctArgs a{};
+        a.u = as<double>(c->d_spec);
+        a.out = as<double>(c->d_spec);
+        a.tw = as<double2>(c->d_twy);
+        a.mk = c->d_mky;
+        a.post = post;
+        a.nrows = nrows;
+        a.ok = 1;
+        a.orr = c->ny;
+#define L_DY(NN) hipLaunchKernelGGL((dct2_rows_kernel<NN, 2>), dim3(grid), dim3(kNB), 0, c->stream, a);
+        FEN_NB_DISPATCH(c->ny, L_DY)
+#undef L_DY
+    } else {
+        Dct3Args a{};
+        a.in = as<double>(c->d_spec);
+        a.out = as<double>(c->d_spec);
+        a.tw = as<double2>(c->d_twy);
+        a.mk = c->d_mky;
+        a.nrows = nrows;
+        a.ik = 1;
+        a.ir = c->ny;
+        a.post = post;
+#define L_DY3(NN) hipLaunchKernelGGL((dct3_rows_kernel<NN, 1>), dim3(grid), dim3(kNB), 0, c->stream, a);
+        FEN_NB_DISPATCH(c->ny, L_DY3)
+#undef L_DY3
+    }
+    prof_end(c, kid, 16.0 * c->n_int);
+}
+
+// y real FFT (forward: A -> B, times post) or its inverse (B -> A) (npn).
+void launch_rfft_y(fen_ctx *c, int inverse, double post)
+{
+    RfftArgs a{};
+    a.re = as<double>(c->d_spec);
+    a.cx = c->d_spec2;
+    a.tw = as<double2>(c->d_twy);
+    a.nrows = (long)c->nx * c->nz;
+    a.post = post;
+    const unsigned grid = row_blocks(a.nrows, c->ny);
+    const int kid = inverse ? K_Y_INV : K_Y_FWD;
+    prof_begin(c, kid);
+#define L_RF(NN)                                                                             \
+    if (inverse) hipLaunchKernelGGL((irfft_rows_kernel<NN>), dim3(grid), dim3(kNB), 0, c->stream, a); \
+    else hipLaunchKernelGGL((rfft_rows_kernel<NN>), dim3(grid), dim3(kNB), 0, c->stream, a);
+    FEN_NB_DISPATCH(c->ny, L_RF)
+#undef L_RF
+    prof_end(c, kid, 16.0 * c->n_int);
+}
+
+// Thomas sweeps of the pn, nn, npn and nnn solvers.
+void launch_thomas_neumann(fen_ctx *c)
+{
+    ThomasGArgs a{};
+    a.c1 = c->d_c1;
+    a.mwx = c->d_mwx;
+    a.mwy = c->d_mwy;
+    a.ta = c->d_ta;
+    a.tb = c->d_tb;
+    a.tc = c->d_tc;
+    const int hy = c->ny / 2 + 1;
+    bool complex_e = false, assoc2d = false;
+    switch (c->solver) {
+    case FEN_SOLVER_PN:   // complex [kx][j], columns along j
+        a.x = c->d_spec; complex_e = true; assoc2d = true;
+        a.ncol = c->hx; a.ni = 1; a.os = c->ny; a.s = 1; a.n = c->ny;
+        break;
+    case FEN_SOLVER_NN:   // real [kx][j]
+        a.x = c->d_spec; assoc2d = true;
+        a.ncol = c->nx; a.ni = 1; a.os = c->ny; a.s = 1; a.n = c->ny;
+        break;
+    case FEN_SOLVER_NPN:  // complex [kx][k][jy], columns along k
+        a.x = c->d_spec2; complex_e = true;
+        a.ncol = (long)c->nx * hy; a.ni = hy; a.os = (long)c->nz * hy; a.s = hy; a.n = c->nz;
+        break;
+    case FEN_SOLVER_NNN:  // real [kx][k][j]
+        a.x = c->d_spec;
+        a.ncol = (long)c->nx * c->ny; a.ni = c->ny; a.os = (long)c->nz * c->ny; a.s = c->ny; a.n = c->nz;
+        break;
+    default: throw std::runtime_error("not a Neumann solver");
+    }
+    if (a.n < 2) throw std::runtime_error("the tridiagonal direction needs at least 2 cells");
+    const dim3 grid((unsigned)((a.ncol + kNB - 1) / kNB));
+    prof_begin(c, K_THOMAS);
+    if (complex_e && assoc2d) hipLaunchKernelGGL((thomas_general_kernel<double2, 1>), grid, dim3(kNB), 0, c->stream, a);
+    else if (complex_e) hipLaunchKernelGGL((thomas_general_kernel<double2, 0>), grid, dim3(kNB), 0, c->stream, a);
+    else if (assoc2d) hipLaunchKernelGGL((thomas_general_kernel<double, 1>), grid, dim3(kNB), 0, c->stream, a);
+    else hipLaunchKernelGGL((thomas_general_kernel<double, 0>), grid, dim3(kNB), 0, c->stream, a);
+    prof_end(c, K_THOMAS, 32.0 * (double)a.ncol * a.n);
+}
+
+}  // namespace fen

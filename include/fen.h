@@ -77,9 +77,12 @@ enum { FEN_SOLVER_PP = 0, FEN_SOLVER_PN = 1, FEN_SOLVER_NN = 2, FEN_SOLVER_PPP =
 /* Replaces grid%setup (src/grid.f90:67-200) + init_solver (src/solver.f90:34-99):
  * allocate_navier_stokes_fields (src/navier_stokes.f90:752-1067) and
  * init_poisson_solver (src/poisson.f90:57-113).  dim is 2 or 3 (CPP DIM);
- * bc[6] holds FEN_BC_* codes for left,right,bottom,top,front,back.  Grid
- * sizes must be powers of two (>= 8) in the FFT directions.  Returns
- * FEN_ERR_UNSUPPORTED where the reference stops (src/poisson.f90:91-94). */
+ * bc[6] holds FEN_BC_* codes for left,right,bottom,top,front,back; the
+ * periodic tuple selects the solver: pp, pn, nn (2-D), ppp, ppn, npn, nnn
+ * (3-D).  Grid sizes must be powers of two (8..1024) in the transform
+ * directions (and in y for pn); the tridiagonal direction needs >= 2 cells.
+ * pn, nn, npn and nnn are FP64 only.  Returns FEN_ERR_UNSUPPORTED for those
+ * limits and where the reference stops (src/poisson.f90:91-94, 106-109). */
 int fen_create(fen_ctx **ctx, int dim, int nx, int ny, int nz, double lx, double ly, double lz,
                const double *x0, const int *bc, int precision);
 

@@ -27,10 +27,13 @@ def _create(dim=3, n=(16, 16, 16), bc=(0, 0, 0, 0, 0, 0), precision=0, out=True)
 @pytest.mark.parametrize("kwargs,code", [
     (dict(out=False), -1),                          # no output pointer
     (dict(dim=4), -1),                              # DIM must be 2 or 3
-    (dict(bc=(1, 1, 0, 0, 1, 1)), -2),              # npn: not available (reference: src/poisson.f90:84-85)
     (dict(bc=(0, 0, 1, 1, 0, 0)), -2),              # (T,F,T): the reference stops (src/poisson.f90:91-94)
-    (dict(dim=2, bc=(1, 1, 1, 1, 0, 0)), -2),       # 2-D nn: not available
+    (dict(dim=2, bc=(1, 1, 0, 0, 0, 0)), -2),       # 2-D (F,T): the reference stops (:106-109)
     (dict(n=(12, 16, 16)), -2),                     # FFT lengths are powers of two
+    (dict(bc=(1, 1, 1, 1, 1, 1), n=(16, 12, 16)), -2),   # nnn: y is a transform direction
+    (dict(dim=2, bc=(0, 0, 1, 1, 0, 0), n=(16, 12, 1)), -2),   # pn: y spectra use power-of-two rows
+    (dict(bc=(1, 1, 0, 0, 1, 1), precision=1), -2),     # npn, nnn, pn, nn: FP64 only
+    (dict(dim=2, bc=(1, 1, 1, 1, 0, 0), precision=1), -2),
     (dict(precision=7), -2),
 ])
 def test_create_rejects_before_device_work(kwargs, code):
