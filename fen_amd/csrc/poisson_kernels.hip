@@ -290,13 +290,18 @@ __global__ __launch_bounds__(NT) void z_solve_kernel(ZArgs<R> a)
     const int j0 = blockIdx.x * L;
     C *base = a.spec + kx * a.kxs + j0;
     constexpr int PER = L * N / NT;
+    static_assert(NT % L == 0, "a thread keeps its line in the divide loop");
+    const int lt = threadIdx.x % L;
+    const double mxy = j0 + lt < a.ny ? a.mwx[kx] + a.mwy[j0 + lt] : 0.0;
     C v[PER];
+    double mz[PER];   // z wavenumbers of this thread's divide-loop elements, loaded with the line
 #pragma unroll
     for (int m = 0; m < PER; m++) {
         const int e = threadIdx.x + m * NT;
         const int l = e % L, k = e / L;
         const C x = base[(long)k * a.ny + (j0 + l < a.ny ? l : 0)];
         v[m] = (j0 + l < a.ny) ? x : C{R(0), R(0)};
+        mz[m] = a.mwz[k];
     }
 #pragma unroll
     for (int m = 0; m < PER; m++) {
@@ -306,12 +311,13 @@ __global__ __launch_bounds__(NT) void z_solve_kernel(ZArgs<R> a)
     __syncthreads();
     C *line = buf + (threadIdx.x / T) * LL;
     fft_lds<N, -1>(line, threadIdx.x % T, tw);
-    const double mx = a.mwx[kx];
-    for (int e = threadIdx.x; e < L * N; e += NT) {
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * NT;
         const int l = e % L, k = e / L;
         if (j0 + l >= a.ny) continue;
         C X = buf[l * LL + lds_pad(k)];
-        const double s = (mx + a.mwy[j0 + l]) + a.mwz[k];
+        const double s = mxy + mz[m];
         if (s == 0.0) X = C{R(0), R(0)};
         else X = scale_div(X, a.inv_n, s);
         buf[l * LL + lds_pad(k)] = X;

@@ -372,18 +372,36 @@ __global__ __launch_bounds__(kThreads) void project_kernel_2d(ProjArgs<R> a)
     block_max3<kThreads>(md, mv, bad, a.part, (long)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
-__global__ __launch_bounds__(256) void checks_final_kernel(const double *part, long n, double *out)
+// One workgroup reduces the per-workgroup partials.  Each thread issues the
+// loads of B entries before comparing any, so the pass costs about one
+// memory latency per B entries instead of one per entry.
+constexpr int kFinalThreads = 1024;
+__global__ __launch_bounds__(kFinalThreads) void checks_final_kernel(const double *part, long n, double *out)
 {
-    __shared__ double sd[256], sv[256], sf[256];
+    constexpr int NT = kFinalThreads, B = 4;
+    __shared__ double sd[NT], sv[NT], sf[NT];
     double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
-    for (long q = threadIdx.x; q < n; q += 256) {
-        if (part[3 * q] > md) md = part[3 * q];
-        if (part[3 * q + 1] > mv) mv = part[3 * q + 1];
-        if (part[3 * q + 2] > bad) bad = part[3 * q + 2];
+    for (long q0 = threadIdx.x; q0 < n; q0 += (long)NT * B) {
+        double x[B][3];
+#pragma unroll
+        for (int m = 0; m < B; m++) {
+            const long q = q0 + (long)m * NT;
+            const long qq = q < n ? q : q0;   // a valid entry; its value is dropped below
+            x[m][0] = part[3 * qq];
+            x[m][1] = part[3 * qq + 1];
+            x[m][2] = part[3 * qq + 2];
+        }
+#pragma unroll
+        for (int m = 0; m < B; m++) {
+            if (q0 + (long)m * NT >= n) continue;
+            if (x[m][0] > md) md = x[m][0];
+            if (x[m][1] > mv) mv = x[m][1];
+            if (x[m][2] > bad) bad = x[m][2];
+        }
     }
     sd[threadIdx.x] = md; sv[threadIdx.x] = mv; sf[threadIdx.x] = bad;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             if (sd[threadIdx.x + s] > sd[threadIdx.x]) sd[threadIdx.x] = sd[threadIdx.x + s];
             if (sv[threadIdx.x + s] > sv[threadIdx.x]) sv[threadIdx.x] = sv[threadIdx.x + s];
@@ -439,8 +457,8 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     }
     prof_end(c, K_PROJECT, (d3 ? 9.0 : 7.0) * sizeof(R) * c->n_int);
     prof_begin(c, K_REDUCE);
-    hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(256), 0, c->stream, (const double *)c->d_part, nblk,
-                       checks_out);
+    hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(kFinalThreads), 0, c->stream, (const double *)c->d_part,
+                       nblk, checks_out);
     prof_end(c, K_REDUCE, 24.0 * nblk);
 }
 
@@ -504,7 +522,8 @@ template <class R> void launch_checks(fen_ctx *c, const R *const *vel, double *o
                            vel[2], b, id, c->d_part);
     prof_end(c, K_CHECKS, (b.dim == 3 ? 3.0 : 2.0) * sizeof(R) * c->n_int);
     prof_begin(c, K_REDUCE);
-    hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(256), 0, c->stream, (const double *)c->d_part, nblk, out);
+    hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(kFinalThreads), 0, c->stream, (const double *)c->d_part,
+                       nblk, out);
     prof_end(c, K_REDUCE, 24.0 * nblk);
 }
 
