@@ -64,7 +64,7 @@ namespace {
 
 const char *kKernelNames[fen::K_COUNT] = {
     "momentum", "ghost", "div_r2c", "y_fwd", "z_solve", "thomas", "y_inv", "y_solve", "c2r", "project",
-    "checks", "reduce"};
+    "checks", "reduce", "project_momentum"};
 
 int fail(fen_ctx *c, int code, const std::string &msg)
 {
@@ -257,7 +257,7 @@ void free_all(fen_ctx *c)
     if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     for (int d = 0; d < 3; d++) { f(c->d_u[d]); f(c->d_us[d]); f(c->d_dvo[d]); f(c->d_S[d]); }
-    f(c->d_p); f(c->d_phi); f(c->d_spec); f(c->d_c1); f(c->d_dd);
+    f(c->d_p); f(c->d_p2); f(c->d_phi); f(c->d_spec); f(c->d_c1); f(c->d_dd);
     f(c->d_mwx); f(c->d_mwy); f(c->d_mwz); f(c->d_ta); f(c->d_tb); f(c->d_tc);
     f(c->d_twx); f(c->d_twy); f(c->d_twz); f(c->d_spec2); f(c->d_mkx); f(c->d_mky); f(c->d_part); f(c->d_scal);
     for (int s = 0; s < 5; s++) for (int p = 0; p < 6; p++) f(c->d_plane[s][p]);
@@ -379,24 +379,12 @@ template <class R> void x_backward(fen_ctx *c, R *phi, int gl, long sx, long sy,
     fen::launch_c2r<R>(c, phi, gl, sx, sy, mean, img);
 }
 
-// One time step; see the kernel sequence at the top of this file.
-template <class R> void advance(fen_ctx *c, double dt)
+// K2..K6 of a step: the Poisson solve on u*, v*, w* (`us`) up to phi with
+// its ghosts; see the kernel sequence at the top of this file.
+template <class R> void solve_phi(fen_ctx *c, double dt, void *const *us)
 {
     using namespace fen;
-    const bool d3 = c->dim == 3;
-    const double A = 1.0 + 0.5 * dt / c->dt_o;          // src/navier_stokes.f90:157-158
-    const double B = -(0.5 * dt / c->dt_o);
-    c->last_dt = dt;
-    c->step_count++;
-
-    launch_momentum<R>(c, dt, A, B);
-    if (const int rest = remaining_groups(c, c->img_vel)) {
-        GhostDesc<R> g[3] = {make_desc<R>(c, FEN_U, c->d_us[0]), make_desc<R>(c, FEN_V, c->d_us[1]),
-                             make_desc<R>(c, FEN_W, c->d_us[2])};
-        for (auto &d : g) d.groups = rest;
-        launch_ghost_desc<R>(c, g, d3 ? 3 : 2);
-    }
-    const R *vs[3] = {as<R>(c->d_us[0]), as<R>(c->d_us[1]), as<R>(c->d_us[2])};
+    const R *vs[3] = {as<R>(us[0]), as<R>(us[1]), as<R>(us[2])};
     solve_to_x_spectrum<R>(c, vs, nullptr, 0, 0, 0, c->density, dt);
     const bool mean = removes_mean(c);
     bool dirichlet = false;
@@ -415,6 +403,13 @@ template <class R> void advance(fen_ctx *c, double dt)
         g.groups = rest;
         launch_ghost_desc<R>(c, &g, 1);
     }
+}
+
+// K7..K8 of a step: projection, pressure update, checks and ghosts of u, p.
+template <class R> void finish_step(fen_ctx *c, double dt)
+{
+    using namespace fen;
+    const bool d3 = c->dim == 3;
     launch_project<R>(c, dt, c->d_scal);
     if (const int rest = remaining_groups(c, c->img_vel & c->img_p)) {
         GhostDesc<R> g[4] = {make_desc<R>(c, FEN_U, c->d_u[0]), make_desc<R>(c, FEN_V, c->d_u[1]),
@@ -426,7 +421,69 @@ template <class R> void advance(fen_ctx *c, double dt)
     c->checks_valid = true;
 }
 
-}  // namespace
+// K1 of a step: momentum into u*, v*, w* and their ghosts.
+template <class R> void predict(fen_ctx *c, double dt)
+{
+    using namespace fen;
+    const double A = 1.0 + 0.5 * dt / c->dt_o;          // src/navier_stokes.f90:157-158
+    const double B = -(0.5 * dt / c->dt_o);
+    c->last_dt = dt;
+    c->step_count++;
+    launch_momentum<R>(c, dt, A, B);
+    if (const int rest = remaining_groups(c, c->img_vel)) {
+        GhostDesc<R> g[3] = {make_desc<R>(c, FEN_U, c->d_us[0]), make_desc<R>(c, FEN_V, c->d_us[1]),
+                             make_desc<R>(c, FEN_W, c->d_us[2])};
+        for (auto &d : g) d.groups = rest;
+        launch_ghost_desc<R>(c, g, c->dim == 3 ? 3 : 2);
+    }
+}
+
+// One time step; see the kernel sequence at the top of this file.
+template <class R> void advance(fen_ctx *c, double dt)
+{
+    predict<R>(c, dt);
+    solve_phi<R>(c, dt, c->d_us);
+    finish_step<R>(c, dt);
+}
+
+// Whether fen_ns_run may fuse each step's projection with the next step's
+// momentum: 3-D, every field periodic in every direction, fixed dt.
+bool fused_run_ok(const fen_ctx *c)
+{
+    return c->fused_run && c->dim == 3 && c->solver == FEN_SOLVER_PPP && c->img_vel == 7 && c->img_p == 7 &&
+           c->img_phi == 7 && !c->constant_cfl;
+}
+
+// nsteps steps at fixed dt with the projection and checks of each step but
+// the last fused with the next step's momentum (launch_project_momentum).
+// The state after an inner step is never stored: u*, v*, w* alternate
+// between the u* arrays and the state arrays, p between p and a second
+// array, and the last step's projection writes u, v, w, p where advance()
+// leaves them.  Every step's checks are reduced as in advance(); the kernels
+// compute the same values, so the result equals nsteps calls of advance()
+// bit for bit.
+template <class R> void run_fused(fen_ctx *c, int nsteps, double dt)
+{
+    using namespace fen;
+    if (!c->d_p2) c->d_p2 = dev_alloc_bytes(c->n_ghost * elem(c));
+    const double A = 1.0 + 0.5 * dt / c->dt_o, B = -(0.5 * dt / c->dt_o);
+    void *us[2][3] = {{c->d_us[0], c->d_us[1], c->d_us[2]}, {c->d_u[0], c->d_u[1], c->d_u[2]}};
+    void *ps[2] = {c->d_p, c->d_p2};
+    predict<R>(c, dt);
+    solve_phi<R>(c, dt, us[0]);
+    // an even number of fused steps leaves u* in the u* arrays and p in p
+    const int nfused = (nsteps % 2) ? nsteps - 1 : nsteps - 2;
+    int cur = 0;
+    for (int s = 0; s < nfused; s++) {
+        c->last_dt = dt;
+        c->step_count++;
+        launch_project_momentum<R>(c, dt, dt, A, B, us[cur], ps[cur], us[1 - cur], ps[1 - cur], c->d_scal);
+        cur ^= 1;
+        solve_phi<R>(c, dt, us[cur]);
+    }
+    finish_step<R>(c, dt);
+    if (nsteps % 2 == 0) advance<R>(c, dt);
+}}  // namespace
 
 void fen::advance_fp64(fen_ctx *c, double dt) { advance<double>(c, dt); }
 
@@ -498,6 +555,7 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
     for (int s = 0; s < 6; s++) c->bc_phys[s] = (dim == 3 || s < 4) ? bc[s] : FEN_BC_PERIODIC;
     c->precision = precision;
     if (const char *g = std::getenv("FEN_STEP_GRAPH")) c->use_graph = std::atoi(g) != 0;
+    if (const char *g = std::getenv("FEN_FUSED_RUN")) c->fused_run = std::atoi(g) != 0;
     c->delta = lx / (double)(float)nx;                                   // src/grid.f90:140
     c->periodic[0] = c->bc_phys[0] == FEN_BC_PERIODIC && c->bc_phys[1] == FEN_BC_PERIODIC;
     c->periodic[1] = c->bc_phys[2] == FEN_BC_PERIODIC && c->bc_phys[3] == FEN_BC_PERIODIC;
@@ -819,6 +877,14 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
 int fen_ns_run(fen_ctx *c, int step0, int nsteps, double *dt)
 {
     if (!c || !dt || nsteps < 0) return FEN_ERR_ARG;
+    if (nsteps >= 2 && fused_run_ok(c)) {
+        if (!c->have_dt_o) return fail(c, FEN_ERR_STATE, "dt_o is undefined: call fen_set_timestep or fen_set_dt_o");
+        FEN_TRY(c, {
+            with_type(c, [&](auto tag) { run_fused<decltype(tag)>(c, nsteps, *dt); });
+            FEN_HIP_CHECK(hipGetLastError());
+        });
+        return FEN_OK;
+    }
     for (int s = 0; s < nsteps; s++) {
         const int rc = fen_ns_step(c, step0 + 1 + s, dt, nullptr, nullptr);
         if (rc != FEN_OK) return rc;

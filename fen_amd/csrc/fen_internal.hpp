@@ -135,6 +135,7 @@ enum KernelId {
     K_PROJECT,        // projection u = u* - dt grad(phi)/rho, p = p + phi
     K_CHECKS,         // max div(u), max(|u|+|v|+|w|)
     K_REDUCE,         // final small reductions
+    K_PROJ_MOM,       // projection + checks of one step fused with the next step's momentum
     K_COUNT
 };
 
@@ -181,6 +182,7 @@ struct fen_ctx {
     void *d_u[3] = {nullptr, nullptr, nullptr};    // velocity (state)
     void *d_us[3] = {nullptr, nullptr, nullptr};   // predicted velocity u*
     void *d_p = nullptr, *d_phi = nullptr;
+    void *d_p2 = nullptr;   // second pressure array of a fused run (allocated on first use)
     void *d_dvo[3] = {nullptr, nullptr, nullptr};
     void *d_S[3] = {nullptr, nullptr, nullptr};
     bool has_source = false;
@@ -219,6 +221,7 @@ struct fen_ctx {
     // kernel arguments: dt, dt_o and everything `config_version` counts
     // (parameters, bc types, the source term); a change of any rebuilds it.
     bool use_graph = true;                  // FEN_STEP_GRAPH=0 at create turns it off
+    bool fused_run = true;                  // FEN_FUSED_RUN=0 at create turns fen_ns_run's fusion off
     hipGraphExec_t step_graph = nullptr;
     double graph_dt = 0.0, graph_dt_o = 0.0;
     int graph_version = -1;
@@ -253,6 +256,9 @@ template <class R> void launch_z_solve(fen_ctx *c, double inv_n);
 template <class R> void launch_thomas(fen_ctx *c);
 template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img);
 template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out);
+template <class R>
+void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
+                             const void *p_in, void *const *us_out, void *p_out, double *checks_out);
 template <class R> void launch_checks(fen_ctx *c, const R *const *vel, double *out2);
 long stencil_blocks(const Box &b);   // workgroups of the momentum / projection grids
 // Neumann-direction solver passes (neumann_kernels.hip, FP64).

@@ -9,6 +9,7 @@
 // Producers (momentum, projection) write the periodic images of the boundary
 // cells they compute, so a periodic ghost fill costs no extra pass; the ghost
 // kernel then runs only over the axes that are not periodic.
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -221,10 +222,12 @@ template <class R> struct Shape;
 template <> struct Shape<double> {
     static constexpr int MTY = 8, MKB = 32, MUN = 1, MPF = 1, MOCC = 1;   // momentum
     static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 1;   // projection
+    static constexpr int FTY = 8, FKB = 32;                               // projection + momentum
 };
 template <> struct Shape<float> {
     static constexpr int MTY = 4, MKB = 16, MUN = 2, MPF = 2, MOCC = 1;
     static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 4;
+    static constexpr int FTY = 4, FKB = 16;
 };
 constexpr int MTX = 64;
 
@@ -241,17 +244,22 @@ static void grid_blocks(const Box &b, int ty, int kb, int *nbx, int *nby, int *n
     }
 }
 
-long stencil_blocks(const Box &b)   // workgroups of the projection (one partial each), either element type
+// Workgroups of the kernels that write one partial each (projection, fused
+// projection + momentum), either element type: the partial buffer's size.
+long stencil_blocks(const Box &b)
 {
-    int nbx, nby, nbz;
-    grid_blocks(b, Shape<double>::PTY, Shape<double>::PKB, &nbx, &nby, &nbz);
-    const long n64 = (long)nbx * nby * nbz;
-    grid_blocks(b, Shape<float>::PTY, Shape<float>::PKB, &nbx, &nby, &nbz);
-    const long n32 = (long)nbx * nby * nbz;
-    return n64 > n32 ? n64 : n32;
+    const int shapes[4][2] = {{Shape<double>::PTY, Shape<double>::PKB}, {Shape<float>::PTY, Shape<float>::PKB},
+                              {Shape<double>::FTY, Shape<double>::FKB}, {Shape<float>::FTY, Shape<float>::FKB}};
+    long most = 0;
+    for (const auto &sh : shapes) {
+        int nbx, nby, nbz;
+        grid_blocks(b, sh[0], sh[1], &nbx, &nby, &nbz);
+        most = std::max(most, (long)nbx * nby * nbz);
+    }
+    return most;
 }
 
-template <class R> void launch_momentum(fen_ctx *c, double dt, double A, double B)
+template <class R> MomArgs<R> momentum_args(fen_ctx *c, double dt, double A, double B)
 {
     MomArgs<R> a{};
     a.u = as<R>(c->d_u[0]); a.v = as<R>(c->d_u[1]); a.w = as<R>(c->d_u[2]); a.p = as<R>(c->d_p);
@@ -269,6 +277,12 @@ template <class R> void launch_momentum(fen_ctx *c, double dt, double A, double 
     a.dt = (R)dt; a.A = (R)A; a.B = (R)B;
     a.g0 = (R)c->g[0]; a.g1 = (R)c->g[1]; a.g2 = (R)c->g[2];
     a.img = c->img_vel;
+    return a;
+}
+
+template <class R> void launch_momentum(fen_ctx *c, double dt, double A, double B)
+{
+    MomArgs<R> a = momentum_args<R>(c, dt, A, B);
     int nbz;
     using SH = Shape<R>;
     grid_blocks(c->box, SH::MTY, SH::MKB, &a.nbx, &a.nby, &nbz);
@@ -462,6 +476,49 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     prof_end(c, K_REDUCE, 24.0 * nblk);
 }
 
+// Step n's projection + checks fused with step n+1's momentum (a grid
+// periodic in every direction).  us_in/p_in: u*, v*, w* and p of step n;
+// us_out/p_out: u*, v*, w* of step n+1 and p after step n (distinct arrays).
+template <class R>
+void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
+                             const void *p_in, void *const *us_out, void *p_out, double *checks_out)
+{
+    if (c->box.dim != 3 || c->img_vel != 7 || c->img_p != 7 || c->img_phi != 7 || c->shift_active)
+        throw std::runtime_error("fused projection + momentum needs a grid periodic in every direction");
+    PMArgs<R> a{};
+    a.m = momentum_args<R>(c, dt_m, A, B);
+    a.m.us = as<R>(us_out[0]); a.m.vs = as<R>(us_out[1]); a.m.ws = as<R>(us_out[2]);
+    a.us = as<R>(us_in[0]); a.vs = as<R>(us_in[1]); a.ws = as<R>(us_in[2]);
+    a.phi = as<R>(c->d_phi);
+    a.p_in = static_cast<const R *>(p_in);
+    a.p_out = as<R>(p_out);
+    a.part = c->d_part;
+    a.pid = 1.0 / c->delta;
+    a.pdt = dt_p;
+    a.prhof = 0.5 * (c->density + c->density);
+    a.prpow2 = pow2_value(a.prhof) ? 1 : 0;
+    a.pirhof = a.prpow2 ? 1.0 / a.prhof : 0.0;
+    using SH = Shape<R>;
+    int nbz;
+    grid_blocks(c->box, SH::FTY, SH::FKB, &a.m.nbx, &a.m.nby, &nbz);
+    const long nblk = (long)a.m.nbx * a.m.nby * nbz;
+    if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
+    prof_begin(c, K_PROJ_MOM);
+    const dim3 grid((unsigned)nblk);
+    if (c->has_source)
+        hipLaunchKernelGGL((project_momentum_kernel<R, 1, SH::FTY, SH::FKB>), grid, dim3(64 * SH::FTY), 0, c->stream,
+                           a);
+    else
+        hipLaunchKernelGGL((project_momentum_kernel<R, 0, SH::FTY, SH::FKB>), grid, dim3(64 * SH::FTY), 0, c->stream,
+                           a);
+    // reads u*, v*, w*, p, phi, dv_o (+ S); writes p, u*, v*, w*, dv_o
+    prof_end(c, K_PROJ_MOM, (c->has_source ? 18.0 : 15.0) * sizeof(R) * c->n_int);
+    prof_begin(c, K_REDUCE);
+    hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(kFinalThreads), 0, c->stream, (const double *)c->d_part,
+                       nblk, checks_out);
+    prof_end(c, K_REDUCE, 24.0 * nblk);
+}
+
 // ---------------------------------------------------------------------------
 // Stand-alone checks on the resident velocity (used before the first step
 // of a CFL-controlled run): same quantities as above, ghosts already filled.
@@ -533,6 +590,10 @@ template void launch_momentum<double>(fen_ctx *, double, double, double);
 template void launch_momentum<float>(fen_ctx *, double, double, double);
 template void launch_project<double>(fen_ctx *, double, double *);
 template void launch_project<float>(fen_ctx *, double, double *);
+template void launch_project_momentum<double>(fen_ctx *, double, double, double, double, void *const *, const void *,
+                                              void *const *, void *, double *);
+template void launch_project_momentum<float>(fen_ctx *, double, double, double, double, void *const *, const void *,
+                                             void *const *, void *, double *);
 template void launch_checks<double>(fen_ctx *, const double *const *, double *);
 template void launch_checks<float>(fen_ctx *, const float *const *, double *);
 

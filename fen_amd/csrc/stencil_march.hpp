@@ -312,4 +312,186 @@ __global__ __launch_bounds__(64 * TY, OCC) void project_march_kernel(ProjArgs<R>
     ProjectMarch<R, SHIFT, TY, KB, UNROLL, PF>::run(a, lds);
 }
 
+// ------------------------------------------------- projection + momentum
+// Step n's projection, pressure update and checks fused with step n+1's
+// momentum, for a grid periodic in every direction.  The state (u, v, w, p)
+// after step n is formed tile by tile from u*, v*, w*, p and phi of step n
+// with the projection's arithmetic (each value rounded once to R), placed in
+// the momentum's 4-field LDS ring, checked, and advanced to u*, v*, w* of
+// step n+1; only p of step n and the momentum's outputs are stored.  Every
+// ghost position of the state is the periodic image of an interior cell, so
+// tile cells outside the interior are formed at the wrapped interior index:
+// the same values the unfused projection writes and its images copy.
+// u*, v*, w* and p are read at the cell, phi at the cell and one cell above
+// along each axis (phi's face ghosts are its periodic images).
+template <class R> struct PMArgs {
+    MomArgs<R> m;                  // momentum of step n+1 (m.u, m.v, m.w, m.p unused)
+    const R *__restrict__ us;      // u*, v*, w* of step n
+    const R *__restrict__ vs;
+    const R *__restrict__ ws;
+    const R *__restrict__ phi;
+    const R *__restrict__ p_in;    // p before step n
+    R *__restrict__ p_out;         // p after step n (interior)
+    double *part;                  // per-workgroup (max div, max |u|+|v|+|w|, non-finite)
+    double pid, pdt, prhof, pirhof;
+    int prpow2;
+};
+
+template <class R, int SRC, int TY, int KB>
+struct ProjectMomentumMarch {
+    using S = TileShape<TY>;
+    static constexpr int NT = S::NT, NRAW = 8;
+    struct Raw {
+        R v[NRAW][S::PER];
+    };
+
+    // Tile element offsets at k = 0 with every ghost index wrapped to its
+    // periodic interior image (elements past the grid are clamped first).
+    __device__ __forceinline__ static TileIdx<S> index(const Box &b, int i0, int j0)
+    {
+        TileIdx<S> t;
+#pragma unroll
+        for (int q = 0; q < S::PER; q++) {
+            const int e = threadIdx.x + q * S::NT;
+            const int ri = e % S::LX, rj = e / S::LX;
+            int gi = min(i0 - 1 + ri, b.nx + 1), gj = min(j0 - 1 + rj, b.ny + 1);
+            if (e >= S::N) { gi = i0; gj = j0; }
+            gi = gi == 0 ? b.nx : (gi == b.nx + 1 ? 1 : gi);
+            gj = gj == 0 ? b.ny : (gj == b.ny + 1 ? 1 : gj);
+            t.off[q] = gi + (int)b.gx * gj;
+        }
+        return t;
+    }
+
+    __device__ __forceinline__ static void fetch(Raw &r, const PMArgs<R> &a, const TileIdx<S> &t, int kk)
+    {
+        const Box &b = a.m.b;
+        const int kw = kk == 0 ? b.nz : (kk == b.nz + 1 ? 1 : kk);
+        const long kofs = (long)kw * b.gxy, sj = b.gx, sk = b.gxy;
+#pragma unroll
+        for (int q = 0; q < S::PER; q++) {
+            const long o = t.off[q] + kofs;
+            r.v[0][q] = a.us[o];
+            r.v[1][q] = a.vs[o];
+            r.v[2][q] = a.ws[o];
+            r.v[3][q] = a.p_in[o];
+            r.v[4][q] = a.phi[o];
+            r.v[5][q] = a.phi[o + 1];
+            r.v[6][q] = a.phi[o + sj];
+            r.v[7][q] = a.phi[o + sk];
+        }
+    }
+
+    // The projection of one tile plane (project_cell's arithmetic) into ring slot `slot`.
+    template <class SL> __device__ __forceinline__ static void commit(R *lds, SL slot, const Raw &r, const PMArgs<R> &a)
+    {
+        auto dr = [&](double x) { return over_rho(x, a.prhof, a.pirhof, a.prpow2); };
+#pragma unroll
+        for (int q = 0; q < S::PER; q++) {
+            const int e = threadIdx.x + q * S::NT;
+            const double f0 = (double)r.v[4][q];
+            const R un = (R)((double)r.v[0][q] - dr(((double)r.v[5][q] - f0) * a.pid * a.pdt));
+            const R vn = (R)((double)r.v[1][q] - dr(((double)r.v[6][q] - f0) * a.pid * a.pdt));
+            const R wn = (R)((double)r.v[2][q] - dr(((double)r.v[7][q] - f0) * a.pid * a.pdt));
+            const R pn = (R)((double)r.v[3][q] + f0);
+            const R vals[4] = {un, vn, wn, pn};
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                int li = (f * 3 + (int)slot) * S::N + e;
+                if (q == S::PER - 1 && e >= S::N) li = 4 * 3 * S::N + (int)threadIdx.x;
+                lds[li] = vals[f];
+            }
+        }
+    }
+
+    template <class SM, class SC, class SN>
+    __device__ __forceinline__ static void plane(const PMArgs<R> &a, R *lds, Raw &r, const TileIdx<S> &t, int i,
+                                                 int j, int k, int kend, bool active, int c, R (&o)[3], double &md,
+                                                 double &mv, double &bad, bool xedge, SM sm, SC sc, SN sn)
+    {
+        const MomArgs<R> &m = a.m;
+        commit(lds, sn, r, a);   // state at plane k+1
+        __syncthreads();
+        if (k + 2 <= kend + 1) fetch(r, a, t, k + 2);
+        R on[3] = {R(0), R(0), R(0)};
+        const long q = active ? m.b.q(i, j, k) : 0;
+        const long qs = (long)m.b.nx * m.b.ny;
+        if (active && k + 1 <= kend) {
+            on[0] = m.dvo0[q + qs];
+            on[1] = m.dvo1[q + qs];
+            on[2] = m.dvo2[q + qs];
+        }
+        if (active) {
+            auto ring = [&](int f, int slotv, int off) { return lds[(f * 3 + slotv) * S::N + c + off]; };
+            // checks of step n on the state (src/navier_stokes.f90:570-619)
+            const double uc = ring(0, (int)sc, 0), um = ring(0, (int)sc, -1);
+            const double vc = ring(1, (int)sc, 0), vm = ring(1, (int)sc, -S::LX);
+            const double wc = ring(2, (int)sc, 0), wm = ring(2, (int)sm, 0);
+            double d = (uc - um) * a.pid + (vc - vm) * a.pid;
+            double vel = fabs(uc) + fabs(vc);
+            d = d + (wc - wm) * a.pid;
+            vel = vel + fabs(wc);
+            if (!(d == d) || !(vel == vel) || isinf(vel) || isinf(d)) bad = 1.0;
+            if (d > md) md = d;
+            if (vel > mv) mv = vel;
+            a.p_out[m.b.g(i, j, k)] = ring(3, (int)sc, 0);
+            // momentum of step n+1
+            auto at = [&](int f) {
+                return [=](int di, int dj, int dk) {
+                    const int slotv = dk < 0 ? (int)sm : (dk > 0 ? (int)sn : (int)sc);
+                    return lds[(f * 3 + slotv) * S::N + c + dj * S::LX + di];
+                };
+            };
+            auto img = [&](R *f, R v) { march_images(f, m.b, m.img, i, j, k, v, xedge); };
+            momentum_cell<1, SRC>(m, i, j, k, at(0), at(1), at(2), at(3), o[0], o[1], o[2], img);
+        }
+        o[0] = on[0]; o[1] = on[1]; o[2] = on[2];
+        __syncthreads();
+    }
+
+    __device__ __forceinline__ static void run(const PMArgs<R> &a, R *lds)
+    {
+        const MomArgs<R> &m = a.m;
+        const Box &b = m.b;
+        const int bid = xcd_remap(blockIdx.x, gridDim.x);
+        const int bx = bid % m.nbx, rest = bid / m.nbx;
+        const int by = rest % m.nby, bz = rest / m.nby;
+        const int tx = threadIdx.x % S::TX, ty = __builtin_amdgcn_readfirstlane(threadIdx.x / S::TX);
+        const bool xedge = bx == 0 || bx == m.nbx - 1;
+        const int i0 = bx * S::TX + 1, j0 = by * TY + 1, k0 = bz * KB + 1;
+        const int kend = min(k0 + KB - 1, b.nz);
+        const int i = i0 + tx, j = j0 + ty;
+        const bool active = i <= b.nx && j <= b.ny;
+        const int c = (ty + 1) * S::LX + tx + 1;
+        const TileIdx<S> t = index(b, i0, j0);
+        Raw r;
+        fetch(r, a, t, k0 - 1);
+        commit(lds, IC<0>{}, r, a);
+        fetch(r, a, t, k0);
+        commit(lds, IC<1>{}, r, a);
+        fetch(r, a, t, k0 + 1);
+        R o[3] = {R(0), R(0), R(0)};
+        if (active) {
+            const long q = b.q(i, j, k0);
+            o[0] = m.dvo0[q]; o[1] = m.dvo1[q]; o[2] = m.dvo2[q];
+        }
+        double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
+        int sm = 0, sc = 1, sn = 2;
+        for (int k = k0; k <= kend; k++) {
+            plane(a, lds, r, t, i, j, k, kend, active, c, o, md, mv, bad, xedge, sm, sc, sn);
+            const int tmp = sm;
+            sm = sc; sc = sn; sn = tmp;
+        }
+        block_max3<NT>(md, mv, bad, a.part, bid);
+    }
+};
+
+template <class R, int SRC, int TY, int KB>
+__global__ __launch_bounds__(64 * TY) void project_momentum_kernel(PMArgs<R> a)
+{
+    __shared__ R lds[ring_size<TileShape<TY>, 4>()];
+    ProjectMomentumMarch<R, SRC, TY, KB>::run(a, lds);
+}
+
+
 }  // namespace fen

@@ -1,0 +1,79 @@
+"""fen_ns_run on a grid periodic in every direction fuses each step's
+projection and checks with the next step's momentum.  The fused run must
+equal the same number of single steps (fen_ns_step) bit for bit: every
+field with its ghosts, dv_o, phi and the last step's checks."""
+import numpy as np
+import pytest
+
+from fen_amd import cases
+from fen_amd._lib import DVO_X, DVO_Y, DVO_Z, FP32_REFINE, P, PHI, S_X, S_Y, S_Z, U, V, W
+from tests.common import make_gpu
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = (U, V, W, P, PHI, DVO_X, DVO_Y, DVO_Z)
+
+
+def _state(g):
+    out = [g.download(f) for f in FIELDS]
+    sc = g.scalars()
+    return out + [sc["maxdiv"], sc["maxcfl"]]
+
+
+@pytest.mark.parametrize("precision", [0, FP32_REFINE])
+@pytest.mark.parametrize("nsteps", [2, 3, 6, 7])
+def test_fused_run_equals_single_steps(precision, nsteps, monkeypatch):
+    print(f"fused run precision {precision} nsteps {nsteps}", flush=True)
+    case = cases.tgv3d(32, steps=nsteps)
+    res = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FEN_FUSED_RUN", fused)
+        g = make_gpu(case, precision=precision)
+        g.run(0, 2, case.dt)          # non-zero dv_o before the compared run
+        g.run(2, nsteps, case.dt)
+        res[fused] = _state(g)
+        g.close()
+    for k, (a, b) in enumerate(zip(res["1"], res["0"])):
+        assert np.array_equal(np.asarray(a), np.asarray(b)), k
+
+
+def test_fused_run_with_source(monkeypatch):
+    """A body-force field S (the momentum variant that reads it)."""
+    case = cases.tgv3d(32, steps=5)
+    rng = np.random.default_rng(4)
+    n = case.n
+    src = {f: np.asfortranarray(0.1 * rng.standard_normal(n)) for f in (S_X, S_Y, S_Z)}
+    res = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FEN_FUSED_RUN", fused)
+        g = make_gpu(case)
+        for f, a in src.items():
+            g.upload(f, a)
+        g.run(0, 5, case.dt)
+        res[fused] = _state(g)
+        g.close()
+    for k, (a, b) in enumerate(zip(res["1"], res["0"])):
+        assert np.array_equal(np.asarray(a), np.asarray(b)), k
+
+
+def test_fused_run_partial_tiles(monkeypatch):
+    """A grid narrower than one tile in x and shorter than one tile's planes
+    in z: 16 x 8 x 16 (periodic, ppp), random state."""
+    from fen_amd import Solver
+    n, L = (16, 8, 16), (1.0, 0.5, 1.0)
+    rng = np.random.default_rng(8)
+    init = {f: np.asfortranarray(rng.standard_normal((n[0] + 2, n[1] + 2, n[2] + 2))) for f in (U, V, W, P)}
+    res = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FEN_FUSED_RUN", fused)
+        g = Solver(3, n, L)
+        g.set_params(density=1.0, viscosity=0.01, cfl=0.5)
+        for f, a in init.items():
+            g.upload(f, a)
+            g.update_ghost_nodes(f)
+        dt = g.set_timestep(4.0)
+        g.run(0, 7, dt)
+        res[fused] = _state(g)
+        g.close()
+    for k, (a, b) in enumerate(zip(res["1"], res["0"])):
+        assert np.array_equal(np.asarray(a), np.asarray(b)), k
