@@ -171,8 +171,15 @@ def measure(N, steps, warmup, precision):
     # timed region: every SAMPLE-th step, one event just before and one just
     # after the dominant kernel on the library's stream (an event drains the
     # stream, so the other steps run without them)
-    g.profile([dominant])
-    g.profile_sample(SAMPLE)
+    if steps > SAMPLE:
+        g.profile([dominant])
+        g.profile_sample(SAMPLE)
+    else:
+        # runs of at most SAMPLE steps (profiler passes): every launch of every
+        # kernel, timed from the end of the launch before it (the fused run's
+        # first step has no projection + momentum launch to sample)
+        g.profile(None)
+        g.profile_sample(1)
     g.profile_reset()
     g.synchronize()
     t0 = time.perf_counter()

@@ -222,12 +222,12 @@ template <class R> struct Shape;
 template <> struct Shape<double> {
     static constexpr int MTY = 8, MKB = 32, MUN = 1, MPF = 1, MOCC = 1;   // momentum
     static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 1;   // projection
-    static constexpr int FTY = 8, FKB = 32;                               // projection + momentum
+    static constexpr int FTY = 8, FKB = 32, FOCC = 1;                     // projection + momentum
 };
 template <> struct Shape<float> {
     static constexpr int MTY = 4, MKB = 16, MUN = 2, MPF = 2, MOCC = 1;
     static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 4;
-    static constexpr int FTY = 4, FKB = 16;
+    static constexpr int FTY = 8, FKB = 32, FOCC = 1;
 };
 constexpr int MTX = 64;
 
@@ -506,10 +506,10 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
     prof_begin(c, K_PROJ_MOM);
     const dim3 grid((unsigned)nblk);
     if (c->has_source)
-        hipLaunchKernelGGL((project_momentum_kernel<R, 1, SH::FTY, SH::FKB>), grid, dim3(64 * SH::FTY), 0, c->stream,
+        hipLaunchKernelGGL((project_momentum_kernel<R, 1, SH::FTY, SH::FKB, SH::FOCC>), grid, dim3(64 * SH::FTY), 0, c->stream,
                            a);
     else
-        hipLaunchKernelGGL((project_momentum_kernel<R, 0, SH::FTY, SH::FKB>), grid, dim3(64 * SH::FTY), 0, c->stream,
+        hipLaunchKernelGGL((project_momentum_kernel<R, 0, SH::FTY, SH::FKB, SH::FOCC>), grid, dim3(64 * SH::FTY), 0, c->stream,
                            a);
     // reads u*, v*, w*, p, phi, dv_o (+ S); writes p, u*, v*, w*, dv_o
     prof_end(c, K_PROJ_MOM, (c->has_source ? 18.0 : 15.0) * sizeof(R) * c->n_int);

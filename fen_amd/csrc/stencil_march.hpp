@@ -486,8 +486,8 @@ struct ProjectMomentumMarch {
     }
 };
 
-template <class R, int SRC, int TY, int KB>
-__global__ __launch_bounds__(64 * TY) void project_momentum_kernel(PMArgs<R> a)
+template <class R, int SRC, int TY, int KB, int OCC = 1>
+__global__ __launch_bounds__(64 * TY, OCC) void project_momentum_kernel(PMArgs<R> a)
 {
     __shared__ R lds[ring_size<TileShape<TY>, 4>()];
     ProjectMomentumMarch<R, SRC, TY, KB>::run(a, lds);

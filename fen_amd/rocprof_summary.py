@@ -24,6 +24,7 @@ import sys
 
 # demangled kernel name -> the library's profiler name (fen_profile_name)
 _PATTERNS = [
+    (r"project_momentum_kernel", "project_momentum"),
     (r"momentum_march_kernel|momentum_kernel_2d", "momentum"),
     (r"project_march_kernel|project_kernel_2d", "project"),
     (r"div_r2c_kernel", "div_r2c"),
@@ -31,7 +32,10 @@ _PATTERNS = [
     (r"y_kernel<(?:double|float), \d+, 1, 0>", "y_inv"),
     (r"y_kernel<(?:double|float), \d+, -?1, 1>", "y_solve"),
     (r"z_solve_kernel", "z_solve"),
-    (r"thomas_kernel", "thomas"),
+    (r"thomas_kernel|thomas_general_kernel", "thomas"),
+    (r"dct2_rows_kernel", "dct2"),
+    (r"dct3_rows_kernel", "dct3"),
+    (r"rfft_rows_kernel|irfft_rows_kernel", "rfft_y"),
     (r"c2r_kernel", "c2r"),
     (r"ghost_kernel", "ghost"),
     (r"checks_final_kernel|checks_kernel", "checks"),
