@@ -8,6 +8,8 @@
 //   K6 x c2r or DCT-III -> phi (+ mean, pp/ppn/pn)   ghost(phi)
 //   K7 projection + p update           ghost(u, p)
 //   K8 checks
+// fen_ns_run on a fully periodic 3-D grid fuses K7-K8 of each step with K1
+// of the next (run_fused).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
