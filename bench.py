@@ -128,11 +128,9 @@ def tgv2d_time(steps=200):
     return {"workload": "2-D Taylor-Green vortex 128^2 (pp), FP64", "ms_per_step": ms, "steps": steps}
 
 
-def channel_rate(steps=20, precision=0):
-    """Channel 512x512x256 (x, y periodic, walls at z; ppn: FFT x, y + Thomas z)."""
-    from fen_amd import cases
+def case_rate(case, steps=20, precision=0):
+    """Time per step of a case (3 warm-up steps, then `steps` steps of fen_ns_run)."""
     from tests.common import make_gpu
-    case = cases.channel(steps=steps)
     g = make_gpu(case, precision=precision)
     g.run(0, 3, case.dt)
     g.synchronize()
@@ -143,6 +141,28 @@ def channel_rate(steps=20, precision=0):
     g.close()
     cells = case.n[0] * case.n[1] * case.n[2]
     return {"ms_per_step": ms, "value": cells / (ms * 1e-3), "steps": steps}
+
+
+def channel_rate(steps=20, precision=0):
+    """Channel 512x512x256 (x, y periodic, walls at z; ppn: FFT x, y + Thomas z)."""
+    from fen_amd import cases
+    return case_rate(cases.channel(steps=steps), steps, precision)
+
+
+def neumann_rates(steps=20):
+    """FP64 time per step with the Neumann-direction solvers at 256^3 (and
+    256^2 in 2-D): lid-driven cavity (nnn), duct (npn), 2-D cavity (nn) and
+    2-D channel (pn)."""
+    from fen_amd import cases
+    out = {}
+    for name, case in (("cavity3d_256_nnn", cases.cavity3d(256, 256, 256, steps=steps)),
+                       ("duct3d_256_npn", cases.duct3d(256, 256, 256, steps=steps)),
+                       ("cavity2d_256_nn", cases.cavity2d(256, 256, steps=steps)),
+                       ("channel2d_256_pn", cases.channel2d(256, 256, steps=steps))):
+        r = case_rate(case, steps, 0)
+        r["workload"] = f"{case.name} {'x'.join(str(v) for v in case.n[:case.dim])} ({name.split('_')[-1]})"
+        out[name] = r
+    return out
 
 
 def measure(N, steps, warmup, precision):
@@ -301,6 +321,7 @@ def main():
                               "speedup_vs_fp64": r512["ms"] / r512f["ms"], "steps": 20}
         extra["channel_512x512x256"] = {"fp64": channel_rate(20, 0), "fp32_refine": channel_rate(20, FP32_REFINE)}
         extra["tgv2d_128"] = tgv2d_time()
+        extra["neumann_solvers"] = neumann_rates(20)
 
     cpu = None
     if not args.no_cpu_baseline:
