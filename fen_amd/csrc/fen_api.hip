@@ -449,11 +449,13 @@ template <class R> void advance(fen_ctx *c, double dt)
 }
 
 // Whether fen_ns_run may fuse each step's projection with the next step's
-// momentum: 3-D, every field periodic in every direction, fixed dt.
+// momentum: 3-D at fixed dt, every field periodic in every direction (ppp)
+// or x, y periodic with no-slip walls at z (ppn).
 bool fused_run_ok(const fen_ctx *c)
 {
-    return c->fused_run && c->dim == 3 && c->solver == FEN_SOLVER_PPP && c->img_vel == 7 && c->img_p == 7 &&
-           c->img_phi == 7 && !c->constant_cfl;
+    if (!c->fused_run || c->dim != 3 || c->constant_cfl) return false;
+    if (c->solver == FEN_SOLVER_PPP) return c->img_vel == 7 && c->img_p == 7 && c->img_phi == 7;
+    return c->solver == FEN_SOLVER_PPN && fen::fused_z_walls(c);
 }
 
 // nsteps steps at fixed dt with the projection and checks of each step but
@@ -481,6 +483,12 @@ template <class R> void run_fused(fen_ctx *c, int nsteps, double dt)
         c->step_count++;
         launch_project_momentum<R>(c, dt, dt, A, B, us[cur], ps[cur], us[1 - cur], ps[1 - cur], c->d_scal);
         cur ^= 1;
+        if (const int rest = remaining_groups(c, c->img_vel)) {   // z walls: ghosts of u*, v*, w*
+            GhostDesc<R> g[3] = {make_desc<R>(c, FEN_U, us[cur][0]), make_desc<R>(c, FEN_V, us[cur][1]),
+                                 make_desc<R>(c, FEN_W, us[cur][2])};
+            for (auto &d : g) d.groups = rest;
+            launch_ghost_desc<R>(c, g, 3);
+        }
         solve_phi<R>(c, dt, us[cur]);
     }
     finish_step<R>(c, dt);

@@ -256,6 +256,7 @@ template <class R> void launch_z_solve(fen_ctx *c, double inv_n);
 template <class R> void launch_thomas(fen_ctx *c);
 template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img);
 template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out);
+bool fused_z_walls(const fen_ctx *c);
 template <class R>
 void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
                              const void *p_in, void *const *us_out, void *p_out, double *checks_out);

@@ -476,16 +476,38 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     prof_end(c, K_REDUCE, 24.0 * nblk);
 }
 
+// x and y periodic for every field, z faces no-slip walls (the ghost rules
+// the fused kernel evaluates itself): u, v, w Dirichlet, p and phi Neumann.
+bool fused_z_walls(const fen_ctx *c)
+{
+    if (c->box.dim != 3 || c->img_vel != 3 || c->img_p != 3 || c->img_phi != 3) return false;
+    for (int f : {FEN_FRONT, FEN_BACK}) {
+        if (c->bc_type[FEN_U][f] != 1 || c->bc_type[FEN_V][f] != 1 || c->bc_type[FEN_W][f] != 1) return false;
+        if (c->bc_type[FEN_P][f] != 2 || c->bc_type[FEN_PHI][f] != 2) return false;
+    }
+    return true;
+}
+
 // Step n's projection + checks fused with step n+1's momentum (a grid
-// periodic in every direction).  us_in/p_in: u*, v*, w* and p of step n;
+// periodic in every direction, or x, y periodic with z walls).
+// us_in/p_in: u*, v*, w* and p of step n;
 // us_out/p_out: u*, v*, w* of step n+1 and p after step n (distinct arrays).
 template <class R>
 void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
                              const void *p_in, void *const *us_out, void *p_out, double *checks_out)
 {
-    if (c->box.dim != 3 || c->img_vel != 7 || c->img_p != 7 || c->img_phi != 7 || c->shift_active)
-        throw std::runtime_error("fused projection + momentum needs a grid periodic in every direction");
+    const bool periodic = c->img_vel == 7 && c->img_p == 7 && c->img_phi == 7 && !c->shift_active;
+    const bool zwalls = fused_z_walls(c);
+    if (c->box.dim != 3 || !(periodic || zwalls))
+        throw std::runtime_error("fused projection + momentum needs a periodic grid or x, y periodic with z walls");
     PMArgs<R> a{};
+    if (zwalls) {
+        a.shift = c->shift_active ? c->d_shift : nullptr;
+        for (int d = 0; d < 3; d++) {
+            a.zlo[d] = as<R>(c->d_plane[FEN_U + d][FEN_FRONT]);
+            a.zhi[d] = as<R>(c->d_plane[FEN_U + d][FEN_BACK]);
+        }
+    }
     a.m = momentum_args<R>(c, dt_m, A, B);
     a.m.us = as<R>(us_out[0]); a.m.vs = as<R>(us_out[1]); a.m.ws = as<R>(us_out[2]);
     a.us = as<R>(us_in[0]); a.vs = as<R>(us_in[1]); a.ws = as<R>(us_in[2]);
@@ -504,13 +526,17 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
     const long nblk = (long)a.m.nbx * a.m.nby * nbz;
     if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
     prof_begin(c, K_PROJ_MOM);
-    const dim3 grid((unsigned)nblk);
-    if (c->has_source)
-        hipLaunchKernelGGL((project_momentum_kernel<R, 1, SH::FTY, SH::FKB, SH::FOCC>), grid, dim3(64 * SH::FTY), 0, c->stream,
-                           a);
-    else
-        hipLaunchKernelGGL((project_momentum_kernel<R, 0, SH::FTY, SH::FKB, SH::FOCC>), grid, dim3(64 * SH::FTY), 0, c->stream,
-                           a);
+    const dim3 grid((unsigned)nblk), block(64 * SH::FTY);
+#define FEN_PM(SRC, ZW) \
+    hipLaunchKernelGGL((project_momentum_kernel<R, SRC, ZW, SH::FTY, SH::FKB, SH::FOCC>), grid, block, 0, c->stream, a)
+    if (c->has_source) {
+        if (zwalls) FEN_PM(1, 1);
+        else FEN_PM(1, 0);
+    } else {
+        if (zwalls) FEN_PM(0, 1);
+        else FEN_PM(0, 0);
+    }
+#undef FEN_PM
     // reads u*, v*, w*, p, phi, dv_o (+ S); writes p, u*, v*, w*, dv_o
     prof_end(c, K_PROJ_MOM, (c->has_source ? 18.0 : 15.0) * sizeof(R) * c->n_int);
     prof_begin(c, K_REDUCE);

@@ -324,6 +324,12 @@ __global__ __launch_bounds__(64 * TY, OCC) void project_march_kernel(ProjArgs<R>
 // the same values the unfused projection writes and its images copy.
 // u*, v*, w* and p are read at the cell, phi at the cell and one cell above
 // along each axis (phi's face ghosts are its periodic images).
+// ZW: the z faces are no-slip walls instead (channel: x, y periodic).  The
+// state's ghost planes k = 0 and nz+1 then follow the ghost fill's z rules
+// applied to the projected first and last planes: u, v = 2 bc - f (the bc
+// planes are indexed at the cell's own, unwrapped (i, j)), w = bc (and the
+// last interior w is overwritten with bc), p = the neighbouring interior
+// value; a pending mean shift of phi is subtracted as the projection does.
 template <class R> struct PMArgs {
     MomArgs<R> m;                  // momentum of step n+1 (m.u, m.v, m.w, m.p unused)
     const R *__restrict__ us;      // u*, v*, w* of step n
@@ -333,17 +339,38 @@ template <class R> struct PMArgs {
     const R *__restrict__ p_in;    // p before step n
     R *__restrict__ p_out;         // p after step n (interior)
     double *part;                  // per-workgroup (max div, max |u|+|v|+|w|, non-finite)
+    const double *shift;           // pending mean of phi (ZW), or null
+    const R *zlo[3], *zhi[3];      // ZW: bc planes of u, v, w at the front and back faces
     double pid, pdt, prhof, pirhof;
     int prpow2;
 };
 
-template <class R, int SRC, int TY, int KB>
+template <class R, int SRC, int TY, int KB, int ZW>
 struct ProjectMomentumMarch {
     using S = TileShape<TY>;
     static constexpr int NT = S::NT, NRAW = 8;
     struct Raw {
         R v[NRAW][S::PER];
     };
+    struct PlaneIdx {   // ZW: bc plane index i + (nx+2) j of each tile element
+        int v[S::PER];
+    };
+
+    __device__ __forceinline__ static PlaneIdx plane_index(const Box &b, int i0, int j0)
+    {
+        PlaneIdx pi{};
+        if constexpr (ZW) {
+#pragma unroll
+            for (int q = 0; q < S::PER; q++) {
+                const int e = threadIdx.x + q * S::NT;
+                const int ri = e % S::LX, rj = e / S::LX;
+                int gi = min(i0 - 1 + ri, b.nx + 1), gj = min(j0 - 1 + rj, b.ny + 1);
+                if (e >= S::N) { gi = i0; gj = j0; }
+                pi.v[q] = gi + (b.nx + 2) * gj;
+            }
+        }
+        return pi;
+    }
 
     // Tile element offsets at k = 0 with every ghost index wrapped to its
     // periodic interior image (elements past the grid are clamped first).
@@ -366,7 +393,9 @@ struct ProjectMomentumMarch {
     __device__ __forceinline__ static void fetch(Raw &r, const PMArgs<R> &a, const TileIdx<S> &t, int kk)
     {
         const Box &b = a.m.b;
-        const int kw = kk == 0 ? b.nz : (kk == b.nz + 1 ? 1 : kk);
+        // the interior plane the state at plane kk is formed from
+        const int kw = ZW ? (kk == 0 ? 1 : (kk == b.nz + 1 ? b.nz : kk))
+                          : (kk == 0 ? b.nz : (kk == b.nz + 1 ? 1 : kk));
         const long kofs = (long)kw * b.gxy, sj = b.gx, sk = b.gxy;
 #pragma unroll
         for (int q = 0; q < S::PER; q++) {
@@ -382,18 +411,39 @@ struct ProjectMomentumMarch {
         }
     }
 
-    // The projection of one tile plane (project_cell's arithmetic) into ring slot `slot`.
-    template <class SL> __device__ __forceinline__ static void commit(R *lds, SL slot, const Raw &r, const PMArgs<R> &a)
+    // The state at tile plane kk (project_cell's arithmetic, then the z wall
+    // rules when ZW) into ring slot `slot`.
+    template <class SL>
+    __device__ __forceinline__ static void commit(R *lds, SL slot, const Raw &r, const PMArgs<R> &a,
+                                                  const PlaneIdx &pi, int kk, double sh)
     {
         auto dr = [&](double x) { return over_rho(x, a.prhof, a.pirhof, a.prpow2); };
+        const int nz = a.m.b.nz;
 #pragma unroll
         for (int q = 0; q < S::PER; q++) {
             const int e = threadIdx.x + q * S::NT;
-            const double f0 = (double)r.v[4][q];
-            const R un = (R)((double)r.v[0][q] - dr(((double)r.v[5][q] - f0) * a.pid * a.pdt));
-            const R vn = (R)((double)r.v[1][q] - dr(((double)r.v[6][q] - f0) * a.pid * a.pdt));
-            const R wn = (R)((double)r.v[2][q] - dr(((double)r.v[7][q] - f0) * a.pid * a.pdt));
+            double f0 = (double)r.v[4][q], fx = (double)r.v[5][q], fy = (double)r.v[6][q], fz = (double)r.v[7][q];
+            if constexpr (ZW) {
+                f0 = f0 - sh; fx = fx - sh; fy = fy - sh; fz = fz - sh;
+            }
+            R un = (R)((double)r.v[0][q] - dr((fx - f0) * a.pid * a.pdt));
+            R vn = (R)((double)r.v[1][q] - dr((fy - f0) * a.pid * a.pdt));
+            R wn = (R)((double)r.v[2][q] - dr((fz - f0) * a.pid * a.pdt));
             const R pn = (R)((double)r.v[3][q] + f0);
+            if constexpr (ZW) {
+                const R two = R(2.0);
+                if (kk == 0) {
+                    un = two * a.zlo[0][pi.v[q]] - un;
+                    vn = two * a.zlo[1][pi.v[q]] - vn;
+                    wn = a.zlo[2][pi.v[q]];
+                } else if (kk >= nz) {
+                    if (kk > nz) {
+                        un = two * a.zhi[0][pi.v[q]] - un;
+                        vn = two * a.zhi[1][pi.v[q]] - vn;
+                    }
+                    wn = a.zhi[2][pi.v[q]];
+                }
+            }
             const R vals[4] = {un, vn, wn, pn};
 #pragma unroll
             for (int f = 0; f < 4; f++) {
@@ -405,12 +455,13 @@ struct ProjectMomentumMarch {
     }
 
     template <class SM, class SC, class SN>
-    __device__ __forceinline__ static void plane(const PMArgs<R> &a, R *lds, Raw &r, const TileIdx<S> &t, int i,
-                                                 int j, int k, int kend, bool active, int c, R (&o)[3], double &md,
-                                                 double &mv, double &bad, bool xedge, SM sm, SC sc, SN sn)
+    __device__ __forceinline__ static void plane(const PMArgs<R> &a, R *lds, Raw &r, const TileIdx<S> &t,
+                                                 const PlaneIdx &pi, double sh, int i, int j, int k, int kend,
+                                                 bool active, int c, R (&o)[3], double &md, double &mv, double &bad,
+                                                 bool xedge, SM sm, SC sc, SN sn)
     {
         const MomArgs<R> &m = a.m;
-        commit(lds, sn, r, a);   // state at plane k+1
+        commit(lds, sn, r, a, pi, k + 1, sh);   // state at plane k+1
         __syncthreads();
         if (k + 2 <= kend + 1) fetch(r, a, t, k + 2);
         R on[3] = {R(0), R(0), R(0)};
@@ -464,11 +515,13 @@ struct ProjectMomentumMarch {
         const bool active = i <= b.nx && j <= b.ny;
         const int c = (ty + 1) * S::LX + tx + 1;
         const TileIdx<S> t = index(b, i0, j0);
+        const PlaneIdx pi = plane_index(b, i0, j0);
+        const double sh = (ZW && a.shift) ? a.shift[0] : 0.0;
         Raw r;
         fetch(r, a, t, k0 - 1);
-        commit(lds, IC<0>{}, r, a);
+        commit(lds, IC<0>{}, r, a, pi, k0 - 1, sh);
         fetch(r, a, t, k0);
-        commit(lds, IC<1>{}, r, a);
+        commit(lds, IC<1>{}, r, a, pi, k0, sh);
         fetch(r, a, t, k0 + 1);
         R o[3] = {R(0), R(0), R(0)};
         if (active) {
@@ -478,7 +531,7 @@ struct ProjectMomentumMarch {
         double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
         int sm = 0, sc = 1, sn = 2;
         for (int k = k0; k <= kend; k++) {
-            plane(a, lds, r, t, i, j, k, kend, active, c, o, md, mv, bad, xedge, sm, sc, sn);
+            plane(a, lds, r, t, pi, sh, i, j, k, kend, active, c, o, md, mv, bad, xedge, sm, sc, sn);
             const int tmp = sm;
             sm = sc; sc = sn; sn = tmp;
         }
@@ -486,11 +539,11 @@ struct ProjectMomentumMarch {
     }
 };
 
-template <class R, int SRC, int TY, int KB, int OCC = 1>
+template <class R, int SRC, int ZW, int TY, int KB, int OCC = 1>
 __global__ __launch_bounds__(64 * TY, OCC) void project_momentum_kernel(PMArgs<R> a)
 {
     __shared__ R lds[ring_size<TileShape<TY>, 4>()];
-    ProjectMomentumMarch<R, SRC, TY, KB>::run(a, lds);
+    ProjectMomentumMarch<R, SRC, TY, KB, ZW>::run(a, lds);
 }
 
 

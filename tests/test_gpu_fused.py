@@ -77,3 +77,32 @@ def test_fused_run_partial_tiles(monkeypatch):
         g.close()
     for k, (a, b) in enumerate(zip(res["1"], res["0"])):
         assert np.array_equal(np.asarray(a), np.asarray(b)), k
+
+
+@pytest.mark.parametrize("precision", [0, FP32_REFINE])
+@pytest.mark.parametrize("nsteps", [2, 5, 6])
+def test_fused_channel_equals_single_steps(precision, nsteps, monkeypatch):
+    """x, y periodic, no-slip walls at z (ppn) with moving-wall bc planes on
+    u, v and a transpiration plane on w: the fused kernel's wall ghost rules
+    and the pending mean shift of phi against single steps."""
+    print(f"fused channel precision {precision} nsteps {nsteps}", flush=True)
+    from fen_amd._lib import BC_WALL
+    case = cases.channel(32, 16, 40, steps=nsteps)
+    assert case.bc[4] == BC_WALL and case.bc[5] == BC_WALL
+    rng = np.random.default_rng(12)
+    planes = {(f, face): 0.05 * rng.standard_normal((34, 18)) for f in (U, V, W) for face in (4, 5)}
+    res = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FEN_FUSED_RUN", fused)
+        g = make_gpu(case, precision=precision)
+        assert g.solver == "ppn"
+        for (f, face), pl in planes.items():
+            g.set_bc_plane(f, face, pl)
+        for f in (U, V, W, P):
+            g.update_ghost_nodes(f)
+        g.run(0, 2, case.dt)
+        g.run(2, nsteps, case.dt)
+        res[fused] = _state(g)
+        g.close()
+    for k, (a, b) in enumerate(zip(res["1"], res["0"])):
+        assert np.array_equal(np.asarray(a), np.asarray(b)), k
