@@ -1,0 +1,81 @@
+"""The benchmark configurations at their full sizes on the GPU.
+
+* 3-D Taylor-Green 256^3, 200 steps (FP64, fused run) against the oracle
+  run on the host's cores: 1e-12 relative in u, v, w, p, max|div| at
+  round-off.
+* Channel 512x512x256, 100 steps (FP64): size-independent properties --
+  max|div| at round-off (|u| ~ 24, delta = 1/128) every 20 steps, and the mean of w at round-off (a
+  divergence-free field has the same w flux through every z plane, and the
+  walls carry none); the bulk velocity stays finite and near its start.
+* 3-D Taylor-Green 512^3, 200 steps: the FP32+refine form against the FP64
+  run, relative L2 velocity error <= 1e-6.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from fen_amd import cases
+from fen_amd._lib import FP32_REFINE, P, U, V, W
+from tests.common import make_gpu, make_oracle, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tgv3d_256_200_steps_matches_oracle():
+    case = cases.tgv3d(256, steps=200)
+    t0 = time.perf_counter()
+    o = make_oracle(case)
+    dt = case.dt
+    for s in range(1, case.steps + 1):
+        _, mdo, _ = o.step(s, dt)
+    t1 = time.perf_counter()
+    g = make_gpu(case)
+    g.run(0, case.steps, dt)
+    md = g.scalars()["maxdiv"]
+    errs = {f: rel_err(g.download(f), o.field(f)) for f in (U, V, W, P)}
+    print(f"tgv3d 256^3 200 steps: oracle {t1 - t0:.1f} s; rel err {errs}; maxdiv gpu {md:.2e} oracle {mdo:.2e}")
+    assert abs(md) < 1e-12 and abs(mdo) < 1e-12
+    for f, e in errs.items():
+        assert e < 1e-12, (f, e)
+    g.close()
+    o.close()
+
+
+def test_channel_512_full_size_properties():
+    case = cases.channel(steps=100)
+    g = make_gpu(case)
+    dt = case.dt
+    ub0 = float(g.download(U)[1:-1, 1:-1, 1:-1].mean())
+    step = 0
+    for chunk in range(5):
+        g.run(step, 20, dt)
+        step += 20
+        md = g.scalars()["maxdiv"]
+        assert abs(md) < 1e-9, (step, md)   # |u| ~ 24, delta = 1/128: round-off ~ 1e-12
+    u = g.download(U)[1:-1, 1:-1, 1:-1]
+    w = g.download(W)[1:-1, 1:-1, 1:-1]
+    scale = float(np.abs(u).max())
+    assert abs(float(w.mean())) < 1e-12 * scale
+    ub = float(u.mean())
+    print(f"channel 512x512x256 100 steps: maxdiv {md:.2e}, bulk u {ub0:.6f} -> {ub:.6f}, "
+          f"mean w {float(w.mean()):.2e}")
+    assert np.isfinite(ub) and abs(ub - ub0) < 0.05 * abs(ub0)
+    g.close()
+
+
+def test_tgv3d_512_fp32_refine_error():
+    case = cases.tgv3d(512, steps=200)
+    res = {}
+    for prec in (0, FP32_REFINE):
+        g = make_gpu(case, precision=prec)
+        g.run(0, case.steps, case.dt)
+        res[prec] = ({f: g.download(f)[1:-1, 1:-1, 1:-1] for f in (U, V, W)}, g.scalars()["maxdiv"])
+        g.close()
+    num = sum(float(np.linalg.norm((res[FP32_REFINE][0][f] - res[0][0][f]).ravel())) ** 2 for f in (U, V, W))
+    den = sum(float(np.linalg.norm(res[0][0][f].ravel())) ** 2 for f in (U, V, W))
+    err = (num / den) ** 0.5
+    print(f"tgv3d 512^3 200 steps FP32+refine vs FP64: velocity rel L2 {err:.3e}, "
+          f"maxdiv fp64 {res[0][1]:.2e} fp32 {res[FP32_REFINE][1]:.2e}")
+    assert err <= 1e-6
+    assert abs(res[0][1]) < 1e-12
