@@ -17,7 +17,7 @@ FEN_OK = 0
 ERRORS = {-1: "FEN_ERR_ARG", -2: "FEN_ERR_UNSUPPORTED", -3: "FEN_ERR_HIP", -4: "FEN_ERR_NONFINITE",
           -5: "FEN_ERR_STATE"}
 
-P, PHI, U, V, W, DVO_X, DVO_Y, DVO_Z, S_X, S_Y, S_Z = range(11)
+P, PHI, U, V, W, DVO_X, DVO_Y, DVO_Z, S_X, S_Y, S_Z, VOF, RHO, MU = range(14)
 BC_PERIODIC, BC_WALL, BC_INFLOW, BC_OUTFLOW = range(4)
 LEFT, RIGHT, BOTTOM, TOP, FRONT, BACK = range(6)
 FP64, FP32_REFINE = 0, 1

@@ -61,6 +61,10 @@ struct orc_ctx {
     offt fx, fy, fz;
     double *sre, *sim;  /* half spectrum (nx/2+1)*ny*nz, kx fastest (real spectra use sre) */
     double *cosx, *cosy; /* cos(pi m / (2n)), m < 4n, for the DCTs along x and y */
+    /* two-phase (MF) state: multiphase_mod and volume_of_fluid_mod */
+    int mf, x_first;
+    double rho0, rho1, mu0, mu1, sigma, beta, rhomin, irhomin;
+    ofield vof, h, dd, curv, nrx, nry, lx2, ly2, rho, mu, p_hat, p_o;
 };
 
 #define GX(c) ((size_t)(c)->nx + 2)
@@ -824,6 +828,11 @@ orc_ctx *orc_create(int dim, int nx, int ny, int nz, double lx, double ly, doubl
 void orc_destroy(orc_ctx *c)
 {
     if (!c) return;
+    if (c->mf) {
+        ofield *mf[12] = {&c->vof, &c->h, &c->dd, &c->curv, &c->nrx, &c->nry, &c->lx2, &c->ly2,
+                          &c->rho, &c->mu, &c->p_hat, &c->p_o};
+        for (int q = 0; q < 12; q++) field_free(mf[q]);
+    }
     field_free(&c->p); field_free(&c->phi); field_free(&c->u); field_free(&c->v); field_free(&c->w);
     for (int d = 0; d < 3; d++) { free(c->dv[d]); free(c->dvo[d]); free(c->S[d]); free(c->gp[d]); }
     fft_free(&c->fx); fft_free(&c->fy); fft_free(&c->fz);
@@ -866,6 +875,9 @@ static ofield *ghosted(orc_ctx *c, int field)
     case ORC_U: return &c->u;
     case ORC_V: return &c->v;
     case ORC_W: return &c->w;
+    case ORC_VOF: return c->mf ? &c->vof : NULL;
+    case ORC_RHO: return c->mf ? &c->rho : NULL;
+    case ORC_MU: return c->mf ? &c->mu : NULL;
     }
     return NULL;
 }
@@ -881,7 +893,7 @@ double *orc_field(orc_ctx *c, int field)
 
 size_t orc_field_len(const orc_ctx *c, int field)
 {
-    return field <= ORC_W ? ghosted_len(c) : interior_len(c);
+    return (field <= ORC_W || field >= ORC_VOF) ? ghosted_len(c) : interior_len(c);
 }
 
 double *orc_bc_plane(orc_ctx *c, int field, int face)
@@ -1079,9 +1091,15 @@ static void update_timestep(orc_ctx *c, double *dt)
     if (*dt > 1.1 * c->dt_o) *dt = 1.1 * c->dt_o;
 }
 
+static void mf_step(orc_ctx *c, double dt, double *maxdiv, double *maxcfl);
+
 void orc_ns_step(orc_ctx *c, int step, double *dt_io, double *maxdiv, double *maxcfl)
 {
     (void)step;
+    if (c->mf) {   /* the MF build's step, fixed dt */
+        mf_step(c, *dt_io, maxdiv, maxcfl);
+        return;
+    }
     const int nx = c->nx, ny = c->ny, nz = c->nz, d3 = c->dim == 3;
     if (c->constant_cfl) update_timestep(c, dt_io);                            /* :78 */
     const double dt = *dt_io;
@@ -1146,6 +1164,374 @@ void orc_ns_step(orc_ctx *c, int step, double *dt_io, double *maxdiv, double *ma
     c->maxcfl = dt * max_velocity_sum(c) / c->delta;
     if (maxdiv) *maxdiv = c->maxdiv;
     if (maxcfl) *maxcfl = c->maxcfl;
+}
+
+/* -------------------------------------------------------------------------- */
+/* Two-phase solver (MF build), 2-D                                            */
+/* -------------------------------------------------------------------------- */
+
+static const double kSmall = 1.0e-14;   /* global_mod small (src/global.f90:16) */
+static const double kCut = 1.0e-8;      /* cut (src/volume_of_fluid.f90:37) */
+
+/* Two-point Gauss nodes on [0,1] (src/volume_of_fluid.f90:33-34). */
+static double gauss_rp(void) { return 0.5 * (1.0 + 1.0 / sqrt(3.0)); }
+static double gauss_rm(void) { return 0.5 * (1.0 - 1.0 / sqrt(3.0)); }
+
+/* The quadratic surface of one cell (module variables cx, cy, a10, a01, a20, a02). */
+typedef struct { double cx, cy, a10, a01, a20, a02; } osurf;
+
+/* P(x, y), Eq. 9 of Ii et al. 2012 (src/volume_of_fluid.f90:387-398). */
+static double surf_P(const osurf *q, double x, double y)
+{
+    return ((q->cx * q->a20 * (x * x) + q->cy * q->a02 * (y * y)) + q->a10 * x) + q->a01 * y;
+}
+
+/* The surface coefficients of cell g (src/volume_of_fluid.f90:260-273, 586-609). */
+static osurf cell_surface(const orc_ctx *c, size_t g)
+{
+    osurf q;
+    const double nx = c->nrx.f[g], ny = c->nry.f[g];
+    if (fabs(nx) == fmax(fabs(nx), fabs(ny))) { q.cx = 0.0; q.cy = 1.0; }
+    else { q.cx = 1.0; q.cy = 0.0; }
+    q.a10 = nx - 0.5 * q.cx * c->lx2.f[g];
+    q.a01 = ny - 0.5 * q.cy * c->ly2.f[g];
+    q.a20 = 0.5 * q.cx * c->lx2.f[g];
+    q.a02 = 0.5 * q.cy * c->ly2.f[g];
+    return q;
+}
+
+static double solve_quadratic(double a, double b, double cc)   /* :402-416 */
+{
+    const double x1 = (-b + sqrt(b * b - 4.0 * a * cc)) / (2.0 * a);
+    const double x2 = (-b - sqrt(b * b - 4.0 * a * cc)) / (2.0 * a);
+    return fmax(x1, x2);
+}
+
+/* compute_norm, src/volume_of_fluid.f90:299-383 (Young's method). */
+static void compute_norm(orc_ctx *c)
+{
+    const double delta = c->delta, id = 1.0 / delta, id2 = 1.0 / (delta * delta);
+    const ptrdiff_t sj = (ptrdiff_t)GX(c);
+    const double *f = c->vof.f;
+    for (int j = 1; j <= c->ny; j++)
+        for (int i = 1; i <= c->nx; i++) {
+            const size_t g = GI(c, i, j, 1);
+            const double *F = f + g;
+#define VF(di, dj) F[(di) + (dj) * sj]
+            double mx[4], my[4], nxs[4], nys[4];
+            mx[0] = 0.5 * (VF(0, -1) + VF(0, 0) - VF(-1, -1) - VF(-1, 0)) * id;
+            mx[1] = 0.5 * (VF(0, 0) + VF(0, 1) - VF(-1, 0) - VF(-1, 1)) * id;
+            mx[2] = 0.5 * (VF(1, 0) + VF(1, 1) - VF(0, 0) - VF(0, 1)) * id;
+            mx[3] = 0.5 * (VF(1, -1) + VF(1, 0) - VF(0, -1) - VF(0, 0)) * id;
+            const double mxc = 0.25 * (mx[0] + mx[1] + mx[2] + mx[3]);
+            my[0] = 0.5 * (VF(-1, 0) + VF(0, 0) - VF(-1, -1) - VF(0, -1)) * id;
+            my[1] = 0.5 * (VF(-1, 1) + VF(0, 1) - VF(-1, 0) - VF(0, 0)) * id;
+            my[2] = 0.5 * (VF(0, 1) + VF(1, 1) - VF(0, 0) - VF(1, 0)) * id;
+            my[3] = 0.5 * (VF(0, 0) + VF(1, 0) - VF(0, -1) - VF(1, -1)) * id;
+            const double myc = 0.25 * (my[0] + my[1] + my[2] + my[3]);
+#undef VF
+            for (int q = 0; q < 4; q++) {
+                nxs[q] = mx[q] / sqrt(mx[q] * mx[q] + my[q] * my[q] + kSmall);
+                nys[q] = my[q] / sqrt(mx[q] * mx[q] + my[q] * my[q] + kSmall);
+            }
+            c->nrx.f[g] = mxc / sqrt(mxc * mxc + myc * myc + kSmall);
+            c->nry.f[g] = myc / sqrt(mxc * mxc + myc * myc + kSmall);
+            c->lx2.f[g] = 0.5 * delta * (nxs[3] + nxs[2] - nxs[1] - nxs[0]);   /* quadratic = .true. */
+            c->ly2.f[g] = 0.5 * delta * (nys[1] + nys[2] - nys[0] - nys[3]);
+            c->curv.f[g] = -(c->lx2.f[g] + c->ly2.f[g]) * id2;
+        }
+    ghost_fill(c, &c->curv);
+    ghost_fill(c, &c->nrx); ghost_fill(c, &c->nry);
+    ghost_fill(c, &c->lx2); ghost_fill(c, &c->ly2);
+}
+
+/* get_h_from_vof, src/volume_of_fluid.f90:223-295. */
+static void get_h_from_vof(orc_ctx *c)
+{
+    compute_norm(c);
+    const double beta = c->beta, rp = gauss_rp(), rm = gauss_rm();
+    for (int j = 1; j <= c->ny; j++)
+        for (int i = 1; i <= c->nx; i++) {
+            const size_t g = GI(c, i, j, 1);
+            const double f = c->vof.f[g];
+            if (f <= kCut || f >= (1.0 - kCut)) {
+                c->h.f[g] = f;
+                c->dd.f[g] = 0.0;
+                continue;
+            }
+            const osurf q = cell_surface(c, g);
+            const double A = (1.0 - q.cx) * exp(2.0 * beta * q.a10) + (1.0 - q.cy) * exp(2.0 * beta * q.a01);
+            const double Bp = (1.0 - q.cx) * exp(2.0 * beta * surf_P(&q, 0.0, rp)) +
+                              (1.0 - q.cy) * exp(2.0 * beta * surf_P(&q, rp, 0.0));
+            const double Bm = (1.0 - q.cx) * exp(2.0 * beta * surf_P(&q, 0.0, rm)) +
+                              (1.0 - q.cy) * exp(2.0 * beta * surf_P(&q, rm, 0.0));
+            const double Q = (1.0 - q.cx) * exp(2.0 * beta * q.a10 * (2.0 * f - 1.0)) +
+                             (1.0 - q.cy) * exp(2.0 * beta * q.a01 * (2.0 * f - 1.0));
+            const double aa = A * Bm * Bp * (A - Q);
+            const double bb = A * (Bp + Bm) * (1.0 - Q);
+            const double cc = 1.0 - A * Q;
+            c->dd.f[g] = log(solve_quadratic(aa, bb, cc)) / (2.0 * beta);
+            c->h.f[g] = 0.5 * (1.0 + tanh(beta * (surf_P(&q, 0.5, 0.5) + c->dd.f[g])));
+        }
+    ghost_fill(c, &c->h);
+    ghost_fill(c, &c->dd);
+}
+
+/* An_Int and Num_Int, src/volume_of_fluid.f90:636-661. */
+static double an_int(const osurf *q, double beta, double a, double b, double xa, double xb, double ya, double yb,
+                     double cf, double d0)
+{
+    return 0.5 * (b - a + 1.0 / (cf * beta) * log(cosh(beta * (surf_P(q, xb, yb) + d0)) /
+                                                   cosh(beta * (surf_P(q, xa, ya) + d0))));
+}
+
+static double num_int(double a, double b, double qrm, double qrp) { return 0.5 * (qrm + qrp) * (b - a); }
+
+/* compute_flux, src/volume_of_fluid.f90:548-632: flux through the high face of
+ * cell (i, j) along dir (1 x, 2 y) with face velocity u. */
+static double compute_flux(const orc_ctx *c, int dir, int i, int j, double u, double dt)
+{
+    const double delta = c->delta, beta = c->beta, rp = gauss_rp(), rm = gauss_rm();
+    double xa, xb, ya, yb;
+    int sgn, ii = i, jj = j;
+    if (dir == 1) {
+        if (u >= 0.0) { xa = 1.0 - dt * u / delta; xb = 1.0; sgn = 1; }
+        else { xa = 0.0; xb = -dt * u / delta; sgn = -1; ii = i + 1; }
+        ya = 0.0; yb = 1.0;
+    } else {
+        xa = 0.0; xb = 1.0;
+        if (u >= 0.0) { ya = 1.0 - dt * u / delta; yb = 1.0; sgn = 1; }
+        else { ya = 0.0; yb = -dt * u / delta; sgn = -1; jj = j + 1; }
+    }
+    const size_t g = GI(c, ii, jj, 1);
+    const double f = c->vof.f[g];
+    if (f <= kCut || f >= (1.0 - kCut)) return (double)sgn * delta * f * (xb - xa) * (yb - ya);
+    const osurf q = cell_surface(c, g);
+    const double d0 = c->dd.f[g];
+    if (q.cx == 0.0)   /* |norm x| dominant: analytical in x, Gauss in y */
+        return (double)sgn * delta *
+               num_int(ya, yb, an_int(&q, beta, xa, xb, xa, xb, rm * (ya + yb), rm * (yb + ya), q.a10, d0),
+                       an_int(&q, beta, xa, xb, xa, xb, rp * (ya + yb), rp * (yb + ya), q.a10, d0));
+    return (double)sgn * delta *
+           num_int(xa, xb, an_int(&q, beta, ya, yb, rm * (xa + xb), rm * (xa + xb), ya, yb, q.a01, d0),
+                   an_int(&q, beta, ya, yb, rp * (xa + xb), rp * (xa + xb), ya, yb, q.a01, d0));
+}
+
+/* One directional sweep: out = (in - (fp - fm)/delta) / (1 - dt (u_h - u_l)/delta),
+ * fluxes from the current reconstruction of c->vof. */
+static void vof_sweep(orc_ctx *c, int dir, const double *in, double *out, double dt)
+{
+    const double delta = c->delta;
+    const double *vel = dir == 1 ? c->u.f : c->v.f;
+    const ptrdiff_t sh = dir == 1 ? 1 : (ptrdiff_t)GX(c);
+    for (int j = 1; j <= c->ny; j++)
+        for (int i = 1; i <= c->nx; i++) {
+            const size_t g = GI(c, i, j, 1);
+            const double fp = compute_flux(c, dir, i, j, vel[g], dt);
+            const double fm = dir == 1 ? compute_flux(c, 1, i - 1, j, vel[g - sh], dt)
+                                       : compute_flux(c, 2, i, j - 1, vel[g - sh], dt);
+            out[g] = (in[g] - (fp - fm) / delta) / (1.0 - dt * (vel[g] - vel[g - sh]) / delta);
+        }
+}
+
+/* advect_vof, src/volume_of_fluid.f90:420-544 (direction order alternates). */
+void orc_advect_vof(orc_ctx *c, double dt)
+{
+    if (!c->mf) return;
+    const size_t ng = ghosted_len(c);
+    const double delta = c->delta;
+    double *vof1 = (double *)calloc(ng, sizeof(double)), *vof2 = (double *)calloc(ng, sizeof(double));
+    get_h_from_vof(c);
+    const int first = c->x_first ? 1 : 2, second = c->x_first ? 2 : 1;
+    vof_sweep(c, first, c->vof.f, vof1, dt);
+    memcpy(c->vof.f, vof1, ng * sizeof(double));   /* vof = vof1 (ghosts of vof1 are zero) */
+    ghost_fill(c, &c->vof);
+    get_h_from_vof(c);
+    vof_sweep(c, second, vof1, vof2, dt);
+    const ptrdiff_t sj = (ptrdiff_t)GX(c);
+    const double *u = c->u.f, *v = c->v.f;
+    for (int j = 1; j <= c->ny; j++)
+        for (int i = 1; i <= c->nx; i++) {
+            const size_t g = GI(c, i, j, 1);
+            const double du = u[g] - u[g - 1], dv = v[g] - v[g - sj];
+            if (c->x_first)
+                c->vof.f[g] = vof2[g] - dt * (vof1[g] * du / delta + vof2[g] * dv / delta);
+            else
+                c->vof.f[g] = vof2[g] - dt * (vof2[g] * du / delta + vof1[g] * dv / delta);
+        }
+    c->x_first = !c->x_first;
+    ghost_fill(c, &c->vof);
+    free(vof1);
+    free(vof2);
+}
+
+void orc_vof_reconstruct(orc_ctx *c) { if (c->mf) get_h_from_vof(c); }
+
+double *orc_vof_aux(orc_ctx *c, int which)
+{
+    if (!c->mf) return NULL;
+    ofield *a[7] = {&c->h, &c->dd, &c->curv, &c->nrx, &c->nry, &c->lx2, &c->ly2};
+    return which >= 0 && which < 7 ? a[which]->f : NULL;
+}
+
+/* update_material_properties, src/multiphase.f90:126-140 (whole arrays, then ghosts). */
+void orc_update_material_properties(orc_ctx *c)
+{
+    if (!c->mf) return;
+    const size_t ng = ghosted_len(c);
+    for (size_t q = 0; q < ng; q++) {
+        c->rho.f[q] = c->rho1 * c->vof.f[q] + c->rho0 * (1.0 - c->vof.f[q]);
+        c->mu.f[q] = c->mu1 * c->vof.f[q] + c->mu0 * (1.0 - c->vof.f[q]);
+    }
+    ghost_fill(c, &c->rho);
+    ghost_fill(c, &c->mu);
+}
+
+int orc_set_multiphase(orc_ctx *c, double rho0, double rho1, double mu0, double mu1, double sigma, double beta)
+{
+    if (c->dim != 2 || c->mf) return -1;
+    for (int s = 0; s < 4; s++)
+        if (c->bc[s] != ORC_BC_PERIODIC && c->bc[s] != ORC_BC_WALL) return -1;   /* :54-219 */
+    c->mf = 1;
+    c->x_first = 1;
+    c->rho0 = rho0; c->rho1 = rho1; c->mu0 = mu0; c->mu1 = mu1; c->sigma = sigma; c->beta = beta;
+    c->rhomin = fmin(rho0, rho1);                                 /* src/solver.f90:91-92 */
+    c->irhomin = 1.0 / c->rhomin;
+    ofield *f[12] = {&c->vof, &c->h, &c->dd, &c->curv, &c->nrx, &c->nry, &c->lx2, &c->ly2,
+                     &c->rho, &c->mu, &c->p_hat, &c->p_o};
+    for (int q = 0; q < 12; q++) {
+        field_alloc(c, f[q], 'c');
+        /* Periodic -> 0, Wall -> Neumann (vof family :54-219, rho/mu/p_hat/p_o as p) */
+        for (int s = 0; s < 4; s++) f[q]->type[s] = c->bc[s] == ORC_BC_PERIODIC ? 0 : 2;
+    }
+    return 0;
+}
+
+/* set_timestep, MF branch (src/navier_stokes.f90:653-660). */
+double orc_mf_set_timestep(orc_ctx *c, double U)
+{
+    const double pi = acos(-1.0);
+    c->dt_conv = c->cfl * c->delta / U;
+    c->dt_visc = 0.125 * c->delta * c->delta * fmin(c->rho0 / c->mu0, c->rho1 / c->mu1);
+    double dt = fmin(c->dt_conv, c->dt_visc);
+    if (c->sigma > 0.0) {
+        const double dt_surf = sqrt(0.5 * (c->rho0 + c->rho1) * (c->delta * c->delta * c->delta) /
+                                    (pi * c->sigma + 1.0e-16));
+        dt = fmin(dt, dt_surf);
+    }
+    c->dt_o = dt;
+    return dt;
+}
+
+/* The MF step, src/navier_stokes.f90:50-136 with MF defined (2-D, fixed dt). */
+static void mf_step(orc_ctx *c, double dt, double *maxdiv, double *maxcfl)
+{
+    const int nx = c->nx, ny = c->ny;
+    const size_t ni = interior_len(c), ng = ghosted_len(c);
+    const ptrdiff_t sj = (ptrdiff_t)GX(c);
+    const double id = 1.0 / c->delta;
+
+    orc_advect_vof(c, dt);                                                      /* :82 */
+    orc_update_material_properties(c);                                          /* :85 */
+    for (size_t q = 0; q < ng; q++) c->p_hat.f[q] = 2.0 * c->p.f[q] - c->p_o.f[q];   /* :91 */
+    ghost_fill(c, &c->p_hat);
+
+    /* predicted_velocity_field (:140-213) */
+    const double A = 1.0 + 0.5 * dt / c->dt_o;
+    const double B = -(0.5 * dt / c->dt_o);
+    double *rfx = (double *)malloc(ni * sizeof(double)), *rfy = (double *)malloc(ni * sizeof(double));
+    for (int j = 1; j <= ny; j++)                                               /* center_to_face */
+        for (int i = 1; i <= nx; i++) {
+            const size_t g = GI(c, i, j, 1), q = II(c, i, j, 1);
+            rfx[q] = 0.5 * (c->rho.f[g + 1] + c->rho.f[g]);
+            rfy[q] = 0.5 * (c->rho.f[g + sj] + c->rho.f[g]);
+        }
+    for (int d = 0; d < 3; d++) memset(c->dv[d], 0, ni * sizeof(double));
+    add_advection(c, c->u.f, c->v.f, c->w.f, c->dv[0], c->dv[1], c->dv[2]);
+    /* add_diffusion, variable viscosity (:405-452), then add_surface_tension (:458-501)
+     * and the body force S/rhof (:245-255) */
+    const double *U = c->u.f, *V = c->v.f, *M = c->mu.f, *vof = c->vof.f, *K = c->curv.f;
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            const size_t g = GI(c, i, j, 1), q = II(c, i, j, 1);
+            const ptrdiff_t ip = 1, im = -1, jp = sj, jm = -sj;
+            const double txxp = 2.0 * M[g + ip] * (U[g + ip] - U[g]) * id;
+            const double txxm = 2.0 * M[g] * (U[g] - U[g + im]) * id;
+            const double dtxx = (txxp - txxm) * id;
+            const double txyp = 0.25 * (M[g] + M[g + ip] + M[g + jp] + M[g + ip + jp]) *
+                                ((U[g + jp] - U[g]) * id + (V[g + ip] - V[g]) * id);
+            const double txym = 0.25 * (M[g + jm] + M[g + ip + jm] + M[g] + M[g + ip]) *
+                                ((U[g] - U[g + jm]) * id + (V[g + ip + jm] - V[g + jm]) * id);
+            const double dtxy = (txyp - txym) * id;
+            c->dv[0][q] = c->dv[0][q] + (dtxx + dtxy) / rfx[q];
+            const double tyxp = txyp;
+            const double tyxm = 0.25 * (M[g + im] + M[g] + M[g + im + jp] + M[g + jp]) *
+                                ((U[g + im + jp] - U[g + im]) * id + (V[g] - V[g + im]) * id);
+            const double dtyx = (tyxp - tyxm) * id;
+            const double tyyp = 2.0 * M[g + jp] * (V[g + jp] - V[g]) * id;
+            const double tyym = 2.0 * M[g] * (V[g] - V[g + jm]) * id;
+            const double dtyy = (tyyp - tyym) * id;
+            c->dv[1][q] = c->dv[1][q] + (dtyx + dtyy) / rfy[q];
+        }
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            const size_t g = GI(c, i, j, 1), q = II(c, i, j, 1);
+            c->dv[0][q] = c->dv[0][q] + c->sigma * 0.5 * (K[g + 1] + K[g]) * (vof[g + 1] - vof[g]) * id / rfx[q];
+            c->dv[1][q] = c->dv[1][q] + c->sigma * 0.5 * (K[g + sj] + K[g]) * (vof[g + sj] - vof[g]) * id / rfy[q];
+        }
+    for (size_t q = 0; q < ni; q++) {
+        c->dv[0][q] = c->dv[0][q] + c->S[0][q] / rfx[q];
+        c->dv[1][q] = c->dv[1][q] + c->S[1][q] / rfy[q];
+    }
+    gradient(c, c->p.f, c->gp[0], c->gp[1], c->gp[2]);
+    double *gphx = (double *)malloc(ni * sizeof(double)), *gphy = (double *)malloc(ni * sizeof(double));
+    gradient(c, c->p_hat.f, gphx, gphy, NULL);
+    double *vel[2] = {c->u.f, c->v.f};
+    double *rf[2] = {rfx, rfy}, *gh[2] = {gphx, gphy};
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            const size_t g = GI(c, i, j, 1), q = II(c, i, j, 1);
+            for (int d = 0; d < 2; d++) {
+                const double gp = c->gp[d][q], r = rf[d][q];
+                double rhs = ((-(gp / r) + A * c->dv[d][q]) + B * c->dvo[d][q]) + c->g[d];   /* :174-176 */
+                rhs = ((rhs + gp / r) - c->irhomin * gp) - (1.0 / r - c->irhomin) * gh[d][q]; /* :178-184 */
+                vel[d][g] = vel[d][g] + dt * rhs;
+            }
+        }
+    for (int d = 0; d < 2; d++) memcpy(c->dvo[d], c->dv[d], ni * sizeof(double));
+    ghost_fill(c, &c->u); ghost_fill(c, &c->v);
+
+    /* Poisson right-hand side with rho_min (:111-112) */
+    double *div = c->gp[0];
+    divergence(c, c->u.f, c->v.f, c->w.f, div);
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) c->phi.f[GI(c, i, j, 1)] = div[II(c, i, j, 1)];
+    for (size_t q = 0; q < ng; q++) c->phi.f[q] = c->phi.f[q] * c->rhomin / dt;   /* whole array */
+    orc_poisson_solve(c, c->phi.f, 1);
+    ghost_fill(c, &c->phi);
+
+    /* correct_velocity_field, MF (:526-531) */
+    gradient(c, c->phi.f, c->gp[0], c->gp[1], c->gp[2]);
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            const size_t g = GI(c, i, j, 1), q = II(c, i, j, 1);
+            for (int d = 0; d < 2; d++) vel[d][g] = vel[d][g] - c->gp[d][q] * dt * c->irhomin;
+        }
+    ghost_fill(c, &c->u); ghost_fill(c, &c->v);
+
+    /* update_pressure (:550-566): p_o = p, p = p + phi */
+    memcpy(c->p_o.f, c->p.f, ng * sizeof(double));
+    for (size_t q = 0; q < ng; q++) c->p.f[q] = c->p.f[q] + c->phi.f[q];
+    ghost_fill(c, &c->p);
+
+    divergence(c, c->u.f, c->v.f, c->w.f, div);
+    double md = -HUGE_VAL;
+    for (size_t q = 0; q < ni; q++) if (div[q] > md) md = div[q];
+    c->maxdiv = md;
+    c->maxcfl = dt * max_velocity_sum(c) / c->delta;
+    if (maxdiv) *maxdiv = c->maxdiv;
+    if (maxcfl) *maxcfl = c->maxcfl;
+    free(rfx); free(rfy); free(gphx); free(gphy);
 }
 
 /* -------------------------------------------------------------------------- */

@@ -24,7 +24,8 @@ enum {
     ORC_P = 0, ORC_PHI = 1, ORC_U = 2, ORC_V = 3, ORC_W = 4,
     ORC_DVO_X = 5, ORC_DVO_Y = 6, ORC_DVO_Z = 7,
     ORC_S_X = 8, ORC_S_Y = 9, ORC_S_Z = 10,
-    ORC_NFIELDS = 11
+    ORC_VOF = 11, ORC_RHO = 12, ORC_MU = 13,   /* two-phase (MF) fields, ghosted */
+    ORC_NFIELDS = 14
 };
 
 /* Physical boundary codes (grid%boundary_conditions strings). */
@@ -60,6 +61,22 @@ void orc_update_ghost_nodes(orc_ctx *c, int field);
 
 /* One navier_stokes_solver call. */
 void orc_ns_step(orc_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl);
+
+/* Two-phase (MF build) solver, 2-D: MTHINC volume of fluid (src/volume_of_fluid.f90),
+ * material properties rho, mu from the vof (src/multiphase.f90:126-140), CSF surface
+ * tension and the Dodd & Ferrante pressure splitting (src/navier_stokes.f90:80-96,
+ * 174-184, 526-531).  orc_set_multiphase switches the context to it (returns 0, or -1
+ * for 3-D or for faces other than Periodic/Wall); after the vof is written,
+ * orc_update_material_properties forms rho and mu as init_solver does
+ * (src/solver.f90:85-93).  orc_ns_step then runs the MF step at fixed dt. */
+int orc_set_multiphase(orc_ctx *c, double rho0, double rho1, double mu0, double mu1, double sigma,
+                       double beta);
+void orc_update_material_properties(orc_ctx *c);
+double orc_mf_set_timestep(orc_ctx *c, double U);    /* set_timestep, MF branch */
+void orc_advect_vof(orc_ctx *c, double dt);          /* advect_vof with the resident u, v */
+void orc_vof_reconstruct(orc_ctx *c);                /* get_h_from_vof (with compute_norm) */
+/* Reconstruction fields: 0 h, 1 d, 2 curv, 3 norm x, 4 norm y, 5 l x, 6 l y (ghosted). */
+double *orc_vof_aux(orc_ctx *c, int which);
 
 /* Operators on host arrays shaped like the context's fields. */
 void orc_add_advection(orc_ctx *c, double *rx, double *ry, double *rz);   /* interior-sized (gl=0) */

@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-P, PHI, U, V, W, DVO_X, DVO_Y, DVO_Z, S_X, S_Y, S_Z = range(11)
+P, PHI, U, V, W, DVO_X, DVO_Y, DVO_Z, S_X, S_Y, S_Z, VOF, RHO, MU = range(14)
 BC_PERIODIC, BC_WALL, BC_INFLOW, BC_OUTFLOW = range(4)
 SOLVERS = {0: "pp", 1: "pn", 2: "nn", 3: "ppp", 4: "ppn", 5: "npn", 6: "nnn"}
 
@@ -71,6 +71,15 @@ def lib():
         L.orc_set_num_threads.argtypes = [ctypes.c_int]
         L.orc_adjoint_step.argtypes = [ctypes.c_void_p, ctypes.c_double] + [dp] * 7
         L.orc_adjoint_step.restype = ctypes.c_int
+        L.orc_set_multiphase.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 6
+        L.orc_set_multiphase.restype = ctypes.c_int
+        L.orc_update_material_properties.argtypes = [ctypes.c_void_p]
+        L.orc_mf_set_timestep.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_mf_set_timestep.restype = ctypes.c_double
+        L.orc_advect_vof.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_vof_reconstruct.argtypes = [ctypes.c_void_p]
+        L.orc_vof_aux.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_vof_aux.restype = dp
         _lib = L
     return _lib
 
@@ -135,7 +144,7 @@ class Oracle:
         n = lib().orc_field_len(self.h, fid)
         ptr = lib().orc_field(self.h, fid)
         flat = np.ctypeslib.as_array(ptr, shape=(n,))
-        shape = self.ghosted_shape() if fid <= W else self.n
+        shape = self.ghosted_shape() if (fid <= W or fid >= VOF) else self.n
         return flat.reshape(shape, order="F")
 
     def bc_plane(self, fid, face):
@@ -185,6 +194,29 @@ class Oracle:
         rc = lib().orc_adjoint_step(self.h, dt, *[_p(a) for a in arrs])
         if rc != 0:
             raise ValueError("adjoint step needs a 3-D triply periodic context")
+
+    # ---- two-phase (MF) solver, 2-D
+    def set_multiphase(self, rho0, rho1, mu0, mu1, sigma=0.0, beta=1.0):
+        if lib().orc_set_multiphase(self.h, rho0, rho1, mu0, mu1, sigma, beta) != 0:
+            raise ValueError("the two-phase solver is 2-D with Periodic or Wall faces")
+
+    def update_material_properties(self):
+        lib().orc_update_material_properties(self.h)
+
+    def mf_set_timestep(self, U):
+        return lib().orc_mf_set_timestep(self.h, U)
+
+    def advect_vof(self, dt):
+        lib().orc_advect_vof(self.h, dt)
+
+    def vof_reconstruct(self):
+        lib().orc_vof_reconstruct(self.h)
+
+    def vof_aux(self, which):
+        """0 h, 1 d, 2 curv, 3 norm x, 4 norm y, 5 l x, 6 l y (ghosted views)."""
+        n = int(np.prod(self.ghosted_shape()))
+        ptr = lib().orc_vof_aux(self.h, which)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self.ghosted_shape(), order="F")
 
     def poisson_solve(self, f, gl):
         """solve_poisson in place on a Fortran-ordered array with ghost level gl."""
