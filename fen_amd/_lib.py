@@ -78,6 +78,9 @@ def load():
         "fen_profile_predecessor": (c_int, [vp, c_int, ip]),
         "fen_adjoint_record": (c_int, [vp, c_int, c_int, c_int, dp]),
         "fen_adjoint_backward": (c_int, [vp, dp, dp, dp, dp, dp, dp, dp]),
+        "fen_set_multiphase": (c_int, [vp, c_double, c_double, c_double, c_double, c_double, c_double]),
+        "fen_update_material_properties": (c_int, [vp]),
+        "fen_advect_vof": (c_int, [vp, c_double]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

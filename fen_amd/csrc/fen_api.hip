@@ -249,6 +249,9 @@ void *field_ptr(fen_ctx *c, int field)
     case FEN_W: return c->d_u[2];
     case FEN_DVO_X: case FEN_DVO_Y: case FEN_DVO_Z: return c->d_dvo[field - FEN_DVO_X];
     case FEN_S_X: case FEN_S_Y: case FEN_S_Z: return c->d_S[field - FEN_S_X];
+    case FEN_VOF: return c->mf ? c->mf->vof : nullptr;
+    case FEN_RHO: return c->mf ? c->mf->rho : nullptr;
+    case FEN_MU: return c->mf ? c->mf->mu : nullptr;
     }
     return nullptr;
 }
@@ -263,6 +266,14 @@ void free_all(fen_ctx *c)
     f(c->d_mwx); f(c->d_mwy); f(c->d_mwz); f(c->d_ta); f(c->d_tb); f(c->d_tc);
     f(c->d_twx); f(c->d_twy); f(c->d_twz); f(c->d_spec2); f(c->d_mkx); f(c->d_mky); f(c->d_part); f(c->d_scal);
     for (int s = 0; s < 5; s++) for (int p = 0; p < 6; p++) f(c->d_plane[s][p]);
+    if (c->mf) {
+        MFState &m = *c->mf;
+        for (double *q : {m.vof, m.vof1, m.vof2, m.h, m.dd, m.curv, m.nrx, m.nry, m.lx, m.ly, m.rho, m.mu, m.p_hat,
+                          m.p_o})
+            f(q);
+        delete c->mf;
+        c->mf = nullptr;
+    }
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     for (auto e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -387,7 +398,8 @@ template <class R> void solve_phi(fen_ctx *c, double dt, void *const *us)
 {
     using namespace fen;
     const R *vs[3] = {as<R>(us[0]), as<R>(us[1]), as<R>(us[2])};
-    solve_to_x_spectrum<R>(c, vs, nullptr, 0, 0, 0, c->density, dt);
+    // phi = div(u*) rho / dt; the MF build scales by rho_min (src/navier_stokes.f90:111-121)
+    solve_to_x_spectrum<R>(c, vs, nullptr, 0, 0, 0, c->mf ? c->mf->rhomin : c->density, dt);
     const bool mean = removes_mean(c);
     bool dirichlet = false;
     for (int f = 0; f < 6; f++) dirichlet = dirichlet || c->bc_type[FEN_PHI][f] == 1;
@@ -446,6 +458,27 @@ template <class R> void advance(fen_ctx *c, double dt)
     predict<R>(c, dt);
     solve_phi<R>(c, dt, c->d_us);
     finish_step<R>(c, dt);
+}
+
+// The MF build's step (src/navier_stokes.f90:50-136 with MF, fixed dt), 2-D FP64.
+void advance_mf(fen_ctx *c, double dt)
+{
+    using namespace fen;
+    c->last_dt = dt;
+    c->step_count++;
+    mf_advect_vof(c, dt);          // :82
+    mf_update_fields(c, true);     // :85 and p_hat = 2 p - p_o (:91), with ghosts
+    mf_momentum(c, dt);            // predicted_velocity_field with MF (:140-213)
+    if (const int rest = remaining_groups(c, c->img_vel)) {
+        GhostDesc<double> g[2] = {make_desc<double>(c, FEN_U, c->d_us[0]), make_desc<double>(c, FEN_V, c->d_us[1])};
+        for (auto &d : g) d.groups = rest;
+        launch_ghost_desc<double>(c, g, 2);
+    }
+    solve_phi<double>(c, dt, c->d_us);
+    // update_pressure with MF: p_o = p before p = p + phi (:553-559)
+    FEN_HIP_CHECK(hipMemcpyAsync(c->mf->p_o, c->d_p, c->n_ghost * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    finish_step<double>(c, dt);
 }
 
 // Whether fen_ns_run may fuse each step's projection with the next step's
@@ -710,8 +743,64 @@ int fen_set_timestep(fen_ctx *c, double U, double *dt)
     c->dt_visc = 0.125 * d * d * c->density / c->viscosity;
     if (c->dim == 3) c->dt_visc = (1.0 / 6.0) * d * d * c->density / c->viscosity;
     *dt = c->dt_conv < c->dt_visc ? c->dt_conv : c->dt_visc;
+    if (c->mf) {   // MF branch (:653-660)
+        const MFState &m = *c->mf;
+        c->dt_visc = 0.125 * d * d * std::fmin(m.rho0 / m.mu0, m.rho1 / m.mu1);
+        *dt = std::fmin(c->dt_conv, c->dt_visc);
+        if (m.sigma > 0.0) {
+            const double pi = std::acos(-1.0);
+            const double dt_surf = std::sqrt(0.5 * (m.rho0 + m.rho1) * (d * d * d) / (pi * m.sigma + 1.0e-16));
+            *dt = std::fmin(*dt, dt_surf);
+        }
+    }
     c->dt_o = *dt;
     c->have_dt_o = c->have_dt_visc = true;
+    return FEN_OK;
+}
+
+int fen_set_multiphase(fen_ctx *c, double rho0, double rho1, double mu0, double mu1, double sigma, double beta)
+{
+    if (!c) return FEN_ERR_ARG;
+    if (c->dim != 2 || c->precision != FEN_FP64 || c->mf)
+        return fail(c, FEN_ERR_UNSUPPORTED, "the two-phase mode is 2-D, FP64, set once");
+    for (int f = 0; f < 4; f++)
+        if (c->bc_phys[f] != FEN_BC_PERIODIC && c->bc_phys[f] != FEN_BC_WALL)
+            return fail(c, FEN_ERR_UNSUPPORTED, "two-phase faces must be Periodic or Wall");
+    if (!(rho0 > 0.0 && rho1 > 0.0 && mu0 > 0.0 && mu1 > 0.0 && beta > 0.0 && sigma >= 0.0))
+        return fail(c, FEN_ERR_ARG, "two-phase properties must be positive");
+    FEN_TRY(c, {
+        MFState *m = new MFState();
+        m->rho0 = rho0; m->rho1 = rho1; m->mu0 = mu0; m->mu1 = mu1; m->sigma = sigma; m->beta = beta;
+        m->rhomin = std::fmin(rho0, rho1);   // src/solver.f90:91-92
+        m->irhomin = 1.0 / m->rhomin;
+        c->mf = m;
+        for (double **q : {&m->vof, &m->vof1, &m->vof2, &m->h, &m->dd, &m->curv, &m->nrx, &m->nry, &m->lx, &m->ly,
+                           &m->rho, &m->mu, &m->p_hat, &m->p_o})
+            *q = dev_alloc<double>(c->n_ghost);
+        c->config_version++;
+    });
+    return FEN_OK;
+}
+
+int fen_update_material_properties(fen_ctx *c)
+{
+    if (!c) return FEN_ERR_ARG;
+    if (!c->mf) return fail(c, FEN_ERR_STATE, "not in two-phase mode");
+    FEN_TRY(c, {
+        fen::mf_update_fields(c, false);
+        sync(c);
+    });
+    return FEN_OK;
+}
+
+int fen_advect_vof(fen_ctx *c, double dt)
+{
+    if (!c) return FEN_ERR_ARG;
+    if (!c->mf) return fail(c, FEN_ERR_STATE, "not in two-phase mode");
+    FEN_TRY(c, {
+        fen::mf_advect_vof(c, dt);
+        sync(c);
+    });
     return FEN_OK;
 }
 
@@ -747,8 +836,9 @@ int fen_get_scalars(fen_ctx *c, double *maxdiv, double *maxcfl, double *dt_o, do
 int fen_field_count(const fen_ctx *c, int field, size_t *count)
 {
     if (!c || !count) return FEN_ERR_ARG;
-    if (field < FEN_P || field > FEN_S_Z) return FEN_ERR_ARG;
-    *count = field <= FEN_W ? c->n_ghost : c->n_int;
+    if (field < FEN_P || field > FEN_MU) return FEN_ERR_ARG;
+    if (field >= FEN_VOF && !c->mf) return FEN_ERR_ARG;
+    *count = (field <= FEN_W || field >= FEN_VOF) ? c->n_ghost : c->n_int;
     return FEN_OK;
 }
 
@@ -759,11 +849,13 @@ int fen_upload(fen_ctx *c, int field, const double *host, size_t count)
     if (fen_field_count(c, field, &n) != FEN_OK || n != count)
         return fail(c, FEN_ERR_ARG, "field id or element count does not match the grid");
     if (field == FEN_W && c->dim == 2) return fail(c, FEN_ERR_ARG, "no w component in 2-D");
+    if (field == FEN_RHO || field == FEN_MU)
+        return fail(c, FEN_ERR_ARG, "rho and mu follow from the vof (fen_update_material_properties)");
     FEN_TRY(c, {
-        if (field >= FEN_S_X && !c->d_S[0]) {
+        if (field >= FEN_S_X && field <= FEN_S_Z && !c->d_S[0]) {
             for (int d = 0; d < 3; d++) c->d_S[d] = dev_alloc_bytes(c->n_int * elem(c));
         }
-        if (field >= FEN_S_X && !c->has_source) {
+        if (field >= FEN_S_X && field <= FEN_S_Z && !c->has_source) {
             c->has_source = true;
             c->config_version++;
         }
@@ -834,6 +926,10 @@ int fen_get_bc_type(const fen_ctx *c, int field, int face, int *type)
 int fen_update_ghost_nodes(fen_ctx *c, int field)
 {
     if (!c) return FEN_ERR_ARG;
+    if (field == FEN_VOF && c->mf) {   // vof%update_ghost_nodes: Periodic or Neumann
+        FEN_TRY(c, fen::mf_ghost_field(c, c->mf->vof));
+        return FEN_OK;
+    }
     if (field_slot(field) < 0) return fail(c, FEN_ERR_ARG, "field has no ghost nodes");
     if (field == FEN_W && c->dim == 2) return fail(c, FEN_ERR_ARG, "no w component in 2-D");
     FEN_TRY(c, {
@@ -856,6 +952,7 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
     (void)step;
     if (!c || !dt) return FEN_ERR_ARG;
     if (!c->have_dt_o) return fail(c, FEN_ERR_STATE, "dt_o is undefined: call fen_set_timestep or fen_set_dt_o");
+    if (c->mf && c->constant_cfl) return fail(c, FEN_ERR_UNSUPPORTED, "constant CFL with the two-phase mode");
     FEN_TRY(c, {
         if (c->constant_cfl) {
             // update_timestep (src/navier_stokes.f90:670-730)
@@ -876,7 +973,8 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
             if (*dt > 1.1 * c->dt_o) *dt = 1.1 * c->dt_o;
         }
         const double dtv = *dt;
-        if (c->use_graph && !c->constant_cfl && !c->prof_mask) advance_graph(c, dtv);
+        if (c->mf) advance_mf(c, dtv);
+        else if (c->use_graph && !c->constant_cfl && !c->prof_mask) advance_graph(c, dtv);
         else with_type(c, [&](auto tag) { advance<decltype(tag)>(c, dtv); });
         FEN_HIP_CHECK(hipGetLastError());
     });

@@ -157,6 +157,16 @@ struct AdjointTape;
 
 }  // namespace fen
 
+// Two-phase (MF) state: volume_of_fluid_mod and multiphase_mod fields (FP64,
+// ghosted, the host's (nx+2)(ny+2)(nz+2) layout).
+struct MFState {
+    double rho0 = 1, rho1 = 1, mu0 = 1, mu1 = 1, sigma = 0, beta = 1, rhomin = 1, irhomin = 1;
+    int x_first = 1;
+    double *vof = nullptr, *vof1 = nullptr, *vof2 = nullptr, *h = nullptr, *dd = nullptr, *curv = nullptr;
+    double *nrx = nullptr, *nry = nullptr, *lx = nullptr, *ly = nullptr;
+    double *rho = nullptr, *mu = nullptr, *p_hat = nullptr, *p_o = nullptr;
+};
+
 struct fen_ctx {
     int dim = 3, nx = 0, ny = 0, nz = 0;
     double lx = 0, ly = 0, lz = 0, x0[3] = {0, 0, 0}, delta = 0;
@@ -183,6 +193,7 @@ struct fen_ctx {
     void *d_us[3] = {nullptr, nullptr, nullptr};   // predicted velocity u*
     void *d_p = nullptr, *d_phi = nullptr;
     void *d_p2 = nullptr;   // second pressure array of a fused run (allocated on first use)
+    MFState *mf = nullptr;  // two-phase mode (fen_set_multiphase)
     void *d_dvo[3] = {nullptr, nullptr, nullptr};
     void *d_S[3] = {nullptr, nullptr, nullptr};
     bool has_source = false;
@@ -257,6 +268,10 @@ template <class R> void launch_thomas(fen_ctx *c);
 template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img);
 template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out);
 bool fused_z_walls(const fen_ctx *c);
+void mf_advect_vof(fen_ctx *c, double dt);
+void mf_update_fields(fen_ctx *c, bool with_phat);
+void mf_momentum(fen_ctx *c, double dt);
+void mf_ghost_field(fen_ctx *c, double *f);
 template <class R>
 void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
                              const void *p_in, void *const *us_out, void *p_out, double *checks_out);

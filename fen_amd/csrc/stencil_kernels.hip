@@ -447,6 +447,11 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     a.rhof = 0.5 * (c->density + c->density);
     a.rpow2 = pow2_value(a.rhof) ? 1 : 0;
     a.irhof = a.rpow2 ? 1.0 / a.rhof : 0.0;
+    if (c->mf) {   // MF: v - grad(phi) dt / rho_min as (grad(phi) dt) * irhomin (src/navier_stokes.f90:526-531)
+        a.rhof = c->mf->rhomin;
+        a.rpow2 = 1;
+        a.irhof = c->mf->irhomin;
+    }
     a.img = c->img_vel & c->img_p;
     int nbz;
     using SH = Shape<R>;

@@ -103,7 +103,7 @@ class Solver:
     # -- fields -----------------------------------------------------------
     def shape(self, field):
         nx, ny, nz = self.n
-        return (nx + 2, ny + 2, nz + 2) if field <= W else (nx, ny, nz)
+        return (nx + 2, ny + 2, nz + 2) if (field <= W or field >= _lib.VOF) else (nx, ny, nz)
 
     def count(self, field):
         c = ctypes.c_size_t()
@@ -170,6 +170,19 @@ class Solver:
         self._check(self.lib.fen_adjoint_backward(self.h, _dp(s[0]), _dp(s[1]), _dp(s[2]), _dp(g[0]), _dp(g[1]),
                                                   _dp(g[2]), _dp(g[3]) if want_p else None))
         return g
+
+    # -- two-phase mode (MF build), 2-D ---------------------------------------
+    def set_multiphase(self, rho0, rho1, mu0, mu1, sigma=0.0, beta=1.0):
+        """allocate_vof_fields + allocate_multiphase_fields and the phase
+        properties (src/solver.f90:81-94); then upload VOF and call
+        update_material_properties."""
+        self._check(self.lib.fen_set_multiphase(self.h, rho0, rho1, mu0, mu1, sigma, beta))
+
+    def update_material_properties(self):
+        self._check(self.lib.fen_update_material_properties(self.h))
+
+    def advect_vof(self, dt):
+        self._check(self.lib.fen_advect_vof(self.h, dt))
 
     def synchronize(self):
         self._check(self.lib.fen_synchronize(self.h))

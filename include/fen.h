@@ -47,7 +47,10 @@ enum {
     FEN_DVO_Z = 7,
     FEN_S_X = 8,    /* volumetric source S%x, interior only */
     FEN_S_Y = 9,
-    FEN_S_Z = 10
+    FEN_S_Z = 10,
+    FEN_VOF = 11,   /* two-phase mode: volume of fluid (volume_of_fluid_mod vof), ghosted */
+    FEN_RHO = 12,   /* two-phase mode: density rho (navier_stokes_mod rho), ghosted, read-only */
+    FEN_MU = 13     /* two-phase mode: viscosity mu, ghosted, read-only */
 };
 
 /* Physical boundary codes: the strings of grid%boundary_conditions
@@ -149,6 +152,29 @@ int fen_ns_step(fen_ctx *ctx, int step, double *dt, double *maxdiv, double *maxc
  * calls bit for bit.  FEN_FUSED_RUN=0 in the environment at fen_create
  * turns the fusion off. */
 int fen_ns_run(fen_ctx *ctx, int step0, int nsteps, double *dt);
+
+/* Two-phase mode (FEN's MF build), 2-D, FP64, fixed dt: replaces
+ * allocate_vof_fields + allocate_multiphase_fields and the rho_0, rho_1,
+ * mu_0, mu_1, sigma (multiphase_mod, src/multiphase.f90:13-20) and beta
+ * (volume_of_fluid_mod, src/volume_of_fluid.f90:24) settings of init_solver
+ * (src/solver.f90:81-94).  After it, fen_ns_step runs the MF step
+ * (src/navier_stokes.f90:80-96 with MF): MTHINC advection of the vof,
+ * material properties, p_hat, momentum with variable rho and mu and CSF
+ * surface tension, the Poisson solve with rho_min, projection with 1/rho_min,
+ * p_o = p.  fen_set_timestep takes the MF branch (:653-660).  Faces must be
+ * Periodic or Wall (the reference stops otherwise); constant CFL is not
+ * supported in this mode.  Returns FEN_ERR_UNSUPPORTED for 3-D or FP32. */
+int fen_set_multiphase(fen_ctx *ctx, double rho0, double rho1, double mu0, double mu1, double sigma,
+                       double beta);
+
+/* update_material_properties (src/multiphase.f90:126-140): rho and mu from the
+ * resident vof; call after uploading FEN_VOF, as init_solver does after
+ * get_vof_from_distance (src/solver.f90:88-89). */
+int fen_update_material_properties(fen_ctx *ctx);
+
+/* advect_vof(v, dt) alone (src/volume_of_fluid.f90:420-544) with the resident
+ * velocity, as the reference's volume-of-fluid tests call it. */
+int fen_advect_vof(fen_ctx *ctx, double dt);
 
 /* Replaces the solve_poisson procedure pointer (src/poisson.f90:21-29):
  * solves lap(phi) = f in place on a host array with ghost level gl (0 or 1),
