@@ -149,6 +149,35 @@ def channel_rate(steps=20, precision=0):
     return case_rate(cases.channel(steps=steps), steps, precision)
 
 
+def multiphase_rate(steps=100):
+    """FP64 time per step of the two-phase mode on the reference's
+    Rayleigh-Taylor case (2-D 128 x 512, MF build)."""
+    import numpy as np
+    from fen_amd import cases
+    from fen_amd._lib import VOF
+    from tests.common import make_gpu
+    case, props, vof = cases.rayleigh_taylor(steps=steps)
+    g = make_gpu(case)
+    g.set_multiphase(**props)
+    full = np.zeros((case.n[0] + 2, case.n[1] + 2, 3), order="F")
+    full[1:-1, 1:-1, 1:2] = vof
+    g.upload(VOF, full)
+    g.update_ghost_nodes(VOF)
+    g.update_material_properties()
+    dt = g.set_timestep(1.0)
+    g.run(0, 3, dt)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(3, steps, dt)
+    g.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    md = g.scalars()["maxdiv"]
+    g.close()
+    cells = case.n[0] * case.n[1]
+    return {"workload": "Rayleigh-Taylor 2-D 128x512 (MF, pn)", "ms_per_step": ms, "value": cells / (ms * 1e-3),
+            "steps": steps, "maxdiv": md}
+
+
 def neumann_rates(steps=20):
     """FP64 time per step with the Neumann-direction solvers at 256^3 (and
     256^2 in 2-D): lid-driven cavity (nnn), duct (npn), 2-D cavity (nn) and
@@ -322,6 +351,7 @@ def main():
         extra["channel_512x512x256"] = {"fp64": channel_rate(20, 0), "fp32_refine": channel_rate(20, FP32_REFINE)}
         extra["tgv2d_128"] = tgv2d_time()
         extra["neumann_solvers"] = neumann_rates(20)
+        extra["multiphase_2d"] = multiphase_rate(100)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -260,5 +260,34 @@ def cavity3d(nx=32, ny=32, nz=32, Re=100.0, steps=30, noise=0.01, cfl=0.5, seed=
                 planes={(U, 5): lid})
 
 
+def vof_from_distance(dist, nx, ny, delta, beta=1.0):
+    """Cell averages of 0.5 (1 + tanh(beta d / delta)) by 2x2 Gauss points, as
+    get_vof_from_distance (reference src/volume_of_fluid.f90:665-706)."""
+    rp, rm = 0.5 * (1.0 + 1.0 / math.sqrt(3.0)), 0.5 * (1.0 - 1.0 / math.sqrt(3.0))
+    X, Y = np.meshgrid((np.arange(1, nx + 1) - 0.5) * delta, (np.arange(1, ny + 1) - 0.5) * delta, indexing="ij")
+    xm, xp = X + delta * (rm - 0.5), X + delta * (rp - 0.5)
+    ym, yp = Y + delta * (rm - 0.5), Y + delta * (rp - 0.5)
+    t = lambda x, y: 0.5 * (1.0 + np.tanh(beta * dist(x, y) / delta))
+    return 0.5 * (0.5 * (t(xm, ym) + t(xp, ym)) + 0.5 * (t(xm, yp) + t(xp, yp)))
+
+
+def rayleigh_taylor(nx=128, ny=512, steps=100):
+    """The reference's Rayleigh-Taylor test (test/small_test/multiphase/
+    Rayleigh-Taylor): Lx = 1, Ly = 4, x periodic, walls at y, air-like
+    rho_0 = 0.1694 below the interface y = Ly/2 + 0.05 cos(2 pi x), rho_1 =
+    1.225 above, mu = 3.13e-3 in both, g = -9.80665 in y, no surface tension.
+    Returns (case, multiphase properties, interior vof)."""
+    Lx, Ly = 1.0, 4.0
+    d = Lx / nx
+    vof = vof_from_distance(lambda x, y: y - 0.05 * np.cos(2.0 * math.pi * x / Lx) - Ly / 2.0, nx, ny, d)
+    shp = (nx, ny, 1)
+    z = np.zeros(shp, order="F")
+    bc = [BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC]
+    props = dict(rho0=0.1694, rho1=1.225, mu0=3.13e-3, mu1=3.13e-3, sigma=0.0)
+    case = Case("rayleigh_taylor", 2, shp, (Lx, Ly, d), bc, 1.0, 3.13e-3, (0.0, -9.80665, 0.0), 0.0, 0.0, steps,
+                {U: z, V: z.copy(), P: z.copy()}, cfl=1.0)
+    return case, props, vof[:, :, None]
+
+
 def interior(shape_ghosted):
     return tuple(slice(1, s - 1) for s in shape_ghosted)
