@@ -10,6 +10,7 @@ module fen_capi
     integer(c_int), parameter :: FEN_P = 0, FEN_PHI = 1, FEN_U = 2, FEN_V = 3, FEN_W = 4
     integer(c_int), parameter :: FEN_DVO_X = 5, FEN_DVO_Y = 6, FEN_DVO_Z = 7
     integer(c_int), parameter :: FEN_S_X = 8, FEN_S_Y = 9, FEN_S_Z = 10
+    integer(c_int), parameter :: FEN_VOF = 11, FEN_RHO = 12, FEN_MU = 13
     integer(c_int), parameter :: FEN_BC_PERIODIC = 0, FEN_BC_WALL = 1, FEN_BC_INFLOW = 2, FEN_BC_OUTFLOW = 3
     integer(c_int), parameter :: FEN_FP64 = 0, FEN_FP32_REFINE = 1
 
@@ -120,6 +121,24 @@ module fen_capi
             type(c_ptr), value :: ctx
             real(c_double), intent(inout) :: f(*)
             integer(c_int), value :: gl
+        end function
+
+        integer(c_int) function fen_set_multiphase(ctx, rho0, rho1, mu0, mu1, sigma, beta) &
+                bind(C, name = 'fen_set_multiphase')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: rho0, rho1, mu0, mu1, sigma, beta
+        end function
+
+        integer(c_int) function fen_update_material_properties(ctx) bind(C, name = 'fen_update_material_properties')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function fen_advect_vof(ctx, dt) bind(C, name = 'fen_advect_vof')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: dt
         end function
 
         integer(c_int) function fen_synchronize(ctx) bind(C, name = 'fen_synchronize')

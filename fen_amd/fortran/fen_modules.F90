@@ -4,7 +4,9 @@
 ! The modules keep the reference's names and signatures so that FEN user
 ! programs compile against them: precision_mod, global_mod, IO_mod, grid_mod,
 ! scalar_mod, vector_mod, halo_mod, fields_mod, poisson_mod,
-! navier_stokes_mod, solver_mod.  Build with -DDIM=2 or -DDIM=3 as FEN is.
+! navier_stokes_mod, solver_mod, and with -DMF (two-phase build, 2-D only,
+! as FEN's volume of fluid) volume_of_fluid_mod and multiphase_mod.  Build
+! with -DDIM=2 or -DDIM=3 as FEN is.
 !
 ! Residency: the navier_stokes_mod fields live in HBM during the run.  Their
 ! host arrays (p%f, v%x%f, ...) are mirrors: everything the host holds is
@@ -38,6 +40,7 @@ module global_mod
 #endif
     real(dp), parameter :: pi = acos(-1.0_dp)
     real(dp), parameter :: small = 1.0e-14_dp
+    real(dp), parameter :: gravity = 9.80665_dp
     integer :: myrank = 0
     integer :: ierror = 0
 end module global_mod
@@ -682,6 +685,11 @@ module navier_stokes_mod
 
     type(scalar) :: p, phi, rho, mu
     type(vector) :: v, dv, dv_o, grad_p, S, rhof
+#ifdef MF
+    !> Volume fraction of phase 1 (re-exported by volume_of_fluid_mod).
+    type(scalar) :: vof
+    logical, private :: multiphase = .false.
+#endif
 
     logical :: constant_CFL = .false.
     logical :: constant_viscosity = .true.
@@ -717,6 +725,9 @@ contains
         call S%allocate(comp_grid)
         rho%f = density
         mu%f = viscosity
+#ifdef MF
+        call vof%allocate(comp_grid, 1)
+#endif
 
         call physical_bc_codes(comp_grid, codes)
         call map_face(1, codes(1))
@@ -775,6 +786,14 @@ contains
             call set_type(rho, face, 0)
             call set_type(mu, face, 0)
         end if
+#ifdef MF
+        ! Periodic or Neumann (src/volume_of_fluid.f90:69-219)
+        if (code == FEN_BC_PERIODIC) then
+            call set_type(vof, face, 0)
+        else
+            call set_type(vof, face, 2)
+        end if
+#endif
         call set_type(v%x, face, tu)
         call set_type(v%y, face, tv)
         call set_type(v%z, face, tw)
@@ -849,6 +868,12 @@ contains
         call upload(v%z)
         call upload(dv_o%z)
 #endif
+#ifdef MF
+        if (multiphase) then
+            call upload(vof)
+            call fen_check(fen_ctx, fen_update_ghost_nodes(fen_ctx, FEN_VOF), 'vof ghost nodes')
+        end if
+#endif
         if (any(S%x%f /= 0.0_dp) .or. any(S%y%f /= 0.0_dp)) then
             fsrc = FEN_S_X
             call fen_check(fen_ctx, fen_upload(fen_ctx, fsrc, S%x%f, int(size(S%x%f), c_size_t)), 'S')
@@ -876,7 +901,39 @@ contains
         call download(v%z)
         call download(dv_o%z)
 #endif
+#ifdef MF
+        if (multiphase) then
+            call download(vof)
+            call download(rho)
+            call download(mu)
+        end if
+#endif
     end subroutine fen_sync_host
+
+    !> Uploads the host mirrors if they hold data the device lacks.
+    subroutine fen_push_host()
+        if (host_modified) call fen_sync_device()
+    end subroutine fen_push_host
+
+    !> Declares that a library call changed resident fields.
+    subroutine fen_device_modified()
+        device_ahead = .true.
+    end subroutine fen_device_modified
+
+#ifdef MF
+    !> Makes the context a two-phase one (fen_set_multiphase) and the vof,
+    !> rho and mu mirrors resident fields.
+    subroutine enable_multiphase(rho_0, rho_1, mu_0, mu_1, sigma, beta)
+        real(dp), intent(in) :: rho_0, rho_1, mu_0, mu_1, sigma, beta
+        call fen_check(fen_ctx, fen_set_multiphase(fen_ctx, rho_0, rho_1, mu_0, mu_1, sigma, beta), &
+                       'fen_set_multiphase')
+        vof%fid = FEN_VOF
+        rho%fid = FEN_RHO
+        mu%fid = FEN_MU
+        multiphase = .true.
+        host_modified = .true.
+    end subroutine enable_multiphase
+#endif
 
     !> Declares that user code changed host mirrors (fields, bc, S) mid-run.
     subroutine fen_host_modified()
@@ -945,6 +1002,10 @@ contains
         call grad_p%destroy()
         call S%destroy()
         call rhof%destroy()
+#ifdef MF
+        call vof%destroy()
+        multiphase = .false.
+#endif
         if (c_associated(fen_ctx)) rc = fen_destroy(fen_ctx)
         fen_ctx = c_null_ptr
         poisson_shared_ctx = c_null_ptr
@@ -952,6 +1013,131 @@ contains
     end subroutine destroy_navier_stokes_solver
 
 end module navier_stokes_mod
+
+#ifdef MF
+!===================================================================================================
+module volume_of_fluid_mod
+    ! MTHINC volume of fluid (reference src/volume_of_fluid.f90).  The vof
+    ! field is navier_stokes_mod's resident one; reconstruction and advection
+    ! run on the GPU inside the step, or alone through advect_vof.
+    use precision_mod, only : dp
+    use scalar_mod
+    use vector_mod
+    use navier_stokes_mod, only : vof
+    implicit none
+    private
+    public :: vof, beta, distance, distance_from_interface, get_vof_from_distance, advect_vof
+
+    !> Interface sharpness of the MTHINC profile (src/volume_of_fluid.f90:24).
+    real(dp) :: beta = 1.0_dp
+
+    abstract interface
+        function distance_from_interface(x, y) result(dist)
+            use precision_mod, only : dp
+            real(dp), intent(in) :: x, y
+            real(dp) :: dist
+        end function distance_from_interface
+    end interface
+
+    !> Signed distance from the interface, set by the user program.
+    procedure(distance_from_interface), pointer :: distance => Null()
+
+contains
+
+    !> vof = cell average of 0.5 (1 + tanh(beta d / delta)) by the 2 x 2 Gauss
+    !> rule, on the host mirror, then the ghost nodes
+    !> (src/volume_of_fluid.f90:665-706).
+    subroutine get_vof_from_distance()
+        use IO_mod, only : print_error_message
+        use navier_stokes_mod, only : fen_host_modified
+        ! Gauss points at 0.5 (1 -+ 1/sqrt(3)) of the cell, averaged pairwise in x, then in y
+        real(dp), parameter :: gm = 0.5_dp*(1.0_dp - 1.0_dp/sqrt(3.0_dp)) - 0.5_dp
+        real(dp), parameter :: gq = 0.5_dp*(1.0_dp + 1.0_dp/sqrt(3.0_dp)) - 0.5_dp
+        real(dp) :: dl, xm, xp, ym, yp
+        integer :: i, j, k
+        if (.not. associated(distance)) then
+            call print_error_message('ERROR: distance function not defined.')
+            return
+        end if
+        call fen_host_modified()
+        dl = vof%G%delta
+        do k = vof%G%lo(3), vof%G%hi(3)
+            do j = vof%G%lo(2), vof%G%hi(2)
+                ym = vof%G%y(j) + dl*gm
+                yp = vof%G%y(j) + dl*gq
+                do i = vof%G%lo(1), vof%G%hi(1)
+                    xm = vof%G%x(i) + dl*gm
+                    xp = vof%G%x(i) + dl*gq
+                    vof%f(i,j,k) = 0.5_dp*(0.5_dp*(prof(xm, ym) + prof(xp, ym)) + 0.5_dp*(prof(xm, yp) + prof(xp, yp)))
+                end do
+            end do
+        end do
+        call vof%update_ghost_nodes()
+    contains
+        real(dp) function prof(x, y)
+            real(dp), intent(in) :: x, y
+            prof = 0.5_dp*(1.0_dp + tanh(beta*distance(x, y)/dl))
+        end function prof
+    end subroutine get_vof_from_distance
+
+    !> Advects the vof with the resident velocity over dt
+    !> (src/volume_of_fluid.f90:420-544); v must be navier_stokes_mod's v.
+    subroutine advect_vof(v, dt)
+        use fen_capi
+        use navier_stokes_mod, only : fen_ctx, fen_push_host, fen_device_modified
+        type(vector), intent(in) :: v
+        real(dp), intent(in) :: dt
+        if (.not. associated(v%G)) return
+        call fen_push_host()
+        call fen_check(fen_ctx, fen_advect_vof(fen_ctx, dt), 'advect_vof')
+        call fen_device_modified()
+    end subroutine advect_vof
+
+end module volume_of_fluid_mod
+
+!===================================================================================================
+module multiphase_mod
+    ! Two-fluid material properties (reference src/multiphase.f90).
+    use precision_mod, only : dp
+    use scalar_mod
+    use vector_mod
+    use navier_stokes_mod, only : rho, mu
+    implicit none
+
+    !> Densities and viscosities of phase 0 (vof = 0) and phase 1 (vof = 1).
+    real(dp) :: rho_0 = 1.0_dp, rho_1 = 1.0_dp, mu_0 = 1.0_dp, mu_1 = 1.0_dp
+    !> Surface tension coefficient.
+    real(dp) :: sigma = 0.0_dp
+    !> Minimum density and its inverse (Dodd & Ferrante splitting).
+    real(dp) :: rhomin, irhomin
+
+    interface
+        subroutine interface_advection(v, dt)
+            use precision_mod, only : dp
+            use vector_mod
+            real(dp), intent(in) :: dt
+            type(vector), intent(in) :: v
+        end subroutine interface_advection
+    end interface
+
+    procedure(interface_advection), pointer :: advect_interface => Null()
+
+contains
+
+    !> rho and mu from the vof, on the GPU (src/multiphase.f90:126-140).
+    subroutine update_material_properties(rho, mu, vof)
+        use fen_capi
+        use navier_stokes_mod, only : fen_ctx, fen_push_host, fen_device_modified
+        type(scalar), intent(inout) :: rho, mu
+        type(scalar), intent(in) :: vof
+        if (rho%fid /= FEN_RHO .or. mu%fid /= FEN_MU .or. vof%fid /= FEN_VOF) return
+        call fen_push_host()
+        call fen_check(fen_ctx, fen_update_material_properties(fen_ctx), 'update_material_properties')
+        call fen_device_modified()
+    end subroutine update_material_properties
+
+end module multiphase_mod
+#endif
 
 !===================================================================================================
 module solver_mod
@@ -980,15 +1166,35 @@ module solver_mod
 
 contains
 
+    !> Allocates the fields and the GPU context; in the two-phase build also
+    !> the vof from the user's distance function and rho, mu from it
+    !> (src/solver.f90:32-97).
     subroutine init_solver(comp_grid)
         use poisson_mod, only : init_poisson_solver
         use navier_stokes_mod, only : allocate_navier_stokes_fields, phi, navier_stokes_solver, &
                                       print_navier_stokes_solver_status
+#ifdef MF
+        use navier_stokes_mod, only : constant_viscosity, rho, mu, enable_multiphase, fen_host_modified
+        use multiphase_mod, only : rho_0, rho_1, mu_0, mu_1, sigma, rhomin, irhomin, &
+                                   update_material_properties, advect_interface
+        use volume_of_fluid_mod, only : vof, beta, get_vof_from_distance, advect_vof
+#endif
         type(grid), intent(in), target :: comp_grid
         call allocate_navier_stokes_fields(comp_grid)
         call init_Poisson_Solver(phi)
         advance_solution => navier_stokes_solver
         print_solver_status => print_navier_stokes_solver_status
+#ifdef MF
+        constant_viscosity = .false.
+        call enable_multiphase(rho_0, rho_1, mu_0, mu_1, sigma, beta)
+        call get_vof_from_distance
+        call update_material_properties(rho, mu, vof)
+        ! refresh the mirrors; fields the program sets after init_solver are uploaded by the first step
+        call fen_host_modified()
+        rhomin = min(rho_0, rho_1)
+        irhomin = 1.0_dp/rhomin
+        advect_interface => advect_vof
+#endif
     end subroutine init_solver
 
     !> Cell-centred fields data/vx_<step>.raw, vy_, [vz_,] p_ (raw float64
@@ -997,6 +1203,9 @@ contains
         use scalar_mod, only : scalar
         use fields_mod, only : face_to_center
         use navier_stokes_mod, only : v, p, fen_sync_host
+#ifdef MF
+        use navier_stokes_mod, only : vof
+#endif
         integer, intent(in) :: step
         type(scalar) :: temp
         character(len=7) :: sn
@@ -1012,6 +1221,9 @@ contains
         call temp%write('data/vz_'//sn//'.raw')
 #endif
         call p%write('data/p_'//sn//'.raw')
+#ifdef MF
+        call vof%write('data/vof_'//sn//'.raw')
+#endif
         call temp%destroy()
     end subroutine save_fields
 
