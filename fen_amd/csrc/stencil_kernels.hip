@@ -40,11 +40,11 @@ template <class T> static bool pow2_value(T r)
 template <class R> struct GhostLaunch {
     GhostDesc<R> d[4];
     Box b;
-    long nx_cells, ny_cells, nz_cells;   // cells per plane of each group
-    int ngx[4], ngy[4], ngz[4];           // planes per group and field (0 when skipped)
+    int nx_cells, ny_cells, nz_cells;   // cells per plane of each group
+    int ngx[4], ngy[4], ngz[4];         // planes per group and field (0 when skipped)
 };
 
-template <class R> __device__ R ghost_s1(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
+template <class R> __device__ __forceinline__ R ghost_s1(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
 {
     const R two = R(2.0);
     const long pj = j + (long)(b.ny + 2) * k;
@@ -72,7 +72,7 @@ template <class R> __device__ R ghost_s1(const GhostDesc<R> &d, const Box &b, in
     return d.f[b.g(i, j, k)];
 }
 
-template <class R> __device__ R ghost_s2(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
+template <class R> __device__ __forceinline__ R ghost_s2(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
 {
     const R two = R(2.0);
     const long pi = i + (long)(b.nx + 2) * k;
@@ -100,7 +100,7 @@ template <class R> __device__ R ghost_s2(const GhostDesc<R> &d, const Box &b, in
     return ghost_s1(d, b, i, j, k);
 }
 
-template <class R> __device__ R ghost_s3(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
+template <class R> __device__ __forceinline__ R ghost_s3(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
 {
     const R two = R(2.0);
     if (b.dim == 3) {
@@ -133,39 +133,52 @@ template <class R> __device__ R ghost_s3(const GhostDesc<R> &d, const Box &b, in
 // Group plane index p -> coordinate: 0 -> 0, 1 -> n+1, 2 -> n (normal overwrite).
 __device__ __forceinline__ int plane_coord(int p, int n) { return p == 0 ? 0 : (p == 1 ? n + 1 : n); }
 
-template <class R> __global__ __launch_bounds__(256) void ghost_kernel(GhostLaunch<R> a)
+// One field per blockIdx.y.  The descriptor is selected with a compile-time
+// index (Q) and the rule functions above are inlined: a run-time index into
+// the by-value argument block, or its address passed to a call that is not
+// inlined, makes the compiler copy the whole block (464 bytes) into per-lane
+// scratch memory, which cost 55 us per field at 256^3.
+template <class R, int Q> __device__ __forceinline__ void ghost_field(const GhostLaunch<R> &a)
 {
-    const int q = blockIdx.y;
-    const GhostDesc<R> &d = a.d[q];
+    const GhostDesc<R> &d = a.d[Q];
     const Box &b = a.b;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long tx = a.nx_cells * a.ngx[q], ty = a.ny_cells * a.ngy[q], tz = a.nz_cells * a.ngz[q];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tx = a.nx_cells * a.ngx[Q], ty = a.ny_cells * a.ngy[Q], tz = a.nz_cells * a.ngz[Q];
     int i, j, k;
     if (t < tx) {
         // x faces: i ghost (or the overwritten normal plane), j and k interior
-        const int p = (int)(t / a.nx_cells);
-        const long r = t % a.nx_cells;
+        const int p = t / a.nx_cells, r = t - p * a.nx_cells;
         i = plane_coord(p, b.nx);
-        j = (int)(r % b.ny) + 1;
-        k = (int)(r / b.ny) + 1;
+        k = r / b.ny;
+        j = r - k * b.ny + 1;
+        k += 1;
     } else if ((t -= tx) < ty) {
         // y faces: j ghost, every i, k interior
-        const int p = (int)(t / a.ny_cells);
-        const long r = t % a.ny_cells;
+        const int p = t / a.ny_cells, r = t - p * a.ny_cells;
         j = plane_coord(p, b.ny);
-        i = (int)(r % (b.nx + 2));
-        k = (int)(r / (b.nx + 2)) + 1;
+        k = r / (b.nx + 2);
+        i = r - k * (b.nx + 2);
+        k += 1;
     } else if ((t -= ty) < tz) {
         // z faces: k ghost, every i and j
-        const int p = (int)(t / a.nz_cells);
-        const long r = t % a.nz_cells;
+        const int p = t / a.nz_cells, r = t - p * a.nz_cells;
         k = plane_coord(p, b.nz);
-        i = (int)(r % (b.nx + 2));
-        j = (int)(r / (b.nx + 2));
+        j = r / (b.nx + 2);
+        i = r - j * (b.nx + 2);
     } else {
         return;
     }
     d.f[b.g(i, j, k)] = ghost_s3(d, b, i, j, k);
+}
+
+template <class R> __global__ __launch_bounds__(256) void ghost_kernel(GhostLaunch<R> a)
+{
+    switch (blockIdx.y) {
+    case 0: ghost_field<R, 0>(a); break;
+    case 1: ghost_field<R, 1>(a); break;
+    case 2: ghost_field<R, 2>(a); break;
+    default: ghost_field<R, 3>(a); break;
+    }
 }
 
 template <class R> void launch_ghost_desc(fen_ctx *c, const GhostDesc<R> *descs, int n)
@@ -174,9 +187,9 @@ template <class R> void launch_ghost_desc(fen_ctx *c, const GhostDesc<R> *descs,
     GhostLaunch<R> a{};
     const Box &b = c->box;
     a.b = b;
-    a.nx_cells = (long)b.ny * b.nz;
-    a.ny_cells = (long)(b.nx + 2) * b.nz;
-    a.nz_cells = (long)(b.nx + 2) * (b.ny + 2);
+    a.nx_cells = b.ny * b.nz;
+    a.ny_cells = (b.nx + 2) * b.nz;
+    a.nz_cells = (b.nx + 2) * (b.ny + 2);
     long most = 0;
     double cells = 0.0;
     for (int q = 0; q < n; q++) {
@@ -185,11 +198,12 @@ template <class R> void launch_ghost_desc(fen_ctx *c, const GhostDesc<R> *descs,
         a.ngx[q] = (d.groups & 1) ? (d.type[FEN_RIGHT] == 1 && d.loc == 'x' ? 3 : 2) : 0;
         a.ngy[q] = (d.groups & 2) ? (d.type[FEN_TOP] == 1 && d.loc == 'y' ? 3 : 2) : 0;
         a.ngz[q] = (b.dim == 3 && (d.groups & 4)) ? (d.type[FEN_BACK] == 1 && d.loc == 'z' ? 3 : 2) : 0;
-        const long tot = a.nx_cells * a.ngx[q] + a.ny_cells * a.ngy[q] + a.nz_cells * a.ngz[q];
+        const long tot = (long)a.nx_cells * a.ngx[q] + (long)a.ny_cells * a.ngy[q] + (long)a.nz_cells * a.ngz[q];
         most = tot > most ? tot : most;
         cells += (double)tot;
     }
     if (most == 0) return;
+    if (most > INT32_MAX - 256) throw std::runtime_error("ghost fill: too many cells for 32-bit indices");
     const dim3 grid((unsigned)((most + 255) / 256), (unsigned)n);
     prof_begin(c, K_GHOST);
     hipLaunchKernelGGL(ghost_kernel<R>, grid, dim3(256), 0, c->stream, a);
