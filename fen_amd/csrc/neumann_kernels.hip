@@ -306,6 +306,56 @@ __global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
     }
 }
 
+// 2-D solvers (ASSOC2D form, contiguous columns): a few hundred columns of up
+// to 1024 cells, so one thread per column leaves the sweeps bound by memory
+// latency.  One wave per column stages the column, the coefficients and c1 in
+// LDS, one lane runs both recurrences there with the arithmetic of
+// thomas_general_kernel<E, 1>, and the wave writes the column back.
+// Dynamic LDS: E x[n], then double c1[n], A[n], B[n], C[n].
+template <class E> __global__ __launch_bounds__(64) void thomas_column_kernel(ThomasGArgs a)
+{
+    extern __shared__ double smem[];
+    const int n = a.n;
+    const long o = blockIdx.x;
+    E *xg = static_cast<E *>(a.x) + o * a.os;
+    E *x = reinterpret_cast<E *>(smem);
+    double *c1 = smem + (long)n * (sizeof(E) / sizeof(double));
+    double *A = c1 + n, *B = A + n, *C = B + n;
+    for (int k = threadIdx.x; k < n; k += 64) {
+        x[k] = xg[k];
+        A[k] = a.ta[k];
+        B[k] = a.tb[k];
+        C[k] = a.tc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double mx = a.mwx[o];
+        double c1p = C[0] / (B[0] + mx);
+        E d = div_(x[0], B[0] + mx);
+        c1[0] = c1p;
+        x[0] = d;
+#pragma unroll 4
+        for (int k = 1; k < n - 1; k++) {
+            const double cn = C[k] / ((B[k] - A[k] * c1p) + mx);
+            d = div_(sub_(x[k], mul_(d, A[k])), (B[k] + mx) - A[k] * c1p);
+            c1p = cn;
+            c1[k] = c1p;
+            x[k] = d;
+        }
+        const double frac = (B[n - 1] + mx) - A[n - 1] * c1p;
+        d = frac != 0.0 ? div_(sub_(x[n - 1], mul_(d, A[n - 1])), frac) : zero_<E>();
+        x[n - 1] = d;
+        E xn = d;
+#pragma unroll 4
+        for (int k = n - 2; k >= 0; k--) {
+            xn = sub_(x[k], mul_(xn, c1[k]));
+            x[k] = xn;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += 64) xg[k] = x[k];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -471,7 +521,14 @@ void launch_thomas_neumann(fen_ctx *c)
     if (a.n < 2) throw std::runtime_error("the tridiagonal direction needs at least 2 cells");
     const dim3 grid((unsigned)((a.ncol + kNB - 1) / kNB));
     prof_begin(c, K_THOMAS);
-    if (complex_e && assoc2d) hipLaunchKernelGGL((thomas_general_kernel<double2, 1>), grid, dim3(kNB), 0, c->stream, a);
+    const size_t lds = (size_t)a.n * ((complex_e ? sizeof(double2) : sizeof(double)) + 4 * sizeof(double));
+    if (assoc2d && a.s == 1 && a.os == a.n && lds <= 65536) {
+        // contiguous columns that fit in LDS: one wave each
+        if (complex_e)
+            hipLaunchKernelGGL(thomas_column_kernel<double2>, dim3((unsigned)a.ncol), dim3(64), lds, c->stream, a);
+        else
+            hipLaunchKernelGGL(thomas_column_kernel<double>, dim3((unsigned)a.ncol), dim3(64), lds, c->stream, a);
+    } else if (complex_e && assoc2d) hipLaunchKernelGGL((thomas_general_kernel<double2, 1>), grid, dim3(kNB), 0, c->stream, a);
     else if (complex_e) hipLaunchKernelGGL((thomas_general_kernel<double2, 0>), grid, dim3(kNB), 0, c->stream, a);
     else if (assoc2d) hipLaunchKernelGGL((thomas_general_kernel<double, 1>), grid, dim3(kNB), 0, c->stream, a);
     else hipLaunchKernelGGL((thomas_general_kernel<double, 0>), grid, dim3(kNB), 0, c->stream, a);
