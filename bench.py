@@ -178,6 +178,43 @@ def multiphase_rate(steps=100):
             "steps": steps, "maxdiv": md}
 
 
+def adjoint_rate(N=128, steps=200, every=20):
+    """The discrete adjoint on the 3-D periodic 128^3 Re 400 case: 200
+    forward steps recorded with checkpoints every 20 (fen_adjoint_record),
+    then the backward sweep with recomputation (fen_adjoint_backward, seed
+    upload and gradient download included), beside the plain forward run."""
+    import numpy as np
+    from fen_amd import Solver, cases
+    from fen_amd._lib import U, V, W
+    from tests.adjoint_harness import final_velocity, load_state
+    case = cases.tgv3d(N, Re=400.0, steps=steps)
+    s = Solver(3, case.n, case.L)
+    s.set_params(density=1.0, viscosity=case.viscosity, cfl=0.5)
+    dt = s.set_timestep(2.0)
+    state = dict(case.fields)
+
+    def timed(fn):
+        s.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        s.synchronize()
+        return 1e3 * (time.perf_counter() - t0), out
+
+    load_state(s, state, dt)
+    s.run(0, 10, dt)                      # warm-up
+    load_state(s, state, dt)
+    t_fwd, _ = timed(lambda: s.run(0, steps, dt))
+    load_state(s, state, dt)
+    t_rec, _ = timed(lambda: s.adjoint_record(0, steps, every, dt))
+    seed = [np.asfortranarray(a) for a in final_velocity(s)]
+    t_bwd, grads = timed(lambda: s.adjoint_backward(seed))
+    s.close()
+    assert all(np.isfinite(g).all() for g in grads[:3])
+    return {"workload": f"adjoint of tgv3d {N}^3 Re 400, {steps} steps, checkpoints every {every} (FP64)",
+            "forward_ms_per_step": t_fwd / steps, "record_ms_per_step": t_rec / steps,
+            "backward_ms_per_step": t_bwd / steps, "gradient_over_forward": (t_rec + t_bwd) / t_fwd}
+
+
 def neumann_rates(steps=20):
     """FP64 time per step with the Neumann-direction solvers at 256^3 (and
     256^2 in 2-D): lid-driven cavity (nnn), duct (npn), 2-D cavity (nn) and
@@ -352,6 +389,7 @@ def main():
         extra["tgv2d_128"] = tgv2d_time()
         extra["neumann_solvers"] = neumann_rates(20)
         extra["multiphase_2d"] = multiphase_rate(100)
+        extra["adjoint_128"] = adjoint_rate()
 
     cpu = None
     if not args.no_cpu_baseline:
