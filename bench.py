@@ -231,6 +231,25 @@ def neumann_rates(steps=20):
     return out
 
 
+def _case(name, steps=20, precision=0):
+    from fen_amd import cases
+    make = {"cavity3d": lambda: cases.cavity3d(256, 256, 256, steps=steps),
+            "duct3d": lambda: cases.duct3d(256, 256, 256, steps=steps),
+            "channel": lambda: cases.channel(steps=steps)}[name]
+    return case_rate(make(), steps, precision)
+
+
+# --case targets: single configurations for profiling runs
+CASES = {
+    "channel": lambda: _case("channel"),
+    "channel_fp32": lambda: _case("channel", precision=1),
+    "cavity3d_nnn": lambda: _case("cavity3d"),
+    "duct3d_npn": lambda: _case("duct3d"),
+    "rayleigh_taylor": lambda: multiphase_rate(100),
+    "adjoint_128": lambda: adjoint_rate(),
+}
+
+
 def measure(N, steps, warmup, precision):
     """Warm-up, one untimed calibration of the per-kernel breakdown, then the
     timed steps.  Returns (solver, result dict)."""
@@ -337,6 +356,8 @@ def main():
                          "the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE calibration")
     ap.add_argument("--fp32-accuracy", type=int, nargs=2, metavar=("N", "STEPS"),
                     help="only report the FP32+refine errors against FP64 at N^3 after STEPS steps")
+    ap.add_argument("--case", choices=sorted(CASES),
+                    help="only time one of the other configurations (the profiling target for rocprofv3)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -346,6 +367,9 @@ def main():
 
     if args.fp32_accuracy:
         print(json.dumps(fp32_accuracy(*args.fp32_accuracy)), flush=True)
+        return
+    if args.case:
+        print(json.dumps({args.case: CASES[args.case]()}), flush=True)
         return
 
     from fen_amd._lib import FP32_REFINE

@@ -694,6 +694,10 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
             if (y_transformed) c->d_twy = dev_copy(twiddles<R>(ny));
             if (c->solver == FEN_SOLVER_PPP) c->d_twz = dev_copy(twiddles<R>(nz));
         });
+        if (c->d_c1) {
+            fen::launch_thomas_c1(c);
+            FEN_HIP_CHECK(hipGetLastError());
+        }
         c->n_part = std::max<size_t>(std::max<size_t>(3 * 1024, (size_t)ny * nz / 2 + 1),
                                      3 * (size_t)fen::stencil_blocks(c->box));
         c->d_part = dev_alloc<double>(c->n_part);

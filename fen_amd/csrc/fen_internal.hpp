@@ -201,7 +201,7 @@ struct fen_ctx {
     // Poisson solver state
     int hx = 0;                  // nx/2 + 1
     void *d_spec = nullptr;      // complex R, [kx][k][j]
-    double *d_c1 = nullptr;      // Thomas scratch (ppn), [kx][k][j]
+    double *d_c1 = nullptr;      // Thomas c1, computed at create (layout of the sweep's columns)
     double2 *d_dd = nullptr;     // FP64 Thomas right-hand side (ppn, FP32 form), [kx][k][j]
     double2 *d_spec2 = nullptr;  // npn: y half spectrum of the real x-DCT lines, [kx][k][jy]
     double2 *d_mkx = nullptr, *d_mky = nullptr;   // (cos, sin)(pi k/(2n)), k < n: DCTs along x, y
@@ -284,6 +284,7 @@ void launch_dct3_x(fen_ctx *c, double *phi, int gl, long sx, long sy, double pos
 void launch_dct_y(fen_ctx *c, int inverse, double post);
 void launch_rfft_y(fen_ctx *c, int inverse, double post);
 void launch_thomas_neumann(fen_ctx *c);
+void launch_thomas_c1(fen_ctx *c);
 template <class R> void launch_apply_shift(fen_ctx *c, R *f, size_t n);
 void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz);   // FP64 contexts
 

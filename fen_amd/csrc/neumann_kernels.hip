@@ -234,7 +234,11 @@ __global__ __launch_bounds__(kNB) void irfft_rows_kernel(RfftArgs a)
 // ASSOC2D: the 2-D solvers' form (src/poisson.f90:346-384): divisions, c1
 // denominator (b - a c1) + mx, d denominator (b + mx) - a c1.  Otherwise
 // the 3-D form (:1094-1135, 1236-1280): reciprocal pivots, last row divides.
-// c1 lives in an FP64 scratch array laid out like the column elements.
+// c1 lives in an FP64 array laid out like the column elements.  It depends
+// on the coefficients and the column's wavenumbers only, so it is computed
+// once when the context is created (thomas_c1_kernel, the same expressions);
+// the sweeps carry their own c1 recurrence in registers for the pivots and
+// read the stored c1 in the back substitution.
 // ---------------------------------------------------------------------------
 struct ThomasGArgs {
     void *x;
@@ -271,13 +275,11 @@ __global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
     if constexpr (ASSOC2D) {
         c1p = C[0] / (B[0] + mx);
         d = div_(x[0], B[0] + mx);
-        c1[0] = c1p;
         x[0] = d;
         for (int k = 1; k < n - 1; k++) {
             const double cn = C[k] / ((B[k] - A[k] * c1p) + mx);
             d = div_(sub_(x[k * s], mul_(d, A[k])), (B[k] + mx) - A[k] * c1p);
             c1p = cn;
-            c1[k * s] = c1p;
             x[k * s] = d;
         }
         const double frac = (B[n - 1] + mx) - A[n - 1] * c1p;
@@ -285,13 +287,11 @@ __global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
     } else {
         double factor = 1.0 / ((B[0] + mx) + my);
         c1p = C[0] * factor;
-        c1[0] = c1p;
         d = mul_(x[0], factor);
         x[0] = d;
         for (int k = 1; k < n - 1; k++) {
             factor = 1.0 / (((B[k] + mx) + my) - A[k] * c1p);
             c1p = C[k] * factor;
-            c1[k * s] = c1p;
             d = mul_(sub_(x[k * s], mul_(d, A[k])), factor);
             x[k * s] = d;
         }
@@ -303,6 +303,34 @@ __global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
     for (int k = n - 2; k >= 0; k--) {
         xn = sub_(x[k * s], mul_(xn, c1[k * s]));
         x[k * s] = xn;
+    }
+}
+
+template <int ASSOC2D> __global__ __launch_bounds__(kNB) void thomas_c1_kernel(ThomasGArgs a)
+{
+    const long t = (long)blockIdx.x * kNB + threadIdx.x;
+    if (t >= a.ncol) return;
+    const long o = t / a.ni, q = t % a.ni, s = a.s;
+    double *c1 = a.c1 + o * a.os + q;
+    const double mx = a.mwx[o], my = ASSOC2D ? 0.0 : a.mwy[q];
+    const double *A = a.ta, *B = a.tb, *C = a.tc;
+    double c1p;
+    if constexpr (ASSOC2D) {
+        c1p = C[0] / (B[0] + mx);
+        c1[0] = c1p;
+        for (int k = 1; k < a.n - 1; k++) {
+            c1p = C[k] / ((B[k] - A[k] * c1p) + mx);
+            c1[k * s] = c1p;
+        }
+    } else {
+        double factor = 1.0 / ((B[0] + mx) + my);
+        c1p = C[0] * factor;
+        c1[0] = c1p;
+        for (int k = 1; k < a.n - 1; k++) {
+            factor = 1.0 / (((B[k] + mx) + my) - A[k] * c1p);
+            c1p = C[k] * factor;
+            c1[k * s] = c1p;
+        }
     }
 }
 
@@ -487,8 +515,9 @@ void launch_rfft_y(fen_ctx *c, int inverse, double post)
     prof_end(c, kid, 16.0 * c->n_int);
 }
 
-// Thomas sweeps of the pn, nn, npn and nnn solvers.
-void launch_thomas_neumann(fen_ctx *c)
+// Column layout of the pn, nn, npn and nnn sweeps (and of ppn's, whose
+// sweep kernel is thomas_kernel): element type and association form.
+static ThomasGArgs thomas_layout(fen_ctx *c, bool *complex_out, bool *assoc2d_out)
 {
     ThomasGArgs a{};
     a.c1 = c->d_c1;
@@ -516,9 +545,34 @@ void launch_thomas_neumann(fen_ctx *c)
         a.x = c->d_spec;
         a.ncol = (long)c->nx * c->ny; a.ni = c->ny; a.os = (long)c->nz * c->ny; a.s = c->ny; a.n = c->nz;
         break;
-    default: throw std::runtime_error("not a Neumann solver");
+    case FEN_SOLVER_PPN:  // complex [kx][k][j] (thomas_kernel)
+        a.x = c->d_spec; complex_e = true;
+        a.ncol = (long)c->hx * c->ny; a.ni = c->ny; a.os = (long)c->ny * c->nz; a.s = c->ny; a.n = c->nz;
+        break;
+    default: throw std::runtime_error("not a tridiagonal solver");
     }
     if (a.n < 2) throw std::runtime_error("the tridiagonal direction needs at least 2 cells");
+    *complex_out = complex_e;
+    *assoc2d_out = assoc2d;
+    return a;
+}
+
+// c1 of every column, once per context (after the coefficients are uploaded).
+void launch_thomas_c1(fen_ctx *c)
+{
+    bool complex_e = false, assoc2d = false;
+    const ThomasGArgs a = thomas_layout(c, &complex_e, &assoc2d);
+    const dim3 grid((unsigned)((a.ncol + kNB - 1) / kNB));
+    if (assoc2d) hipLaunchKernelGGL(thomas_c1_kernel<1>, grid, dim3(kNB), 0, c->stream, a);
+    else hipLaunchKernelGGL(thomas_c1_kernel<0>, grid, dim3(kNB), 0, c->stream, a);
+}
+
+// Thomas sweeps of the pn, nn, npn and nnn solvers.
+void launch_thomas_neumann(fen_ctx *c)
+{
+    bool complex_e = false, assoc2d = false;
+    if (c->solver == FEN_SOLVER_PPN) throw std::runtime_error("ppn sweeps use thomas_kernel");
+    ThomasGArgs a = thomas_layout(c, &complex_e, &assoc2d);
     const dim3 grid((unsigned)((a.ncol + kNB - 1) / kNB));
     prof_begin(c, K_THOMAS);
     const size_t lds = (size_t)a.n * ((complex_e ? sizeof(double2) : sizeof(double)) + 4 * sizeof(double));
@@ -532,7 +586,9 @@ void launch_thomas_neumann(fen_ctx *c)
     else if (complex_e) hipLaunchKernelGGL((thomas_general_kernel<double2, 0>), grid, dim3(kNB), 0, c->stream, a);
     else if (assoc2d) hipLaunchKernelGGL((thomas_general_kernel<double, 1>), grid, dim3(kNB), 0, c->stream, a);
     else hipLaunchKernelGGL((thomas_general_kernel<double, 0>), grid, dim3(kNB), 0, c->stream, a);
-    prof_end(c, K_THOMAS, 32.0 * (double)a.ncol * a.n);
+    // columns read and written in each sweep, c1 read in the back substitution (LDS kernel: once each)
+    const double eb = complex_e ? 16.0 : 8.0, cells = (double)a.ncol * a.n;
+    prof_end(c, K_THOMAS, (assoc2d && a.s == 1 && lds <= 65536 ? 2.0 * eb : 4.0 * eb + 8.0) * cells);
 }
 
 }  // namespace fen

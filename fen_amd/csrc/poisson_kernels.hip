@@ -334,8 +334,10 @@ __global__ __launch_bounds__(NT) void z_solve_kernel(ZArgs<R> a)
 // K4 (ppn): Thomas sweep along z for every (kx, j) column
 // (src/poisson.f90:1094-1135); one thread per column, adjacent threads own
 // adjacent j so every k-row access is coalesced.  The sweep is FP64 for
-// either element type: c1 lives in an FP64 scratch array of the spectrum's
-// layout; d is kept in place in the spectrum when R is double, and in an
+// either element type: c1 lives in an FP64 array of the spectrum's layout,
+// computed once per context (launch_thomas_c1: it does not depend on the
+// right-hand side), so the forward sweep recomputes its pivots in registers
+// and only the back substitution reads it; d is kept in place in the spectrum when R is double, and in an
 // FP64 scratch `dd` when R is float (the back substitution of a nearly
 // singular column would amplify a rounded d by up to ~nz^2).
 // ---------------------------------------------------------------------------
@@ -358,7 +360,7 @@ template <class R> __global__ __launch_bounds__(kBlock) void thomas_kernel(Thoma
     const long base = kx * a.kxs + j;
     const long s = a.ny;
     C *x = a.spec + base;
-    double *c1 = a.c1 + base;
+    const double *c1 = a.c1 + base;
     double2 *dd = IN_PLACE ? nullptr : a.dd + base;
     auto rhs = [&](long o) { const C v = x[o]; return double2{(double)v.x, (double)v.y}; };
     auto put_d = [&](long o, double2 v) {
@@ -373,7 +375,6 @@ template <class R> __global__ __launch_bounds__(kBlock) void thomas_kernel(Thoma
     const int n = a.nz;
     double factor = 1.0 / ((a.tb[0] + mx) + my);
     double c1p = a.tc[0] * factor;
-    c1[0] = c1p;
     double2 d = rhs(0);
     d.x = d.x * factor;
     d.y = d.y * factor;
@@ -381,7 +382,6 @@ template <class R> __global__ __launch_bounds__(kBlock) void thomas_kernel(Thoma
     for (int k = 1; k < n - 1; k++) {
         factor = 1.0 / (((a.tb[k] + mx) + my) - a.ta[k] * c1p);
         c1p = a.tc[k] * factor;
-        c1[k * s] = c1p;
         const double2 r = rhs(k * s);
         d.x = (r.x - a.ta[k] * d.x) * factor;
         d.y = (r.y - a.ta[k] * d.y) * factor;
@@ -576,9 +576,9 @@ template <class R> void launch_thomas(fen_ctx *c)
     prof_begin(c, K_THOMAS);
     hipLaunchKernelGGL(thomas_kernel<R>, dim3((unsigned)((cols + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
                        a);
-    // spectrum read + written, c1 written + read (FP64), and for R = float d written + read (FP64 complex)
+    // spectrum read + written, c1 read (FP64), d written + read (in place for double, FP64 scratch for float)
     const double n_el = (double)c->hx * c->ny * c->nz;
-    prof_end(c, K_THOMAS, 2.0 * spec_bytes<R>(c) + 16.0 * n_el + (sizeof(R) == sizeof(double) ? 0.0 : 32.0 * n_el));
+    prof_end(c, K_THOMAS, 2.0 * spec_bytes<R>(c) + 8.0 * n_el + 32.0 * n_el);
 }
 
 template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img)
