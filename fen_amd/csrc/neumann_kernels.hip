@@ -258,6 +258,8 @@ template <class E> __device__ __forceinline__ E zero_();
 template <> __device__ __forceinline__ double zero_<double>() { return 0.0; }
 template <> __device__ __forceinline__ double2 zero_<double2>() { return double2{0.0, 0.0}; }
 
+constexpr int kGU = 8;   // rows loaded ahead per batch in the strided sweeps
+
 template <class E, int ASSOC2D>
 __global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
 {
@@ -272,37 +274,66 @@ __global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
     const double *A = a.ta, *B = a.tb, *C = a.tc;
     E d;
     double c1p;
+    // one forward step on row k with right-hand side r
+    auto fwd = [&](int k, E r) {
+        if constexpr (ASSOC2D) {
+            const double cn = C[k] / ((B[k] - A[k] * c1p) + mx);
+            d = div_(sub_(r, mul_(d, A[k])), (B[k] + mx) - A[k] * c1p);
+            c1p = cn;
+        } else {
+            const double factor = 1.0 / (((B[k] + mx) + my) - A[k] * c1p);
+            c1p = C[k] * factor;
+            d = mul_(sub_(r, mul_(d, A[k])), factor);
+        }
+        x[k * s] = d;
+    };
     if constexpr (ASSOC2D) {
         c1p = C[0] / (B[0] + mx);
         d = div_(x[0], B[0] + mx);
-        x[0] = d;
-        for (int k = 1; k < n - 1; k++) {
-            const double cn = C[k] / ((B[k] - A[k] * c1p) + mx);
-            d = div_(sub_(x[k * s], mul_(d, A[k])), (B[k] + mx) - A[k] * c1p);
-            c1p = cn;
-            x[k * s] = d;
-        }
+    } else {
+        const double factor = 1.0 / ((B[0] + mx) + my);
+        c1p = C[0] * factor;
+        d = mul_(x[0], factor);
+    }
+    x[0] = d;
+    // the loads of kGU rows are issued together ahead of their steps, so
+    // kGU memory latencies overlap instead of one per step
+    int k = 1;
+    for (; k + kGU <= n - 1; k += kGU) {
+        E r[kGU];
+#pragma unroll
+        for (int u = 0; u < kGU; u++) r[u] = x[(long)(k + u) * s];
+#pragma unroll
+        for (int u = 0; u < kGU; u++) fwd(k + u, r[u]);
+    }
+    for (; k < n - 1; k++) fwd(k, x[(long)k * s]);
+    if constexpr (ASSOC2D) {
         const double frac = (B[n - 1] + mx) - A[n - 1] * c1p;
         d = frac != 0.0 ? div_(sub_(x[(n - 1) * s], mul_(d, A[n - 1])), frac) : zero_<E>();
     } else {
-        double factor = 1.0 / ((B[0] + mx) + my);
-        c1p = C[0] * factor;
-        d = mul_(x[0], factor);
-        x[0] = d;
-        for (int k = 1; k < n - 1; k++) {
-            factor = 1.0 / (((B[k] + mx) + my) - A[k] * c1p);
-            c1p = C[k] * factor;
-            d = mul_(sub_(x[k * s], mul_(d, A[k])), factor);
-            x[k * s] = d;
-        }
-        factor = ((B[n - 1] + mx) + my) - A[n - 1] * c1p;
+        const double factor = ((B[n - 1] + mx) + my) - A[n - 1] * c1p;
         d = factor != 0.0 ? div_(sub_(x[(n - 1) * s], mul_(d, A[n - 1])), factor) : zero_<E>();
     }
     x[(n - 1) * s] = d;
     E xn = d;
-    for (int k = n - 2; k >= 0; k--) {
-        xn = sub_(x[k * s], mul_(xn, c1[k * s]));
-        x[k * s] = xn;
+    k = n - 2;
+    for (; k - kGU + 1 >= 0; k -= kGU) {
+        E dk[kGU];
+        double ck[kGU];
+#pragma unroll
+        for (int u = 0; u < kGU; u++) {
+            dk[u] = x[(long)(k - u) * s];
+            ck[u] = c1[(long)(k - u) * s];
+        }
+#pragma unroll
+        for (int u = 0; u < kGU; u++) {
+            xn = sub_(dk[u], mul_(xn, ck[u]));
+            x[(long)(k - u) * s] = xn;
+        }
+    }
+    for (; k >= 0; k--) {
+        xn = sub_(x[(long)k * s], mul_(xn, c1[(long)k * s]));
+        x[(long)k * s] = xn;
     }
 }
 

@@ -350,6 +350,8 @@ template <class R> struct ThomasArgs {
     const double *mwx, *mwy, *ta, *tb, *tc;
 };
 
+constexpr int kTU = 8;   // Thomas rows loaded ahead per batch
+
 template <class R> __global__ __launch_bounds__(kBlock) void thomas_kernel(ThomasArgs<R> a)
 {
     using C = typename Cplx<R>::type;
@@ -379,14 +381,24 @@ template <class R> __global__ __launch_bounds__(kBlock) void thomas_kernel(Thoma
     d.x = d.x * factor;
     d.y = d.y * factor;
     put_d(0, d);
-    for (int k = 1; k < n - 1; k++) {
+    auto fwd = [&](int k, double2 r) {
         factor = 1.0 / (((a.tb[k] + mx) + my) - a.ta[k] * c1p);
         c1p = a.tc[k] * factor;
-        const double2 r = rhs(k * s);
         d.x = (r.x - a.ta[k] * d.x) * factor;
         d.y = (r.y - a.ta[k] * d.y) * factor;
         put_d(k * s, d);
+    };
+    // the loads of TU rows are issued together ahead of their sweep steps,
+    // so TU memory latencies overlap instead of one per step
+    int k = 1;
+    for (; k + kTU <= n - 1; k += kTU) {
+        double2 r[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) r[u] = rhs((long)(k + u) * s);
+#pragma unroll
+        for (int u = 0; u < kTU; u++) fwd(k + u, r[u]);
     }
+    for (; k < n - 1; k++) fwd(k, rhs((long)k * s));
     factor = ((a.tb[n - 1] + mx) + my) - a.ta[n - 1] * c1p;
     const double2 r = rhs((n - 1) * s);
     if (factor != 0.0) {
@@ -397,13 +409,24 @@ template <class R> __global__ __launch_bounds__(kBlock) void thomas_kernel(Thoma
     }
     x[(n - 1) * s] = C{(R)d.x, (R)d.y};
     double2 xn = d;
-    for (int k = n - 2; k >= 0; k--) {
-        const double2 dk = get_d(k * s);
-        const double ck = c1[k * s];
+    auto bwd = [&](int k, double2 dk, double ck) {
         xn.x = dk.x - ck * xn.x;
         xn.y = dk.y - ck * xn.y;
         x[k * s] = C{(R)xn.x, (R)xn.y};
+    };
+    k = n - 2;
+    for (; k - kTU + 1 >= 0; k -= kTU) {
+        double2 dk[kTU];
+        double ck[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            dk[u] = get_d((long)(k - u) * s);
+            ck[u] = c1[(long)(k - u) * s];
+        }
+#pragma unroll
+        for (int u = 0; u < kTU; u++) bwd(k - u, dk[u], ck[u]);
     }
+    for (; k >= 0; k--) bwd(k, get_d((long)k * s), c1[(long)k * s]);
 }
 
 // Sum of per-block partials in a fixed order, then shift = sum / N.
