@@ -367,52 +367,60 @@ template <int ASSOC2D> __global__ __launch_bounds__(kNB) void thomas_c1_kernel(T
 
 // 2-D solvers (ASSOC2D form, contiguous columns): a few hundred columns of up
 // to 1024 cells, so one thread per column leaves the sweeps bound by memory
-// latency.  One wave per column stages the column, the coefficients and c1 in
-// LDS, one lane runs both recurrences there with the arithmetic of
-// thomas_general_kernel<E, 1>, and the wave writes the column back.
-// Dynamic LDS: E x[n], then double c1[n], A[n], B[n], C[n].
+// latency, and the dependent chain of each step bounds any layout.  One wave
+// per column stages the column, its stored c1 (thomas_c1_kernel) and the
+// coefficients in LDS.  With c1 known the pivot (b_k + mx) - a_k c1_{k-1} is
+// off the chain, which keeps one multiply, one subtract and one division per
+// step; the real and imaginary parts of a complex column are independent
+// chains, run by lanes 0 and 1.  Each lane reads its operands one step ahead
+// of the step's store.  Same arithmetic as thomas_general_kernel<E, 1>.
+// Dynamic LDS: E x[n], then double c1[n], A[n], B[n].
 template <class E> __global__ __launch_bounds__(64) void thomas_column_kernel(ThomasGArgs a)
 {
     extern __shared__ double smem[];
+    constexpr int NC = sizeof(E) / sizeof(double);   // components per element
     const int n = a.n;
     const long o = blockIdx.x;
-    E *xg = static_cast<E *>(a.x) + o * a.os;
-    E *x = reinterpret_cast<E *>(smem);
-    double *c1 = smem + (long)n * (sizeof(E) / sizeof(double));
-    double *A = c1 + n, *B = A + n, *C = B + n;
+    const E *xg = static_cast<const E *>(a.x) + o * a.os;
+    const double *c1g = a.c1 + o * a.os;
+    double *x = smem;                 // element k, component l at x[NC * k + l]
+    double *c1 = smem + (long)n * NC;
+    double *A = c1 + n, *B = A + n;
     for (int k = threadIdx.x; k < n; k += 64) {
-        x[k] = xg[k];
+        reinterpret_cast<E *>(x)[k] = xg[k];
+        c1[k] = c1g[k];
         A[k] = a.ta[k];
         B[k] = a.tb[k];
-        C[k] = a.tc[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    const int l = threadIdx.x;
+    if (l < NC) {
         const double mx = a.mwx[o];
-        double c1p = C[0] / (B[0] + mx);
-        E d = div_(x[0], B[0] + mx);
-        c1[0] = c1p;
-        x[0] = d;
-#pragma unroll 4
+        double d = x[l] / (B[0] + mx);
+        x[l] = d;
+        // operands of row k held one row ahead
+        double Ak = A[1], Bk = B[1], cp = c1[0], xk = x[NC + l];
         for (int k = 1; k < n - 1; k++) {
-            const double cn = C[k] / ((B[k] - A[k] * c1p) + mx);
-            d = div_(sub_(x[k], mul_(d, A[k])), (B[k] + mx) - A[k] * c1p);
-            c1p = cn;
-            c1[k] = c1p;
-            x[k] = d;
+            const double An = A[k + 1], Bn = B[k + 1], cn = c1[k], xn = x[NC * (k + 1) + l];
+            d = (xk - d * Ak) / ((Bk + mx) - Ak * cp);
+            x[NC * k + l] = d;
+            Ak = An; Bk = Bn; cp = cn; xk = xn;
         }
-        const double frac = (B[n - 1] + mx) - A[n - 1] * c1p;
-        d = frac != 0.0 ? div_(sub_(x[n - 1], mul_(d, A[n - 1])), frac) : zero_<E>();
-        x[n - 1] = d;
-        E xn = d;
-#pragma unroll 4
+        const double frac = (Bk + mx) - Ak * cp;   // row n-1 (n >= 2)
+        d = frac != 0.0 ? (xk - d * Ak) / frac : 0.0;
+        x[NC * (n - 1) + l] = d;
+        double xr = d, dk = x[NC * (n - 2) + l], ck = c1[n - 2];
         for (int k = n - 2; k >= 0; k--) {
-            xn = sub_(x[k], mul_(xn, c1[k]));
-            x[k] = xn;
+            const int kp = k > 0 ? k - 1 : 0;
+            const double dn = x[NC * kp + l], cnx = c1[kp];
+            xr = dk - xr * ck;
+            x[NC * k + l] = xr;
+            dk = dn; ck = cnx;
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < n; k += 64) xg[k] = x[k];
+    E *xo = static_cast<E *>(a.x) + o * a.os;
+    for (int k = threadIdx.x; k < n; k += 64) xo[k] = reinterpret_cast<const E *>(x)[k];
 }
 
 }  // namespace
@@ -606,7 +614,7 @@ void launch_thomas_neumann(fen_ctx *c)
     ThomasGArgs a = thomas_layout(c, &complex_e, &assoc2d);
     const dim3 grid((unsigned)((a.ncol + kNB - 1) / kNB));
     prof_begin(c, K_THOMAS);
-    const size_t lds = (size_t)a.n * ((complex_e ? sizeof(double2) : sizeof(double)) + 4 * sizeof(double));
+    const size_t lds = (size_t)a.n * ((complex_e ? sizeof(double2) : sizeof(double)) + 3 * sizeof(double));
     if (assoc2d && a.s == 1 && a.os == a.n && lds <= 65536) {
         // contiguous columns that fit in LDS: one wave each
         if (complex_e)
