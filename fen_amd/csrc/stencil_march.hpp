@@ -468,9 +468,9 @@ struct ProjectMomentumMarch {
         const long q = active ? m.b.q(i, j, k) : 0;
         const long qs = (long)m.b.nx * m.b.ny;
         if (active && k + 1 <= kend) {
-            on[0] = m.dvo0[q + qs];
-            on[1] = m.dvo1[q + qs];
-            on[2] = m.dvo2[q + qs];
+            on[0] = load_stream(&m.dvo0[q + qs]);
+            on[1] = load_stream(&m.dvo1[q + qs]);
+            on[2] = load_stream(&m.dvo2[q + qs]);
         }
         if (active) {
             auto ring = [&](int f, int slotv, int off) { return lds[(f * 3 + slotv) * S::N + c + off]; };
@@ -485,7 +485,7 @@ struct ProjectMomentumMarch {
             if (!(d == d) || !(vel == vel) || isinf(vel) || isinf(d)) bad = 1.0;
             if (d > md) md = d;
             if (vel > mv) mv = vel;
-            a.p_out[m.b.g(i, j, k)] = ring(3, (int)sc, 0);
+            store_stream(&a.p_out[m.b.g(i, j, k)], ring(3, (int)sc, 0));
             // momentum of step n+1
             auto at = [&](int f) {
                 return [=](int di, int dj, int dk) {
@@ -526,7 +526,7 @@ struct ProjectMomentumMarch {
         R o[3] = {R(0), R(0), R(0)};
         if (active) {
             const long q = b.q(i, j, k0);
-            o[0] = m.dvo0[q]; o[1] = m.dvo1[q]; o[2] = m.dvo2[q];
+            o[0] = load_stream(&m.dvo0[q]); o[1] = load_stream(&m.dvo1[q]); o[2] = load_stream(&m.dvo2[q]);
         }
         double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
         int sm = 0, sc = 1, sn = 2;
