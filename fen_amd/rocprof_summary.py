@@ -32,7 +32,16 @@ _PATTERNS = [
     (r"y_kernel<(?:double|float), \d+, 1, 0>", "y_inv"),
     (r"y_kernel<(?:double|float), \d+, -?1, 1>", "y_solve"),
     (r"z_solve_kernel", "z_solve"),
-    (r"thomas_kernel|thomas_general_kernel", "thomas"),
+    (r"thomas_c1_kernel", "thomas_c1"),
+    (r"thomas_kernel|thomas_general_kernel|thomas_column_kernel", "thomas"),
+    (r"vof_reconstruct_kernel", "vof_reconstruct"),
+    (r"vof_sweep_kernel", "vof_sweep"),
+    (r"vof_finish_kernel", "vof_finish"),
+    (r"mf_fields_kernel", "mf_fields"),
+    (r"mf_momentum_kernel", "mf_momentum"),
+    (r"adj_momentum_kernel", "adj_momentum"),
+    (r"adj_project_kernel", "adj_project"),
+    (r"adj_rhs_kernel", "adj_rhs"),
     (r"dct2_rows_kernel", "dct2"),
     (r"dct3_rows_kernel", "dct3"),
     (r"rfft_rows_kernel|irfft_rows_kernel", "rfft_y"),
