@@ -235,6 +235,8 @@ def _case(name, steps=20, precision=0):
     from fen_amd import cases
     make = {"cavity3d": lambda: cases.cavity3d(256, 256, 256, steps=steps),
             "duct3d": lambda: cases.duct3d(256, 256, 256, steps=steps),
+            "cavity2d": lambda: cases.cavity2d(256, 256, steps=steps),
+            "channel2d": lambda: cases.channel2d(256, 256, steps=steps),
             "channel": lambda: cases.channel(steps=steps)}[name]
     return case_rate(make(), steps, precision)
 
@@ -245,6 +247,8 @@ CASES = {
     "channel_fp32": lambda: _case("channel", precision=1),
     "cavity3d_nnn": lambda: _case("cavity3d"),
     "duct3d_npn": lambda: _case("duct3d"),
+    "cavity2d_nn": lambda: _case("cavity2d"),
+    "channel2d_pn": lambda: _case("channel2d"),
     "rayleigh_taylor": lambda: multiphase_rate(100),
     "adjoint_128": lambda: adjoint_rate(),
 }
