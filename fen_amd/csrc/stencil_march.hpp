@@ -21,8 +21,8 @@
 
 namespace fen {
 
-template <int TY> struct TileShape {
-    static constexpr int TX = 64, LX = TX + 2, LY = TY + 2, N = LX * LY, NT = TX * TY;
+template <int TY, int TX_ = 64> struct TileShape {
+    static constexpr int TX = TX_, LX = TX + 2, LY = TY + 2, N = LX * LY, NT = TX * TY;
     static constexpr int PER = (N + NT - 1) / NT;
 };
 
@@ -345,9 +345,9 @@ template <class R> struct PMArgs {
     int prpow2;
 };
 
-template <class R, int SRC, int TY, int KB, int ZW>
+template <class R, int SRC, int TY, int KB, int ZW, int TX = 64>
 struct ProjectMomentumMarch {
-    using S = TileShape<TY>;
+    using S = TileShape<TY, TX>;
     static constexpr int NT = S::NT, NRAW = 8;
     struct Raw {
         R v[NRAW][S::PER];
@@ -539,11 +539,11 @@ struct ProjectMomentumMarch {
     }
 };
 
-template <class R, int SRC, int ZW, int TY, int KB, int OCC = 1>
-__global__ __launch_bounds__(64 * TY, OCC) void project_momentum_kernel(PMArgs<R> a)
+template <class R, int SRC, int ZW, int TY, int KB, int OCC = 1, int TX = 64>
+__global__ __launch_bounds__(TX * TY, OCC) void project_momentum_kernel(PMArgs<R> a)
 {
-    __shared__ R lds[ring_size<TileShape<TY>, 4>()];
-    ProjectMomentumMarch<R, SRC, TY, KB, ZW>::run(a, lds);
+    __shared__ R lds[ring_size<TileShape<TY, TX>, 4>()];
+    ProjectMomentumMarch<R, SRC, TY, KB, ZW, TX>::run(a, lds);
 }
 
 
