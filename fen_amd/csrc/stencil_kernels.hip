@@ -234,16 +234,18 @@ __global__ __launch_bounds__(kThreads) void momentum_kernel_2d(MomArgs<R> a)
 // PF, minimum waves per SIMD for the register allocation OCC; the fused
 // kernel's tiles are FTX = 128 wide: two waves per row, so every row of a
 // tile plane is a 1 KB (FP64) contiguous run -- 0.811 -> 0.804 ms FP64 and
-// 0.464 -> 0.456 ms FP32 per step against 64 x 8 tiles; 256 x 2 lost 10%)
+// 0.464 -> 0.456 ms FP32 per step against 64 x 8 tiles; 256 x 2 lost 10%.
+// 128-wide momentum and projection tiles gained 0.6% on the per-step path
+// of the Neumann solvers and are not used: MTXW = PTXW = 64)
 template <class R> struct Shape;
 template <> struct Shape<double> {
-    static constexpr int MTY = 8, MKB = 32, MUN = 1, MPF = 1, MOCC = 1;   // momentum
-    static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 1;   // projection
+    static constexpr int MTY = 8, MKB = 32, MUN = 1, MPF = 1, MOCC = 1, MTXW = 64;   // momentum
+    static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 1, PTXW = 64;   // projection
     static constexpr int FTY = 4, FKB = 32, FOCC = 1, FTX = 128;          // projection + momentum
 };
 template <> struct Shape<float> {
-    static constexpr int MTY = 4, MKB = 16, MUN = 2, MPF = 2, MOCC = 1;
-    static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 4;
+    static constexpr int MTY = 4, MKB = 16, MUN = 2, MPF = 2, MOCC = 1, MTXW = 64;
+    static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 4, PTXW = 64;
     static constexpr int FTY = 4, FKB = 32, FOCC = 1, FTX = 128;
 };
 constexpr int MTX = 64;
@@ -265,8 +267,8 @@ static void grid_blocks(const Box &b, int ty, int kb, int *nbx, int *nby, int *n
 // projection + momentum), either element type: the partial buffer's size.
 long stencil_blocks(const Box &b)
 {
-    const int shapes[4][3] = {{Shape<double>::PTY, Shape<double>::PKB, MTX},
-                              {Shape<float>::PTY, Shape<float>::PKB, MTX},
+    const int shapes[4][3] = {{Shape<double>::PTY, Shape<double>::PKB, Shape<double>::PTXW},
+                              {Shape<float>::PTY, Shape<float>::PKB, Shape<float>::PTXW},
                               {Shape<double>::FTY, Shape<double>::FKB, Shape<double>::FTX},
                               {Shape<float>::FTY, Shape<float>::FKB, Shape<float>::FTX}};
     long most = 0;
@@ -304,18 +306,18 @@ template <class R> void launch_momentum(fen_ctx *c, double dt, double A, double 
     MomArgs<R> a = momentum_args<R>(c, dt, A, B);
     int nbz;
     using SH = Shape<R>;
-    grid_blocks(c->box, SH::MTY, SH::MKB, &a.nbx, &a.nby, &nbz);
+    grid_blocks(c->box, SH::MTY, SH::MKB, &a.nbx, &a.nby, &nbz, SH::MTXW);
     const int d3 = c->box.dim == 3;
     const double words = d3 ? (c->has_source ? 16.0 : 13.0) : (c->has_source ? 11.0 : 9.0);
     prof_begin(c, K_MOMENTUM);
     if (d3) {
         const dim3 grid((unsigned)(a.nbx * a.nby * nbz));
         if (c->has_source)
-            hipLaunchKernelGGL((momentum_march_kernel<R, 1, SH::MTY, SH::MKB, SH::MUN, SH::MPF, SH::MOCC>), grid,
-                               dim3(64 * SH::MTY), 0, c->stream, a);
+            hipLaunchKernelGGL((momentum_march_kernel<R, 1, SH::MTY, SH::MKB, SH::MUN, SH::MPF, SH::MOCC, SH::MTXW>),
+                               grid, dim3(SH::MTXW * SH::MTY), 0, c->stream, a);
         else
-            hipLaunchKernelGGL((momentum_march_kernel<R, 0, SH::MTY, SH::MKB, SH::MUN, SH::MPF, SH::MOCC>), grid,
-                               dim3(64 * SH::MTY), 0, c->stream, a);
+            hipLaunchKernelGGL((momentum_march_kernel<R, 0, SH::MTY, SH::MKB, SH::MUN, SH::MPF, SH::MOCC, SH::MTXW>),
+                               grid, dim3(SH::MTXW * SH::MTY), 0, c->stream, a);
     } else {
         const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
         if (c->has_source) hipLaunchKernelGGL((momentum_kernel_2d<R, 1>), grid, dim3(kThreads), 0, c->stream, a);
@@ -474,7 +476,7 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     a.img = c->img_vel & c->img_p;
     int nbz;
     using SH = Shape<R>;
-    grid_blocks(c->box, SH::PTY, SH::PKB, &a.nbx, &a.nby, &nbz);
+    grid_blocks(c->box, SH::PTY, SH::PKB, &a.nbx, &a.nby, &nbz, SH::PTXW);
     const long nblk = (long)a.nbx * a.nby * nbz;
     if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
     const bool sh = c->shift_active;
@@ -483,11 +485,11 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     if (d3) {
         const dim3 grid((unsigned)nblk);
         if (sh)
-            hipLaunchKernelGGL((project_march_kernel<R, 1, SH::PTY, SH::PKB, SH::PUN, SH::PPF, SH::POCC>), grid,
-                               dim3(64 * SH::PTY), 0, c->stream, a);
+            hipLaunchKernelGGL((project_march_kernel<R, 1, SH::PTY, SH::PKB, SH::PUN, SH::PPF, SH::POCC, SH::PTXW>),
+                               grid, dim3(SH::PTXW * SH::PTY), 0, c->stream, a);
         else
-            hipLaunchKernelGGL((project_march_kernel<R, 0, SH::PTY, SH::PKB, SH::PUN, SH::PPF, SH::POCC>), grid,
-                               dim3(64 * SH::PTY), 0, c->stream, a);
+            hipLaunchKernelGGL((project_march_kernel<R, 0, SH::PTY, SH::PKB, SH::PUN, SH::PPF, SH::POCC, SH::PTXW>),
+                               grid, dim3(SH::PTXW * SH::PTY), 0, c->stream, a);
     } else {
         const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
         if (sh) hipLaunchKernelGGL((project_kernel_2d<R, 1>), grid, dim3(kThreads), 0, c->stream, a);

@@ -85,10 +85,10 @@ __device__ __forceinline__ void tile_commit(R *lds, SL slot, const TileRegs<S, N
 template <int V> using IC = std::integral_constant<int, V>;
 
 // ----------------------------------------------------------------- momentum
-template <class R, int SRC, int TY, int KB, int UNROLL, int PF = 1>
+template <class R, int SRC, int TY, int KB, int UNROLL, int PF = 1, int TX = 64>
 struct MomentumMarch {
     static_assert((PF == 1 && (UNROLL == 1 || UNROLL == 3)) || (PF == 2 && UNROLL == 2), "launch shape");
-    using S = TileShape<TY>;
+    using S = TileShape<TY, TX>;
     static constexpr int NT = S::NT;
 
     template <class SM, class SC, class SN>
@@ -183,21 +183,21 @@ struct MomentumMarch {
 };
 
 // OCC: minimum waves per SIMD the register allocation must allow (1: no limit).
-template <class R, int SRC, int TY, int KB, int UNROLL, int PF = 1, int OCC = 1>
-__global__ __launch_bounds__(64 * TY, OCC) void momentum_march_kernel(MomArgs<R> a)
+template <class R, int SRC, int TY, int KB, int UNROLL, int PF = 1, int OCC = 1, int TX = 64>
+__global__ __launch_bounds__(TX * TY, OCC) void momentum_march_kernel(MomArgs<R> a)
 {
-    __shared__ R lds[ring_size<TileShape<TY>, 4>()];
-    MomentumMarch<R, SRC, TY, KB, UNROLL, PF>::run(a, lds);
+    __shared__ R lds[ring_size<TileShape<TY, TX>, 4>()];
+    MomentumMarch<R, SRC, TY, KB, UNROLL, PF, TX>::run(a, lds);
 }
 
 // --------------------------------------------------------------- projection
 // Ring fields: phi, u*, v* (u* and v* need the cell below in i and j for the
 // checks); w* and p are prefetched one plane ahead, w* one plane below is
 // carried in a register.
-template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1>
+template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1, int TX = 64>
 struct ProjectMarch {
     static_assert((PF == 1 && (UNROLL == 1 || UNROLL == 3)) || (PF == 2 && UNROLL == 2), "launch shape");
-    using S = TileShape<TY>;
+    using S = TileShape<TY, TX>;
     static constexpr int NT = S::NT;
 
     template <class SM, class SC, class SN>
@@ -305,11 +305,11 @@ struct ProjectMarch {
     }
 };
 
-template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1, int OCC = 1>
-__global__ __launch_bounds__(64 * TY, OCC) void project_march_kernel(ProjArgs<R> a)
+template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1, int OCC = 1, int TX = 64>
+__global__ __launch_bounds__(TX * TY, OCC) void project_march_kernel(ProjArgs<R> a)
 {
-    __shared__ R lds[ring_size<TileShape<TY>, 3>()];
-    ProjectMarch<R, SHIFT, TY, KB, UNROLL, PF>::run(a, lds);
+    __shared__ R lds[ring_size<TileShape<TY, TX>, 3>()];
+    ProjectMarch<R, SHIFT, TY, KB, UNROLL, PF, TX>::run(a, lds);
 }
 
 // ------------------------------------------------- projection + momentum
