@@ -644,6 +644,11 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
     c->hx = nx / 2 + 1;
     try {
         FEN_HIP_CHECK(hipStreamCreate(&c->stream));
+        {
+            int dev = 0;
+            FEN_HIP_CHECK(hipGetDevice(&dev));
+            FEN_HIP_CHECK(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        }
         const size_t es = elem(c);
         for (int d = 0; d < 3; d++) {
             c->d_u[d] = dev_alloc_bytes(c->n_ghost * es);

@@ -200,6 +200,7 @@ struct fen_ctx {
 
     // Poisson solver state
     int hx = 0;                  // nx/2 + 1
+    int num_cu = 256;            // compute units of the device (queried at create)
     void *d_spec = nullptr;      // complex R, [kx][k][j]
     double *d_c1 = nullptr;      // Thomas c1, computed at create (layout of the sweep's columns)
     double2 *d_dd = nullptr;     // FP64 Thomas right-hand side (ppn, FP32 form), [kx][k][j]

@@ -241,12 +241,14 @@ template <class R> struct Shape;
 template <> struct Shape<double> {
     static constexpr int MTY = 8, MKB = 32, MUN = 1, MPF = 1, MOCC = 1, MTXW = 64;   // momentum
     static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 1, PTXW = 64;   // projection
-    static constexpr int FTY = 4, FKB = 32, FOCC = 1, FTX = 128;          // projection + momentum
+    static constexpr int FTY = 4, FKB = 16, FOCC = 1, FTX = 128;          // projection + momentum
+    static constexpr bool FLONG = true;   // march as many planes as keep every CU busy
 };
 template <> struct Shape<float> {
     static constexpr int MTY = 4, MKB = 16, MUN = 2, MPF = 2, MOCC = 1, MTXW = 64;
     static constexpr int PTY = 4, PKB = 16, PUN = 1, PPF = 1, POCC = 4, PTXW = 64;
     static constexpr int FTY = 4, FKB = 32, FOCC = 1, FTX = 128;
+    static constexpr bool FLONG = false;   // two workgroups per CU: longer marches measured slower
 };
 constexpr int MTX = 64;
 
@@ -549,12 +551,20 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
     using SH = Shape<R>;
     int nbz;
     grid_blocks(c->box, SH::FTY, SH::FKB, &a.m.nbx, &a.m.nby, &nbz, SH::FTX);
+    // planes per workgroup: FKB, or with FLONG the longest march (each
+    // workgroup loads two planes before its first) that leaves no CU idle
+    a.kb = SH::FKB;
+    if (SH::FLONG) {
+        const long cols = (long)a.m.nbx * a.m.nby;
+        while (2 * a.kb <= c->box.nz && cols * ((c->box.nz + 2 * a.kb - 1) / (2 * a.kb)) >= c->num_cu) a.kb *= 2;
+    }
+    nbz = (c->box.nz + a.kb - 1) / a.kb;
     const long nblk = (long)a.m.nbx * a.m.nby * nbz;
     if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
     prof_begin(c, K_PROJ_MOM);
     const dim3 grid((unsigned)nblk), block(SH::FTX * SH::FTY);
 #define FEN_PM(SRC, ZW) \
-    hipLaunchKernelGGL((project_momentum_kernel<R, SRC, ZW, SH::FTY, SH::FKB, SH::FOCC, SH::FTX>), grid, block, 0, c->stream, a)
+    hipLaunchKernelGGL((project_momentum_kernel<R, SRC, ZW, SH::FTY, SH::FOCC, SH::FTX>), grid, block, 0, c->stream, a)
     if (c->has_source) {
         if (zwalls) FEN_PM(1, 1);
         else FEN_PM(1, 0);

@@ -343,9 +343,10 @@ template <class R> struct PMArgs {
     const R *zlo[3], *zhi[3];      // ZW: bc planes of u, v, w at the front and back faces
     double pid, pdt, prhof, pirhof;
     int prpow2;
+    int kb;                        // planes marched per workgroup
 };
 
-template <class R, int SRC, int TY, int KB, int ZW, int TX = 64>
+template <class R, int SRC, int TY, int ZW, int TX = 64>
 struct ProjectMomentumMarch {
     using S = TileShape<TY, TX>;
     static constexpr int NT = S::NT, NRAW = 8;
@@ -509,8 +510,8 @@ struct ProjectMomentumMarch {
         const int by = rest % m.nby, bz = rest / m.nby;
         const int tx = threadIdx.x % S::TX, ty = __builtin_amdgcn_readfirstlane(threadIdx.x / S::TX);
         const bool xedge = bx == 0 || bx == m.nbx - 1;
-        const int i0 = bx * S::TX + 1, j0 = by * TY + 1, k0 = bz * KB + 1;
-        const int kend = min(k0 + KB - 1, b.nz);
+        const int i0 = bx * S::TX + 1, j0 = by * TY + 1, k0 = bz * a.kb + 1;
+        const int kend = min(k0 + a.kb - 1, b.nz);
         const int i = i0 + tx, j = j0 + ty;
         const bool active = i <= b.nx && j <= b.ny;
         const int c = (ty + 1) * S::LX + tx + 1;
@@ -539,11 +540,11 @@ struct ProjectMomentumMarch {
     }
 };
 
-template <class R, int SRC, int ZW, int TY, int KB, int OCC = 1, int TX = 64>
+template <class R, int SRC, int ZW, int TY, int OCC = 1, int TX = 64>
 __global__ __launch_bounds__(TX * TY, OCC) void project_momentum_kernel(PMArgs<R> a)
 {
     __shared__ R lds[ring_size<TileShape<TY, TX>, 4>()];
-    ProjectMomentumMarch<R, SRC, TY, KB, ZW, TX>::run(a, lds);
+    ProjectMomentumMarch<R, SRC, TY, ZW, TX>::run(a, lds);
 }
 
 
