@@ -429,15 +429,30 @@ template <class R> __global__ __launch_bounds__(kBlock) void thomas_kernel(Thoma
     for (; k >= 0; k--) bwd(k, get_d((long)k * s), c1[(long)k * s]);
 }
 
-// Sum of per-block partials in a fixed order, then shift = sum / N.
-__global__ __launch_bounds__(kBlock) void mean_shift_kernel(const double *part, long n, double count, double *out)
+// Sum of per-block partials in a fixed order, then shift = sum / N.  One
+// workgroup of kShiftThreads: each thread adds its entries in batches of
+// kShiftBatch loads issued together (one memory latency per batch), then a
+// tree over the threads.
+constexpr int kShiftThreads = 1024, kShiftBatch = 4;
+__global__ __launch_bounds__(kShiftThreads) void mean_shift_kernel(const double *part, long n, double count,
+                                                                   double *out)
 {
-    __shared__ double red[kBlock];
+    __shared__ double red[kShiftThreads];
     double acc = 0.0;
-    for (long q = threadIdx.x; q < n; q += kBlock) acc += part[q];
+    for (long q0 = threadIdx.x; q0 < n; q0 += (long)kShiftThreads * kShiftBatch) {
+        double x[kShiftBatch];
+#pragma unroll
+        for (int m = 0; m < kShiftBatch; m++) {
+            const long q = q0 + (long)m * kShiftThreads;
+            x[m] = part[q < n ? q : q0];
+        }
+#pragma unroll
+        for (int m = 0; m < kShiftBatch; m++)
+            if (q0 + (long)m * kShiftThreads < n) acc += x[m];
+    }
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
+    for (int s = kShiftThreads / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
         __syncthreads();
     }
@@ -632,7 +647,7 @@ template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long
     if (want_sum) {
         const double count = (double)(float)((long)c->nx * c->ny * c->nz);
         prof_begin(c, K_REDUCE);
-        hipLaunchKernelGGL(mean_shift_kernel, dim3(1), dim3(kBlock), 0, c->stream, (const double *)c->d_part, nblk,
+        hipLaunchKernelGGL(mean_shift_kernel, dim3(1), dim3(kShiftThreads), 0, c->stream, (const double *)c->d_part, nblk,
                            count, c->d_shift);
         prof_end(c, K_REDUCE, 8.0 * nblk);
     }
