@@ -22,6 +22,7 @@
 // checkpoints every K steps and each segment is recomputed before its
 // backward sweep.  Adjoint arrays are interior-sized (nx*ny*nz), i fastest;
 // the recorded states are ghosted with their periodic images in place.
+#include <algorithm>
 #include <stdexcept>
 #include <vector>
 
@@ -281,9 +282,9 @@ void adjoint_record(fen_ctx *c, int step0, int nsteps, int every, double dt)
     const int nseg = (nsteps + every - 1) / every;
     for (int s = 0; s < nseg; s++) t->ckpt.push_back(dalloc(ckpt_len(c)));
     t->final_state = dalloc(ckpt_len(c));
-    for (int s = 0; s < nsteps; s++) {
-        if (s % every == 0) save_state(c, t->ckpt[s / every]);
-        advance_fp64(c, dt);
+    for (int s = 0; s < nsteps; s += every) {
+        save_state(c, t->ckpt[s / every]);
+        run_fp64(c, std::min(every, nsteps - s), dt);
     }
     save_state(c, t->final_state);
     FEN_HIP_CHECK(hipStreamSynchronize(c->stream));

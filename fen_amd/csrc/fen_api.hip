@@ -530,6 +530,12 @@ template <class R> void run_fused(fen_ctx *c, int nsteps, double dt)
 
 void fen::advance_fp64(fen_ctx *c, double dt) { advance<double>(c, dt); }
 
+void fen::run_fp64(fen_ctx *c, int nsteps, double dt)
+{
+    if (nsteps >= 2 && fused_run_ok(c)) run_fused<double>(c, nsteps, dt);
+    else for (int s = 0; s < nsteps; s++) advance<double>(c, dt);
+}
+
 namespace {
 
 // One step through the cached graph, capturing it first if dt, dt_o or the

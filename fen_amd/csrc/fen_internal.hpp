@@ -291,6 +291,9 @@ void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz);   //
 
 // One FP64 time step (the step fen_ns_step runs, without the CFL update).
 void advance_fp64(fen_ctx *c, double dt);
+// nsteps FP64 steps as fen_ns_run takes them (fused when the case allows;
+// bitwise equal to nsteps calls of advance_fp64).
+void run_fp64(fen_ctx *c, int nsteps, double dt);
 
 // Discrete adjoint (adjoint.hip).
 const char *adjoint_unsupported(const fen_ctx *c);   // nullptr when the context is supported
