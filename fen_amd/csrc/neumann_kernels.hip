@@ -55,28 +55,43 @@ __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
     __shared__ double2 tw[N];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
-    for (int e = threadIdx.x; e < TJ * N; e += kNB) {
-        const int row = e / N, i = e % N;
-        const long r = row0 + row;
-        double val = 0.0;
-        if (r < a.nrows) {
-            if constexpr (SRC == 2) {
-                val = a.u[r * N + i];
-            } else {
-                const int j = (int)(r % a.b.ny) + 1, k = (int)(r / a.b.ny) + 1;
-                if constexpr (SRC == 1) {
-                    val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
+    // staging: the loads of BATCH elements are issued before their LDS writes
+    constexpr int PER = TJ * N / kNB, BATCH = PER < 8 ? PER : 8;
+    static_assert(PER * kNB == TJ * N && PER % BATCH == 0, "staging batches");
+#pragma unroll
+    for (int b0 = 0; b0 < PER; b0 += BATCH) {
+        double vals[BATCH];
+#pragma unroll
+        for (int m = 0; m < BATCH; m++) {
+            const int e = threadIdx.x + (b0 + m) * kNB;
+            const int row = e / N, i = e % N;
+            const long r = row0 + row;
+            double val = 0.0;
+            if (r < a.nrows) {
+                if constexpr (SRC == 2) {
+                    val = a.u[r * N + i];
                 } else {
-                    const long g = a.b.g(i + 1, j, k);
-                    double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
-                    if (a.d3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
-                    val = d * a.rho / a.dt;
+                    const int j = (int)(r % a.b.ny) + 1, k = (int)(r / a.b.ny) + 1;
+                    if constexpr (SRC == 1) {
+                        val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
+                    } else {
+                        const long g = a.b.g(i + 1, j, k);
+                        double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
+                        if (a.d3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
+                        val = d * a.rho / a.dt;
+                    }
                 }
             }
+            vals[m] = val;
         }
-        double2 *line = buf + (row >> 1) * LL;
-        if (row & 1) line[lds_pad(mk_pos(i, N))].y = val;
-        else line[lds_pad(mk_pos(i, N))].x = val;
+#pragma unroll
+        for (int m = 0; m < BATCH; m++) {
+            const int e = threadIdx.x + (b0 + m) * kNB;
+            const int row = e / N, i = e % N;
+            double2 *line = buf + (row >> 1) * LL;
+            if (row & 1) line[lds_pad(mk_pos(i, N))].y = vals[m];
+            else line[lds_pad(mk_pos(i, N))].x = vals[m];
+        }
     }
     __syncthreads();
     fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
@@ -117,21 +132,32 @@ __global__ __launch_bounds__(kNB) void dct3_rows_kernel(Dct3Args a)
     __shared__ double2 tw[N];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
-    for (int e = threadIdx.x; e < L * N; e += kNB) {
+    // staging: all of a thread's loads are issued before its LDS writes
+    constexpr int PER = L * N / kNB;
+    static_assert(PER * kNB == L * N, "staging");
+    double a1[PER], b1[PER], a2[PER], b2[PER];   // x_k and x_{N-k} of both rows
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kNB;
         const int l = e % L, k = e / L;
         const long r = row0 + 2 * l;
-        double a1 = 0.0, b1 = 0.0, a2 = 0.0, b2 = 0.0;   // x_k and x_{N-k} of both rows
+        a1[m] = 0.0; b1[m] = 0.0; a2[m] = 0.0; b2[m] = 0.0;
         if (r < a.nrows) {
-            a1 = a.in[k * a.ik + r * a.ir];
-            if (k > 0) b1 = a.in[(N - k) * a.ik + r * a.ir];
+            a1[m] = a.in[k * a.ik + r * a.ir];
+            if (k > 0) b1[m] = a.in[(N - k) * a.ik + r * a.ir];
         }
         if (r + 1 < a.nrows) {
-            a2 = a.in[k * a.ik + (r + 1) * a.ir];
-            if (k > 0) b2 = a.in[(N - k) * a.ik + (r + 1) * a.ir];
+            a2[m] = a.in[k * a.ik + (r + 1) * a.ir];
+            if (k > 0) b2[m] = a.in[(N - k) * a.ik + (r + 1) * a.ir];
         }
+    }
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kNB;
+        const int l = e % L, k = e / L;
         const double c = a.mk[k].x, s = a.mk[k].y;
-        const double2 U1 = double2{a1 * c + b1 * s, a1 * s - b1 * c};
-        const double2 U2 = double2{a2 * c + b2 * s, a2 * s - b2 * c};
+        const double2 U1 = double2{a1[m] * c + b1[m] * s, a1[m] * s - b1[m] * c};
+        const double2 U2 = double2{a2[m] * c + b2[m] * s, a2[m] * s - b2[m] * c};
         buf[l * LL + lds_pad(k)] = double2{U1.x - U2.y, U1.y + U2.x};
     }
     __syncthreads();
