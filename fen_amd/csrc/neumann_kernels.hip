@@ -198,10 +198,22 @@ __global__ __launch_bounds__(kNB) void rfft_rows_kernel(RfftArgs a)
     __shared__ double2 tw[N];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
-    for (int e = threadIdx.x; e < TJ * N; e += kNB) {
+    // staging: every load reads a valid row (past the end: the block's
+    // first, value dropped), so all of a thread's loads issue together
+    constexpr int PER = TJ * N / kNB;
+    static_assert(PER * kNB == TJ * N, "staging");
+    double vals[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kNB;
+        const long r = row0 + e / N;
+        vals[m] = a.re[(r < a.nrows ? r : row0) * N + e % N];
+    }
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kNB;
         const int row = e / N, j = e % N;
-        const long r = row0 + row;
-        const double val = r < a.nrows ? a.re[r * N + j] : 0.0;
+        const double val = row0 + row < a.nrows ? vals[m] : 0.0;
         double2 *line = buf + (row >> 1) * LL;
         if (row & 1) line[lds_pad(j)].y = val;
         else line[lds_pad(j)].x = val;
@@ -229,11 +241,27 @@ __global__ __launch_bounds__(kNB) void irfft_rows_kernel(RfftArgs a)
     __shared__ double2 tw[N];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * 2 * L;
-    for (int e = threadIdx.x; e < L * H; e += kNB) {
+    // staging: every load reads a valid row (values past the end dropped),
+    // so all of a thread's loads issue together
+    constexpr int PER = (L * H + kNB - 1) / kNB;
+    double2 V1[PER], V2[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kNB;
+        const int ee = e < L * H ? e : 0;
+        const int l = ee / H, k = ee % H;
+        const long r = row0 + 2 * l;
+        V1[m] = a.cx[(r < a.nrows ? r : row0) * H + k];
+        V2[m] = a.cx[(r + 1 < a.nrows ? r + 1 : row0) * H + k];
+    }
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int e = threadIdx.x + m * kNB;
+        if (e >= L * H) continue;
         const int l = e / H, k = e % H;
         const long r = row0 + 2 * l;
-        const double2 X1 = r < a.nrows ? a.cx[r * H + k] : double2{0.0, 0.0};
-        const double2 X2 = r + 1 < a.nrows ? a.cx[(r + 1) * H + k] : double2{0.0, 0.0};
+        const double2 X1 = r < a.nrows ? V1[m] : double2{0.0, 0.0};
+        const double2 X2 = r + 1 < a.nrows ? V2[m] : double2{0.0, 0.0};
         double2 *line = buf + l * LL;
         if (k == 0 || k == N / 2) {
             line[lds_pad(k)] = double2{X1.x, X2.x};
