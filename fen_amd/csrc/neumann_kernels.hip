@@ -71,7 +71,8 @@ __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
                 if constexpr (SRC == 2) {
                     val = a.u[r * N + i];
                 } else {
-                    const int j = (int)(r % a.b.ny) + 1, k = (int)(r / a.b.ny) + 1;
+                    // rows number ny*nz < 2^31: 32-bit division (a 64-bit one is a long routine)
+                    const int ri = (int)r, j = ri % a.b.ny + 1, k = ri / a.b.ny + 1;
                     if constexpr (SRC == 1) {
                         val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
                     } else {
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(kNB) void dct3_rows_kernel(Dct3Args a)
         const double2 z = buf[(row >> 1) * LL + lds_pad(mk_pos(i, N))];
         const double val = ((row & 1) ? z.y : z.x) * a.post;
         if constexpr (DST == 0) {
-            const long j = r % a.ny, k = r / a.ny;
+            const int ri = (int)r, j = ri % (int)a.ny, k = ri / (int)a.ny;   // 32-bit division
             a.out[(i + a.gl) + a.sx * (j + a.gl) + a.sy * (k + a.gl)] = val;
         } else {
             a.out[r * N + i] = val;

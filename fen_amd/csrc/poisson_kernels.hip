@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void y_kernel(YArgs<R> a)
             const int l = e / N, j = e % N;
             const long ln = line0 + l;
             if (ln >= a.nlines) continue;
-            const int kx = (int)(ln / a.nz);
+            const int kx = (int)ln / (int)a.nz;   // 32-bit division
             C X = buf[l * LL + lds_pad(j)];
             const double s = a.mwx[kx] + a.mwy[j];
             if (s == 0.0) X = C{R(0), R(0)};
