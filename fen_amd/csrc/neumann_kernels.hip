@@ -65,25 +65,25 @@ __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
         for (int m = 0; m < BATCH; m++) {
             const int e = threadIdx.x + (b0 + m) * kNB;
             const int row = e / N, i = e % N;
-            const long r = row0 + row;
-            double val = 0.0;
-            if (r < a.nrows) {
-                if constexpr (SRC == 2) {
-                    val = a.u[r * N + i];
+            const long r0 = row0 + row;
+            const bool ok = r0 < a.nrows;
+            const long r = ok ? r0 : row0;   // a valid row: the loads are kept, the value dropped
+            double val;
+            if constexpr (SRC == 2) {
+                val = a.u[r * N + i];
+            } else {
+                // rows number ny*nz < 2^31: 32-bit division (a 64-bit one is a long routine)
+                const int ri = (int)r, j = ri % a.b.ny + 1, k = ri / a.b.ny + 1;
+                if constexpr (SRC == 1) {
+                    val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
                 } else {
-                    // rows number ny*nz < 2^31: 32-bit division (a 64-bit one is a long routine)
-                    const int ri = (int)r, j = ri % a.b.ny + 1, k = ri / a.b.ny + 1;
-                    if constexpr (SRC == 1) {
-                        val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
-                    } else {
-                        const long g = a.b.g(i + 1, j, k);
-                        double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
-                        if (a.d3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
-                        val = d * a.rho / a.dt;
-                    }
+                    const long g = a.b.g(i + 1, j, k);
+                    double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
+                    if (a.d3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
+                    val = d * a.rho / a.dt;
                 }
             }
-            vals[m] = val;
+            vals[m] = ok ? val : 0.0;
         }
 #pragma unroll
         for (int m = 0; m < BATCH; m++) {
