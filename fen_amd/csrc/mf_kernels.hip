@@ -302,9 +302,9 @@ dim3 cell_grid(const Box &b) { return dim3((unsigned)((b.nx + kBX - 1) / kBX), (
 // src/multiphase.f90:44-120).
 void mf_ghost(fen_ctx *c, double *const *f, int n)
 {
-    for (int s = 0; s < n; s += 4) {
-        GhostDesc<double> d[4];
-        const int m = n - s < 4 ? n - s : 4;
+    for (int s = 0; s < n; s += 8) {
+        GhostDesc<double> d[8];
+        const int m = n - s < 8 ? n - s : 8;
         for (int q = 0; q < m; q++) {
             d[q] = GhostDesc<double>{};
             d[q].f = f[s + q];

@@ -37,11 +37,12 @@ template <class T> static bool pow2_value(T r)
 // the last interior plane (f(N) = bc).  Every cell written here is computed
 // from interior cells that no thread writes, so one launch needs no ordering.
 // ---------------------------------------------------------------------------
+constexpr int kGhostFields = 8;   // fields per ghost-fill launch
 template <class R> struct GhostLaunch {
-    GhostDesc<R> d[4];
+    GhostDesc<R> d[kGhostFields];
     Box b;
-    int nx_cells, ny_cells, nz_cells;   // cells per plane of each group
-    int ngx[4], ngy[4], ngz[4];         // planes per group and field (0 when skipped)
+    int nx_cells, ny_cells, nz_cells;              // cells per plane of each group
+    int ngx[kGhostFields], ngy[kGhostFields], ngz[kGhostFields];   // planes per group and field (0: skipped)
 };
 
 template <class R> __device__ __forceinline__ R ghost_s1(const GhostDesc<R> &d, const Box &b, int i, int j, int k)
@@ -177,13 +178,17 @@ template <class R> __global__ __launch_bounds__(256) void ghost_kernel(GhostLaun
     case 0: ghost_field<R, 0>(a); break;
     case 1: ghost_field<R, 1>(a); break;
     case 2: ghost_field<R, 2>(a); break;
-    default: ghost_field<R, 3>(a); break;
+    case 3: ghost_field<R, 3>(a); break;
+    case 4: ghost_field<R, 4>(a); break;
+    case 5: ghost_field<R, 5>(a); break;
+    case 6: ghost_field<R, 6>(a); break;
+    default: ghost_field<R, 7>(a); break;
     }
 }
 
 template <class R> void launch_ghost_desc(fen_ctx *c, const GhostDesc<R> *descs, int n)
 {
-    if (n <= 0 || n > 4) throw std::runtime_error("ghost fill of 1..4 fields");
+    if (n <= 0 || n > kGhostFields) throw std::runtime_error("ghost fill of 1..8 fields");
     GhostLaunch<R> a{};
     const Box &b = c->box;
     a.b = b;
