@@ -867,6 +867,12 @@ double orc_set_timestep(orc_ctx *c, double U)
 void orc_set_dt_o(orc_ctx *c, double dt_o) { c->dt_o = dt_o; }
 double orc_get_dt_o(const orc_ctx *c) { return c->dt_o; }
 
+void orc_get_dt_limits(const orc_ctx *c, double *dt_visc, double *dt_conv)
+{
+    if (dt_visc) *dt_visc = c->dt_visc;
+    if (dt_conv) *dt_conv = c->dt_conv;
+}
+
 static ofield *ghosted(orc_ctx *c, int field)
 {
     switch (field) {

@@ -47,6 +47,8 @@ void orc_set_params(orc_ctx *c, double density, double viscosity, const double *
 double orc_set_timestep(orc_ctx *c, double U);       /* set_timestep: returns dt, sets dt_o */
 void orc_set_dt_o(orc_ctx *c, double dt_o);
 double orc_get_dt_o(const orc_ctx *c);
+/* The module variables dt_visc and dt_conv (src/navier_stokes.f90:27). */
+void orc_get_dt_limits(const orc_ctx *c, double *dt_visc, double *dt_conv);
 
 /* Direct access to the storage of a field (ghosted for p, phi, u, v, w). */
 double *orc_field(orc_ctx *c, int field);

@@ -47,6 +47,7 @@ def lib():
         L.orc_set_dt_o.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_get_dt_o.argtypes = [ctypes.c_void_p]
         L.orc_get_dt_o.restype = ctypes.c_double
+        L.orc_get_dt_limits.argtypes = [ctypes.c_void_p, dp, dp]
         L.orc_field.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_field.restype = dp
         L.orc_field_len.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -134,6 +135,12 @@ class Oracle:
     @property
     def dt_o(self):
         return lib().orc_get_dt_o(self.h)
+
+    def dt_limits(self):
+        """(dt_visc, dt_conv) as set_timestep / update_timestep left them."""
+        a, b = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        lib().orc_get_dt_limits(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def ghosted_shape(self):
         nx, ny, nz = self.n

@@ -315,6 +315,44 @@ def test_advection_closed_form_order():
     assert order > 1.8, errs
 
 
+def test_advection_3d_closed_form_order():
+    """add_advection in 3-D (src/navier_stokes.f90:297-347) against the continuum
+    divergence form d(u u_j)/dx_j of a field that is neither solenoidal nor
+    Beltrami, so no part of the term can hide in a gradient: each component
+    converges at second order to its own closed form (derivatives by sympy)."""
+    import sympy as sp
+    X, Y, Z = sp.symbols("x y z")
+    ue = sp.sin(X + 0.3) * sp.cos(2 * Y) * sp.cos(Z)
+    ve = sp.cos(X) * sp.sin(Y + 0.7) * sp.sin(2 * Z) + 0.5
+    we = sp.sin(2 * X) * sp.cos(Y) * sp.sin(Z + 0.2) - 0.25
+    vel = (ue, ve, we)
+    adv = [sp.lambdify((X, Y, Z), sum(sp.diff(vel[a] * vel[b], s) for b, s in enumerate((X, Y, Z))), "numpy")
+           for a in range(3)]
+    fun = [sp.lambdify((X, Y, Z), e, "numpy") for e in vel]
+    errs, Ns = [], [16, 32, 64]
+    for N in Ns:
+        L = 2 * math.pi
+        o = O.Oracle(3, (N, N, N), (L, L, L))
+        d = o.delta
+        f = np.arange(1, N + 1) * d           # face coordinate of index i
+        c = (np.arange(1, N + 1) - 0.5) * d   # centre coordinate
+        locs = [(f, c, c), (c, f, c), (c, c, f)]
+        for a, fid in enumerate((U, V, W)):
+            gx, gy, gz = np.meshgrid(*locs[a], indexing="ij")
+            o.field(fid)[1:-1, 1:-1, 1:-1] = fun[a](gx, gy, gz)
+            o.update_ghost_nodes(fid)
+        r = o.add_advection()
+        e = []
+        for a in range(3):
+            gx, gy, gz = np.meshgrid(*locs[a], indexing="ij")
+            e.append(np.abs(r[a] + adv[a](gx, gy, gz)).max())
+        errs.append(e)
+    errs = np.array(errs)
+    for a in range(3):
+        order = -np.polyfit(np.log(Ns), np.log(errs[:, a]), 1)[0]
+        assert order > 1.8, (a, errs[:, a])
+
+
 def test_ghost_rules_mean_what_they_say():
     """Dirichlet tangential / centre: the face value (ghost+first)/2 is bc;
     Dirichlet normal: the face itself is bc (high side overwrites f(N));
