@@ -83,14 +83,27 @@ def cpu_baseline(N, budget_s):
     import statistics
     from oracle import oracle as O
     from fen_amd import cases
-    from tests.common import make_oracle
+    from fen_amd._lib import W
 
     cores = len(os.sched_getaffinity(0))
     case = cases.tgv3d(N, steps=8)
+
+    def make_oracle():
+        o = O.Oracle(case.dim, case.n, case.L, bc=case.bc)
+
+        def up(fid, arr):
+            if fid <= W:
+                o.field(fid)[1:-1, 1:-1, 1:-1] = arr
+            else:
+                o.field(fid)[...] = arr
+
+        case.apply(o, up)
+        return o
+
     out = {}
     for label, threads in (("all", cores), ("single", 1)):
         O.set_num_threads(threads)
-        o = make_oracle(case)
+        o = make_oracle()
         t0 = time.perf_counter()
         o.step(1, case.dt)
         first = time.perf_counter() - t0
@@ -115,7 +128,7 @@ def cpu_baseline(N, budget_s):
 def tgv2d_time(steps=200):
     """2-D 128^2 case: launch-bound, reported as time per step."""
     from fen_amd import cases
-    from tests.common import make_gpu
+    from fen_amd.cases import make_solver as make_gpu
     case = cases.tgv2d(128, steps=steps)
     g = make_gpu(case)
     g.run(0, 10, case.dt)
@@ -130,7 +143,7 @@ def tgv2d_time(steps=200):
 
 def case_rate(case, steps=20, precision=0):
     """Time per step of a case (3 warm-up steps, then `steps` steps of fen_ns_run)."""
-    from tests.common import make_gpu
+    from fen_amd.cases import make_solver as make_gpu
     g = make_gpu(case, precision=precision)
     g.run(0, 3, case.dt)
     g.synchronize()
@@ -155,7 +168,7 @@ def multiphase_rate(steps=100):
     import numpy as np
     from fen_amd import cases
     from fen_amd._lib import VOF
-    from tests.common import make_gpu
+    from fen_amd.cases import make_solver as make_gpu
     case, props, vof = cases.rayleigh_taylor(steps=steps)
     g = make_gpu(case)
     g.set_multiphase(**props)
@@ -186,7 +199,8 @@ def adjoint_rate(N=128, steps=200, every=20):
     import numpy as np
     from fen_amd import Solver, cases
     from fen_amd._lib import U, V, W
-    from tests.adjoint_harness import final_velocity, load_state
+    from fen_amd.cases import interior_velocity as final_velocity
+    from fen_amd.cases import load_state
     case = cases.tgv3d(N, Re=400.0, steps=steps)
     s = Solver(3, case.n, case.L)
     s.set_params(density=1.0, viscosity=case.viscosity, cfl=0.5)
@@ -231,6 +245,48 @@ def neumann_rates(steps=20):
     return out
 
 
+def dropin_rate(N=256, steps=100, read_every=1, warmup=5):
+    """FEN's navier_stokes_solver as the Fortran layer drives it, one C-ABI
+    call sequence per step: fen_set_params (unchanged values), fen_set_dt_o,
+    fen_ns_step (asking for maxdiv and maxCFL on the steps the program reads
+    them), fen_get_scalars (dt_o, dt_visc, dt_conv; no device sync).
+    read_every = 1 is the layer's default (maxdiv, maxCFL current after every
+    step); read_every = 100 is `lazy_checks` with print_solver_status every
+    100 steps.  3-D Taylor-Green N^3 FP64, wall-clock time per step."""
+    import ctypes
+    from fen_amd import cases
+    from fen_amd.cases import make_solver
+    case = cases.tgv3d(N, steps=steps)
+    g = make_solver(case)
+    L, h, by = g.lib, g.h, ctypes.byref
+    gv = (ctypes.c_double * 3)(*case.g)
+    dt = ctypes.c_double(case.dt)
+    md, mc, dto, dtv, dtc = (ctypes.c_double() for _ in range(5))
+    dto.value = case.dt_o
+
+    def step(s):
+        rc = L.fen_set_params(h, case.density, case.viscosity, gv, case.cfl, 0)
+        rc |= L.fen_set_dt_o(h, dto.value)
+        if s % read_every == 0:
+            rc |= L.fen_ns_step(h, s, by(dt), by(md), by(mc))
+        else:
+            rc |= L.fen_ns_step(h, s, by(dt), None, None)
+        rc |= L.fen_get_scalars(h, None, None, by(dto), by(dtv), by(dtc))
+        if rc:
+            raise RuntimeError(f"drop-in step {s}: {L.fen_last_error(h)}")
+
+    for s in range(1, warmup + 1):
+        step(s)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for s in range(warmup + 1, warmup + steps + 1):
+        step(s)
+    g.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    g.close()
+    return {"ms_per_step": ms, "value": N ** 3 / (ms * 1e-3), "steps": steps, "checks_read_every": read_every}
+
+
 def _case(name, steps=20, precision=0):
     from fen_amd import cases
     make = {"cavity3d": lambda: cases.cavity3d(256, 256, 256, steps=steps),
@@ -258,7 +314,7 @@ def measure(N, steps, warmup, precision):
     """Warm-up, one untimed calibration of the per-kernel breakdown, then the
     timed steps.  Returns (solver, result dict)."""
     from fen_amd import cases
-    from tests.common import make_gpu
+    from fen_amd.cases import make_solver as make_gpu
 
     case = cases.tgv3d(N, steps=steps)
     g = make_gpu(case, precision=precision)
@@ -330,7 +386,7 @@ def fp32_accuracy(N, steps):
     same seeded state; errors of the FP32 form against FP64."""
     from fen_amd import cases
     from fen_amd._lib import FP32_REFINE
-    from tests.common import make_gpu
+    from fen_amd.cases import make_solver as make_gpu
 
     case = cases.tgv3d(N, steps=steps)
     res = {}
@@ -353,6 +409,7 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid size N (N^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+refine form")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in (per-step call sequence) timing")
     ap.add_argument("--no-extra", action="store_true", help="skip the 512^3, channel and 2-D measurements")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle steps")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="BYTES",
@@ -383,6 +440,14 @@ def main():
     workload = f"tgv3d_{N}"
     traffic = read_traffic(r64["dominant"], workload)
     step_gbs = WORDS_3D * 8.0 * N ** 3 / (r64["ms"] * 1e-3) / 1e9
+
+    # the drop-in path (the Fortran layer's per-step sequence), next to the run above
+    dropin = None
+    if not args.no_dropin:
+        dropin = {"every_step": dropin_rate(N, args.steps, 1), "every_100_steps": dropin_rate(N, args.steps, 100),
+                  "fen_ns_run_ms_per_step": r64["ms"]}
+        for v in (dropin["every_step"], dropin["every_100_steps"]):
+            v["over_fen_ns_run"] = v["ms_per_step"] / r64["ms"]
 
     fp32 = None
     if not args.no_fp32:
@@ -445,6 +510,7 @@ def main():
                           "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
         "kernel_us_per_launch": r64["breakdown"],
         "maxdiv": r64["maxdiv"],
+        "dropin": dropin,
         "fp32_refine": fp32,
         "other_configs": extra,
         "cpu_baseline": cpu,

@@ -70,6 +70,43 @@ class Case:
             solver.update_ghost_nodes(fid)
 
 
+def ghosted(interior):
+    """A ghosted Fortran-ordered array with `interior` in its interior, zero ghosts."""
+    nx, ny, nz = interior.shape
+    g = np.zeros((nx + 2, ny + 2, nz + 2), order="F")
+    g[1:-1, 1:-1, 1:-1] = interior
+    return g
+
+
+def make_solver(case, precision=0):
+    """A libfen solver on the GPU configured with `case`: grid, parameters,
+    boundary planes and initial fields (ghosts filled by update_ghost_nodes)."""
+    from .solver import Solver
+    s = Solver(case.dim, case.n, case.L, bc=case.bc, precision=precision)
+
+    def up(fid, arr):
+        s.upload(fid, ghosted(arr) if fid <= W else arr)
+
+    case.apply(s, up)
+    return s
+
+
+def load_state(solver, state, dt_o):
+    """Uploads interior fields u, v, w, p (ghosts then filled) and dv_o (zero
+    unless given), and sets dt_o."""
+    from ._lib import DVO_X, DVO_Y, DVO_Z
+    for fid in (U, V, W, P):
+        solver.upload(fid, ghosted(np.asfortranarray(state[fid])))
+        solver.update_ghost_nodes(fid)
+    for fid in (DVO_X, DVO_Y, DVO_Z):
+        solver.upload(fid, np.asfortranarray(state.get(fid, np.zeros(solver.n))))
+    solver.set_dt_o(dt_o)
+
+
+def interior_velocity(solver):
+    return [np.ascontiguousarray(solver.download(f)[1:-1, 1:-1, 1:-1]) for f in (U, V, W)]
+
+
 def _coords(n, delta):
     return np.arange(1, n + 1, dtype=np.float64)
 
@@ -122,6 +159,39 @@ def tgv3d(N=256, Re=1600.0, steps=200, eps=1e-3, cfl=0.5, Uref=2.0, seed=SEED):
     return Case("tgv3d", 3, (N, N, N), (L, L, L), [BC_PERIODIC] * 6, 1.0, nu, (0.0, 0.0, 0.0), dt, dt, steps,
                 {U: np.asfortranarray(u), V: np.asfortranarray(v), W: np.asfortranarray(w), P: np.asfortranarray(p)},
                 cfl=cfl)
+
+
+def isotropic(N=128, nu=1.0e-2, amp=1.0e-2, seed=SEED):
+    """Forced isotropic turbulence start (test/large_test/isotropic_turbulence/
+    isotropic.f90:30-58, 86-106): triply periodic [0,2pi]^3, viscosity 1e-2,
+    ABC velocity evaluated at cell centres plus amp * U[0,1) noise,
+    u = cos y + sin z, v = sin x + cos z, w = cos x + sin y; p = 0.
+    The program sets dt with set_timestep(U = 1) at the default CFL = 1 and
+    then switches to constant CFL = 0.9 (`cfl_run`)."""
+    L = 2.0 * math.pi
+    d = L / float(np.float32(N))
+    c = (np.arange(1, N + 1, dtype=np.float64) - 0.5) * d
+    shp = (N, N, N)
+    n3 = N ** 3
+    X, Y, Z = c[:, None, None], c[None, :, None], c[None, None, :]
+    u = np.broadcast_to(np.cos(Y) + np.sin(Z), shp).copy(order="F")
+    v = np.broadcast_to(np.sin(X) + np.cos(Z), shp).copy(order="F")
+    w = np.broadcast_to(np.cos(X) + np.sin(Y), shp).copy(order="F")
+    for comp, arr in enumerate((u, v, w)):
+        arr += amp * uniform01(comp, n3, seed).reshape(shp, order="F")
+    p = np.zeros(shp, order="F")
+    case = Case("isotropic", 3, shp, (L, L, L), [BC_PERIODIC] * 6, 1.0, nu, (0.0, 0.0, 0.0), 0.0, 0.0, 0,
+                {U: u, V: v, W: w, P: p}, cfl=1.0)
+    case.set_timestep_U = 1.0
+    case.cfl_run = 0.9
+    return case
+
+
+def isotropic_forcing(u, v, w):
+    """The program's linear forcing S = 0.1 (u - mean u), per component, from
+    the interior velocity before each step (isotropic.f90:115-150)."""
+    n = u.size
+    return [0.1 * (f - f.sum() / n) for f in (u, v, w)]
 
 
 def channel(nx=512, ny=512, nz=256, Ub=15.7, noise=0.05, nu=1.0 / 180.0, steps=100, cfl=0.5, seed=SEED):

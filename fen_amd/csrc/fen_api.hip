@@ -8,8 +8,10 @@
 //   K6 x c2r or DCT-III -> phi (+ mean, pp/ppn/pn)   ghost(phi)
 //   K7 projection + p update           ghost(u, p)
 //   K8 checks
-// fen_ns_run on a fully periodic 3-D grid fuses K7-K8 of each step with K1
-// of the next (run_fused).
+// On the 3-D grids the fused kernel supports (periodic, or periodic in x, y
+// with walls at z) a step ends after K6 with K7-K8 pending; the next step
+// runs them fused with its K1 (step_pending), and anything that reads or
+// changes the state first completes them (flush_pending).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -481,60 +483,87 @@ void advance_mf(fen_ctx *c, double dt)
     finish_step<double>(c, dt);
 }
 
-// Whether fen_ns_run may fuse each step's projection with the next step's
-// momentum: 3-D at fixed dt, every field periodic in every direction (ppp)
-// or x, y periodic with no-slip walls at z (ppn).
-bool fused_run_ok(const fen_ctx *c)
+// Whether a step may end with its projection pending, to be fused into the
+// next step's momentum (launch_project_momentum): 3-D, every field periodic
+// in every direction (ppp) or x, y periodic with no-slip walls at z (ppn).
+bool deferrable(const fen_ctx *c)
 {
-    if (!c->fused_run || c->dim != 3 || c->constant_cfl) return false;
+    if (!c->fused_run || c->dim != 3 || c->mf) return false;
     if (c->solver == FEN_SOLVER_PPP) return c->img_vel == 7 && c->img_p == 7 && c->img_phi == 7;
     return c->solver == FEN_SOLVER_PPN && fen::fused_z_walls(c);
 }
 
-// nsteps steps at fixed dt with the projection and checks of each step but
-// the last fused with the next step's momentum (launch_project_momentum).
-// The state after an inner step is never stored: u*, v*, w* alternate
-// between the u* arrays and the state arrays, p between p and a second
-// array, and the last step's projection writes u, v, w, p where advance()
-// leaves them.  Every step's checks are reduced as in advance(); the kernels
-// compute the same values, so the result equals nsteps calls of advance()
+// One step that ends pending (fen_ctx::pending).  If the previous step is
+// pending, its projection, pressure update and checks run fused with this
+// step's momentum; the new u*, v*, w* land in the state arrays and p in the
+// second pressure array, and the arrays then swap roles, so a pending step
+// always leaves u* in d_us and p (before its update) in d_p.  The fused
+// kernel computes the same values as the separate projection and momentum,
+// so a sequence of pending steps equals the same number of advance() calls
 // bit for bit.
-template <class R> void run_fused(fen_ctx *c, int nsteps, double dt)
+template <class R> void step_pending(fen_ctx *c, double dt)
 {
     using namespace fen;
-    if (!c->d_p2) c->d_p2 = dev_alloc_bytes(c->n_ghost * elem(c));
-    const double A = 1.0 + 0.5 * dt / c->dt_o, B = -(0.5 * dt / c->dt_o);
-    void *us[2][3] = {{c->d_us[0], c->d_us[1], c->d_us[2]}, {c->d_u[0], c->d_u[1], c->d_u[2]}};
-    void *ps[2] = {c->d_p, c->d_p2};
-    predict<R>(c, dt);
-    solve_phi<R>(c, dt, us[0]);
-    // an even number of fused steps leaves u* in the u* arrays and p in p
-    const int nfused = (nsteps % 2) ? nsteps - 1 : nsteps - 2;
-    int cur = 0;
-    for (int s = 0; s < nfused; s++) {
+    if (!c->pending) {
+        predict<R>(c, dt);
+    } else {
+        if (!c->d_p2) c->d_p2 = dev_alloc_bytes(c->n_ghost * elem(c));
+        const double A = 1.0 + 0.5 * dt / c->dt_o, B = -(0.5 * dt / c->dt_o);   // src/navier_stokes.f90:157-158
         c->last_dt = dt;
         c->step_count++;
-        launch_project_momentum<R>(c, dt, dt, A, B, us[cur], ps[cur], us[1 - cur], ps[1 - cur], c->d_scal);
-        cur ^= 1;
+        void *out[3] = {c->d_u[0], c->d_u[1], c->d_u[2]};
+        launch_project_momentum<R>(c, c->pend_dt, dt, A, B, c->d_us, c->d_p, out, c->d_p2, c->d_scal);
+        for (int d = 0; d < 3; d++) std::swap(c->d_u[d], c->d_us[d]);
+        std::swap(c->d_p, c->d_p2);
+        c->config_version++;   // a captured step graph holds the old array roles
         if (const int rest = remaining_groups(c, c->img_vel)) {   // z walls: ghosts of u*, v*, w*
-            GhostDesc<R> g[3] = {make_desc<R>(c, FEN_U, us[cur][0]), make_desc<R>(c, FEN_V, us[cur][1]),
-                                 make_desc<R>(c, FEN_W, us[cur][2])};
+            GhostDesc<R> g[3] = {make_desc<R>(c, FEN_U, c->d_us[0]), make_desc<R>(c, FEN_V, c->d_us[1]),
+                                 make_desc<R>(c, FEN_W, c->d_us[2])};
             for (auto &d : g) d.groups = rest;
             launch_ghost_desc<R>(c, g, 3);
         }
-        solve_phi<R>(c, dt, us[cur]);
     }
-    finish_step<R>(c, dt);
-    if (nsteps % 2 == 0) advance<R>(c, dt);
-}}  // namespace
+    solve_phi<R>(c, dt, c->d_us);
+    c->pending = true;
+    c->pending_checked = false;
+    c->pend_dt = dt;
+    c->checks_valid = true;
+}
+
+// Completes a pending step (projection, pressure update, checks, ghosts),
+// leaving the state where advance() leaves it.  Every entry point that reads
+// or changes the state, the fields' boundary data or the parameters the
+// projection uses calls this first.
+void flush_pending(fen_ctx *c)
+{
+    if (!c->pending) return;
+    c->pending = false;
+    with_type(c, [&](auto tag) { finish_step<decltype(tag)>(c, c->pend_dt); });
+}
+
+// The checks of a pending step (max div and max |u|+|v|+|w| of its projected
+// velocity) into d_scal, without forming the state.
+void pending_checks(fen_ctx *c)
+{
+    if (!c->pending || c->pending_checked) return;
+    with_type(c, [&](auto tag) { fen::launch_project<decltype(tag)>(c, c->pend_dt, c->d_scal, false); });
+    c->pending_checked = true;
+}
+}  // namespace
 
 void fen::advance_fp64(fen_ctx *c, double dt) { advance<double>(c, dt); }
 
 void fen::run_fp64(fen_ctx *c, int nsteps, double dt)
 {
-    if (nsteps >= 2 && fused_run_ok(c)) run_fused<double>(c, nsteps, dt);
-    else for (int s = 0; s < nsteps; s++) advance<double>(c, dt);
+    if (deferrable(c)) {
+        for (int s = 0; s < nsteps; s++) step_pending<double>(c, dt);
+        flush_pending(c);
+    } else {
+        for (int s = 0; s < nsteps; s++) advance<double>(c, dt);
+    }
 }
+
+void fen::flush_state(fen_ctx *c) { flush_pending(c); }
 
 namespace {
 
@@ -740,9 +769,17 @@ int fen_set_params(fen_ctx *c, double density, double viscosity, const double *g
 {
     if (!c) return FEN_ERR_ARG;
     if (!(density > 0.0) || !(viscosity >= 0.0)) return fail(c, FEN_ERR_ARG, "density must be > 0, viscosity >= 0");
+    const double gn[3] = {g ? g[0] : 0.0, g ? g[1] : 0.0, g ? g[2] : 0.0};
+    // a caller that sets the same values every step (the Fortran layer does)
+    // must not invalidate the captured step graph
+    bool same = c->density == density && c->viscosity == viscosity && c->cfl == cfl &&
+                (c->constant_cfl != 0) == (constant_cfl != 0);
+    for (int d = 0; d < 3; d++) same = same && c->g[d] == gn[d];
+    if (same) return FEN_OK;
+    FEN_TRY(c, flush_pending(c));
     c->density = density;
     c->viscosity = viscosity;
-    for (int d = 0; d < 3; d++) c->g[d] = g ? g[d] : 0.0;
+    for (int d = 0; d < 3; d++) c->g[d] = gn[d];
     c->cfl = cfl;
     c->constant_cfl = constant_cfl;
     c->config_version++;
@@ -778,6 +815,7 @@ int fen_set_multiphase(fen_ctx *c, double rho0, double rho1, double mu0, double 
     if (!c) return FEN_ERR_ARG;
     if (c->dim != 2 || c->precision != FEN_FP64 || c->mf)
         return fail(c, FEN_ERR_UNSUPPORTED, "the two-phase mode is 2-D, FP64, set once");
+    FEN_TRY(c, flush_pending(c));
     for (int f = 0; f < 4; f++)
         if (c->bc_phys[f] != FEN_BC_PERIODIC && c->bc_phys[f] != FEN_BC_WALL)
             return fail(c, FEN_ERR_UNSUPPORTED, "two-phase faces must be Periodic or Wall");
@@ -802,6 +840,7 @@ int fen_update_material_properties(fen_ctx *c)
     if (!c) return FEN_ERR_ARG;
     if (!c->mf) return fail(c, FEN_ERR_STATE, "not in two-phase mode");
     FEN_TRY(c, {
+        flush_pending(c);
         fen::mf_update_fields(c, false);
         sync(c);
     });
@@ -813,6 +852,7 @@ int fen_advect_vof(fen_ctx *c, double dt)
     if (!c) return FEN_ERR_ARG;
     if (!c->mf) return fail(c, FEN_ERR_STATE, "not in two-phase mode");
     FEN_TRY(c, {
+        flush_pending(c);
         fen::mf_advect_vof(c, dt);
         sync(c);
     });
@@ -838,6 +878,7 @@ int fen_get_scalars(fen_ctx *c, double *maxdiv, double *maxcfl, double *dt_o, do
             if (maxdiv) *maxdiv = 0.0;
             if (maxcfl) *maxcfl = 0.0;
         } else if (maxdiv || maxcfl) {
+            pending_checks(c);
             FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal, c->d_scal, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             sync(c);
             if (maxdiv) *maxdiv = c->h_scal[0];
@@ -867,6 +908,9 @@ int fen_upload(fen_ctx *c, int field, const double *host, size_t count)
     if (field == FEN_RHO || field == FEN_MU)
         return fail(c, FEN_ERR_ARG, "rho and mu follow from the vof (fen_update_material_properties)");
     FEN_TRY(c, {
+        // a source term is read by the next momentum only; anything else
+        // completes a pending step first
+        if (field < FEN_S_X || field > FEN_S_Z) flush_pending(c);
         if (field >= FEN_S_X && field <= FEN_S_Z && !c->d_S[0]) {
             for (int d = 0; d < 3; d++) c->d_S[d] = dev_alloc_bytes(c->n_int * elem(c));
         }
@@ -888,6 +932,7 @@ int fen_download(fen_ctx *c, int field, double *host, size_t count)
     if (fen_field_count(c, field, &n) != FEN_OK || n != count)
         return fail(c, FEN_ERR_ARG, "field id or element count does not match the grid");
     FEN_TRY(c, {
+        flush_pending(c);
         const void *src = field_ptr(c, field);
         if (!src) {   // source never set: zero, as allocated by the reference
             std::memset(host, 0, n * sizeof(double));
@@ -914,7 +959,10 @@ int fen_set_bc_plane(fen_ctx *c, int field, int face, const double *plane, size_
     const int s = field_slot(field);
     if (s < 0 || face < 0 || face > 5) return fail(c, FEN_ERR_ARG, "bad field or face");
     if (count != plane_count(c, face)) return fail(c, FEN_ERR_ARG, "plane size does not match the face");
-    FEN_TRY(c, copy_in(c, c->d_plane[s][face], plane, count));
+    FEN_TRY(c, {
+        flush_pending(c);
+        copy_in(c, c->d_plane[s][face], plane, count);
+    });
     return FEN_OK;
 }
 
@@ -923,6 +971,7 @@ int fen_set_bc_type(fen_ctx *c, int field, int face, int type)
     if (!c) return FEN_ERR_ARG;
     const int s = field_slot(field);
     if (s < 0 || face < 0 || face > 5 || type < 0 || type > 2) return fail(c, FEN_ERR_ARG, "bad field, face or type");
+    FEN_TRY(c, flush_pending(c));
     c->bc_type[s][face] = type;
     refresh_images(c);
     c->config_version++;
@@ -948,6 +997,7 @@ int fen_update_ghost_nodes(fen_ctx *c, int field)
     if (field_slot(field) < 0) return fail(c, FEN_ERR_ARG, "field has no ghost nodes");
     if (field == FEN_W && c->dim == 2) return fail(c, FEN_ERR_ARG, "no w component in 2-D");
     FEN_TRY(c, {
+        flush_pending(c);
         with_type(c, [&](auto tag) {
             using R = decltype(tag);
             if (field == FEN_PHI && c->shift_active) {
@@ -970,9 +1020,12 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
     if (c->mf && c->constant_cfl) return fail(c, FEN_ERR_UNSUPPORTED, "constant CFL with the two-phase mode");
     FEN_TRY(c, {
         if (c->constant_cfl) {
-            // update_timestep (src/navier_stokes.f90:670-730)
+            // update_timestep (src/navier_stokes.f90:670-730): the max of
+            // |u|+|v|+|w| of the state, from the last step's checks
             if (!c->have_dt_visc) return fail(c, FEN_ERR_STATE, "constant CFL needs fen_set_timestep first");
-            if (!c->checks_valid) {
+            if (c->pending) {
+                pending_checks(c);
+            } else if (!c->checks_valid) {
                 with_type(c, [&](auto tag) {
                     using R = decltype(tag);
                     const R *vn[3] = {as<R>(c->d_u[0]), as<R>(c->d_u[1]), as<R>(c->d_u[2])};
@@ -988,9 +1041,14 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
             if (*dt > 1.1 * c->dt_o) *dt = 1.1 * c->dt_o;
         }
         const double dtv = *dt;
-        if (c->mf) advance_mf(c, dtv);
-        else if (c->use_graph && !c->constant_cfl && !c->prof_mask) advance_graph(c, dtv);
-        else with_type(c, [&](auto tag) { advance<decltype(tag)>(c, dtv); });
+        if (deferrable(c)) {
+            with_type(c, [&](auto tag) { step_pending<decltype(tag)>(c, dtv); });
+        } else {
+            flush_pending(c);
+            if (c->mf) advance_mf(c, dtv);
+            else if (c->use_graph && !c->constant_cfl && !c->prof_mask) advance_graph(c, dtv);
+            else with_type(c, [&](auto tag) { advance<decltype(tag)>(c, dtv); });
+        }
         FEN_HIP_CHECK(hipGetLastError());
     });
     if (maxdiv || maxcfl) return fen_get_scalars(c, maxdiv, maxcfl, nullptr, nullptr, nullptr);
@@ -1000,14 +1058,6 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
 int fen_ns_run(fen_ctx *c, int step0, int nsteps, double *dt)
 {
     if (!c || !dt || nsteps < 0) return FEN_ERR_ARG;
-    if (nsteps >= 2 && fused_run_ok(c)) {
-        if (!c->have_dt_o) return fail(c, FEN_ERR_STATE, "dt_o is undefined: call fen_set_timestep or fen_set_dt_o");
-        FEN_TRY(c, {
-            with_type(c, [&](auto tag) { run_fused<decltype(tag)>(c, nsteps, *dt); });
-            FEN_HIP_CHECK(hipGetLastError());
-        });
-        return FEN_OK;
-    }
     for (int s = 0; s < nsteps; s++) {
         const int rc = fen_ns_step(c, step0 + 1 + s, dt, nullptr, nullptr);
         if (rc != FEN_OK) return rc;
@@ -1022,6 +1072,16 @@ int fen_poisson_solve(fen_ctx *c, double *f, int gl)
     const long sx = gl ? c->nx + 2 : c->nx;
     const long sy = gl ? (long)(c->nx + 2) * (c->ny + 2) : (long)c->nx * c->ny;
     FEN_TRY(c, {
+        flush_pending(c);
+        // the solve writes its own mean into the shift slot: apply phi's
+        // pending shift first
+        if (c->shift_active) {
+            with_type(c, [&](auto tag) {
+                using R = decltype(tag);
+                fen::launch_apply_shift<R>(c, as<R>(c->d_phi), c->n_ghost);
+            });
+            c->shift_active = false;
+        }
         void *buf = c->d_us[0];   // scratch between steps
         copy_in(c, buf, f, n);
         const bool mean = removes_mean(c);
@@ -1073,6 +1133,7 @@ int fen_add_advection(fen_ctx *c, double *rx, double *ry, double *rz)
     if (!c || !rx || !ry || (c->dim == 3 && !rz)) return FEN_ERR_ARG;
     if (c->precision != FEN_FP64) return fail(c, FEN_ERR_UNSUPPORTED, "add_advection is provided for FP64 contexts");
     FEN_TRY(c, {
+        flush_pending(c);
         double *r[3] = {as<double>(c->d_us[0]), as<double>(c->d_us[1]), as<double>(c->d_us[2])};
         double *h[3] = {rx, ry, rz};
         const int nd = c->dim == 3 ? 3 : 2;
@@ -1221,7 +1282,10 @@ int fen_adjoint_record(fen_ctx *c, int step0, int nsteps, int every, double *dt)
     if (!c || !dt || nsteps < 1 || every < 1) return FEN_ERR_ARG;
     if (const char *why = fen::adjoint_unsupported(c)) return fail(c, FEN_ERR_UNSUPPORTED, why);
     if (!c->have_dt_o) return fail(c, FEN_ERR_STATE, "dt_o is undefined: call fen_set_timestep or fen_set_dt_o");
-    FEN_TRY(c, fen::adjoint_record(c, step0, nsteps, every, *dt));
+    FEN_TRY(c, {
+        flush_pending(c);
+        fen::adjoint_record(c, step0, nsteps, every, *dt);
+    });
     return FEN_OK;
 }
 
@@ -1232,7 +1296,10 @@ int fen_adjoint_backward(fen_ctx *c, const double *seed_u, const double *seed_v,
     if (!c->tape) return fail(c, FEN_ERR_STATE, "no recorded run: call fen_adjoint_record first");
     const double *seed[3] = {seed_u, seed_v, seed_w};
     double *grad[3] = {grad_u, grad_v, grad_w};
-    FEN_TRY(c, fen::adjoint_backward(c, seed, grad, grad_p));
+    FEN_TRY(c, {
+        flush_pending(c);
+        fen::adjoint_backward(c, seed, grad, grad_p);
+    });
     return FEN_OK;
 }
 

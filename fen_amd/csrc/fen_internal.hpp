@@ -223,6 +223,17 @@ struct fen_ctx {
     double last_dt = 0.0;        // dt of the last step (for maxCFL)
     bool checks_valid = false;
 
+    // Deferred projection (fen_ns_step / fen_ns_run on the grids the fused
+    // projection + momentum kernel supports): a step ends after its Poisson
+    // solve with u*, v*, w* in d_us, p before the step in d_p and phi in
+    // d_phi; the next step's fused kernel, or a flush before anything reads
+    // or changes the state, projects it.  The state arrays d_u and the
+    // second pressure array d_p2 swap roles with d_us and d_p after every
+    // fused step, so a pending state always sits in the same places.
+    bool pending = false;
+    bool pending_checked = false;   // d_scal holds the checks of the pending step
+    double pend_dt = 0.0;           // dt of the pending step
+
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -267,7 +278,9 @@ template <class R> void launch_y_solve_2d(fen_ctx *c, double inv_n);
 template <class R> void launch_z_solve(fen_ctx *c, double inv_n);
 template <class R> void launch_thomas(fen_ctx *c);
 template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img);
-template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out);
+// store == false: the checks of the projection only (max div, max |u|+|v|+|w|
+// of the projected velocity), nothing written but the reduction (3-D).
+template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out, bool store = true);
 bool fused_z_walls(const fen_ctx *c);
 void mf_advect_vof(fen_ctx *c, double dt);
 void mf_update_fields(fen_ctx *c, bool with_phat);
@@ -294,6 +307,8 @@ void advance_fp64(fen_ctx *c, double dt);
 // nsteps FP64 steps as fen_ns_run takes them (fused when the case allows;
 // bitwise equal to nsteps calls of advance_fp64).
 void run_fp64(fen_ctx *c, int nsteps, double dt);
+// Completes a pending step (the state is then where a single step leaves it).
+void flush_state(fen_ctx *c);
 
 // Discrete adjoint (adjoint.hip).
 const char *adjoint_unsupported(const fen_ctx *c);   // nullptr when the context is supported

@@ -195,7 +195,8 @@ template <class R> struct ProjArgs {
 // the cell and usm, vsm, wsm are u*, v*, w* one cell below along their axis.
 // Arithmetic is FP64; each stored value is rounded once to R, and the
 // checks see the stored (rounded) values, as a later read would.
-template <int D3, class R, class FF, class IW>
+// ST == 0: the checks only (nothing is stored; p0 is not used).
+template <int D3, class R, class FF, class IW, int ST = 1>
 __device__ __forceinline__ void project_cell(const ProjArgs<R> &a, int i, int j, int k, FF F, double us0, double vs0,
                                              double ws0, double p0, double usm, double vsm, double wsm, double &md,
                                              double &mv, double &bad, IW IMG)
@@ -206,19 +207,25 @@ __device__ __forceinline__ void project_cell(const ProjArgs<R> &a, int i, int j,
     const double f0 = F(0, 0, 0);
     const R un = (R)(us0 - dr((F(1, 0, 0) - f0) * a.id * a.dt));
     const R vn = (R)(vs0 - dr((F(0, 1, 0) - f0) * a.id * a.dt));
-    a.u[g] = un;
-    a.v[g] = vn;
-    IMG(a.u, un);
-    IMG(a.v, vn);
+    if constexpr (ST) {
+        a.u[g] = un;
+        a.v[g] = vn;
+        IMG(a.u, un);
+        IMG(a.v, vn);
+    }
     R wn = R(0.0);
     if constexpr (D3) {
         wn = (R)(ws0 - dr((F(0, 0, 1) - f0) * a.id * a.dt));
-        a.w[g] = wn;
-        IMG(a.w, wn);
+        if constexpr (ST) {
+            a.w[g] = wn;
+            IMG(a.w, wn);
+        }
     }
-    const R pn = (R)(p0 + f0);
-    a.p[g] = pn;
-    IMG(a.p, pn);
+    if constexpr (ST) {
+        const R pn = (R)(p0 + f0);
+        a.p[g] = pn;
+        IMG(a.p, pn);
+    }
 
     // checks on the values the reference sees after its ghost fill
     const long pjk = j + (long)(b.ny + 2) * k, pik = i + (long)(b.nx + 2) * k, pij = i + (long)(b.nx + 2) * j;

@@ -454,8 +454,9 @@ __global__ __launch_bounds__(kFinalThreads) void checks_final_kernel(const doubl
     if (threadIdx.x == 0) { out[0] = sd[0]; out[1] = sv[0]; out[2] = sf[0]; }
 }
 
-template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out)
+template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out, bool store)
 {
+    if (!store && c->box.dim != 3) throw std::runtime_error("checks-only projection is 3-D");
     ProjArgs<R> a{};
     a.us = as<R>(c->d_us[0]); a.vs = as<R>(c->d_us[1]); a.ws = as<R>(c->d_us[2]); a.phi = as<R>(c->d_phi);
     a.u = as<R>(c->d_u[0]); a.v = as<R>(c->d_u[1]); a.w = as<R>(c->d_u[2]); a.p = as<R>(c->d_p);
@@ -488,21 +489,28 @@ template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out
     if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
     const bool sh = c->shift_active;
     const int d3 = c->box.dim == 3;
-    prof_begin(c, K_PROJECT);
+    const int kid = store ? K_PROJECT : K_CHECKS;
+    prof_begin(c, kid);
     if (d3) {
         const dim3 grid((unsigned)nblk);
-        if (sh)
-            hipLaunchKernelGGL((project_march_kernel<R, 1, SH::PTY, SH::PKB, SH::PUN, SH::PPF, SH::POCC, SH::PTXW>),
-                               grid, dim3(SH::PTXW * SH::PTY), 0, c->stream, a);
-        else
-            hipLaunchKernelGGL((project_march_kernel<R, 0, SH::PTY, SH::PKB, SH::PUN, SH::PPF, SH::POCC, SH::PTXW>),
-                               grid, dim3(SH::PTXW * SH::PTY), 0, c->stream, a);
+#define FEN_PJ(SH_, ST_)                                                                                            \
+    hipLaunchKernelGGL((project_march_kernel<R, SH_, SH::PTY, SH::PKB, SH::PUN, SH::PPF, SH::POCC, SH::PTXW, ST_>), \
+                       grid, dim3(SH::PTXW * SH::PTY), 0, c->stream, a)
+        if (store) {
+            if (sh) FEN_PJ(1, 1);
+            else FEN_PJ(0, 1);
+        } else {
+            if (sh) FEN_PJ(1, 0);
+            else FEN_PJ(0, 0);
+        }
+#undef FEN_PJ
     } else {
         const dim3 grid((unsigned)a.nbx, (unsigned)a.nby);
         if (sh) hipLaunchKernelGGL((project_kernel_2d<R, 1>), grid, dim3(kThreads), 0, c->stream, a);
         else hipLaunchKernelGGL((project_kernel_2d<R, 0>), grid, dim3(kThreads), 0, c->stream, a);
     }
-    prof_end(c, K_PROJECT, (d3 ? 9.0 : 7.0) * sizeof(R) * c->n_int);
+    // the checks alone read u*, v*, w* and phi
+    prof_end(c, kid, (store ? (d3 ? 9.0 : 7.0) : 4.0) * sizeof(R) * c->n_int);
     prof_begin(c, K_REDUCE);
     hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(kFinalThreads), 0, c->stream, (const double *)c->d_part,
                        nblk, checks_out);
@@ -655,8 +663,8 @@ template void launch_ghost_desc<double>(fen_ctx *, const GhostDesc<double> *, in
 template void launch_ghost_desc<float>(fen_ctx *, const GhostDesc<float> *, int);
 template void launch_momentum<double>(fen_ctx *, double, double, double);
 template void launch_momentum<float>(fen_ctx *, double, double, double);
-template void launch_project<double>(fen_ctx *, double, double *);
-template void launch_project<float>(fen_ctx *, double, double *);
+template void launch_project<double>(fen_ctx *, double, double *, bool);
+template void launch_project<float>(fen_ctx *, double, double *, bool);
 template void launch_project_momentum<double>(fen_ctx *, double, double, double, double, void *const *, const void *,
                                               void *const *, void *, double *);
 template void launch_project_momentum<float>(fen_ctx *, double, double, double, double, void *const *, const void *,

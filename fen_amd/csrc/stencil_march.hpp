@@ -194,7 +194,8 @@ __global__ __launch_bounds__(TX * TY, OCC) void momentum_march_kernel(MomArgs<R>
 // Ring fields: phi, u*, v* (u* and v* need the cell below in i and j for the
 // checks); w* and p are prefetched one plane ahead, w* one plane below is
 // carried in a register.
-template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1, int TX = 64>
+// ST == 0: the checks of the projection only; nothing is stored and p is not read.
+template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1, int TX = 64, int ST = 1>
 struct ProjectMarch {
     static_assert((PF == 1 && (UNROLL == 1 || UNROLL == 3)) || (PF == 2 && UNROLL == 2), "launch shape");
     using S = TileShape<TY, TX>;
@@ -214,7 +215,7 @@ struct ProjectMarch {
         const long g = a.b.g(i, j, k);
         if (active && k + 1 <= kend) {
             wnext = a.ws[g + a.b.gxy];
-            pnext = a.p[g + a.b.gxy];
+            if constexpr (ST) pnext = a.p[g + a.b.gxy];
         }
         if (active) {
             auto L = [&](int f, int di, int dj, int dk) -> double {
@@ -226,8 +227,8 @@ struct ProjectMarch {
                 else return L(0, di, dj, dk);
             };
             auto img = [&](R *f, R v) { march_images(f, a.b, a.img, i, j, k, v, xedge); };
-            project_cell<1>(a, i, j, k, F, L(1, 0, 0, 0), L(2, 0, 0, 0), wsc, pc, L(1, -1, 0, 0), L(2, 0, -1, 0),
-                            wsm, md, mv, bad, img);
+            project_cell<1, R, decltype(F), decltype(img), ST>(a, i, j, k, F, L(1, 0, 0, 0), L(2, 0, 0, 0), wsc, pc,
+                                                              L(1, -1, 0, 0), L(2, 0, -1, 0), wsm, md, mv, bad, img);
         }
         wsm = wsc;
         wsc = wnext;
@@ -267,7 +268,7 @@ struct ProjectMarch {
         if (active) {
             wsm = a.ws[b.g(i, j, k0 - 1)];
             wsc = a.ws[b.g(i, j, k0)];
-            pc = a.p[b.g(i, j, k0)];
+            if constexpr (ST) pc = a.p[b.g(i, j, k0)];
         }
         if constexpr (PF == 2) {
             int sm = 0, sc = 1, sn = 2;
@@ -305,11 +306,11 @@ struct ProjectMarch {
     }
 };
 
-template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1, int OCC = 1, int TX = 64>
+template <class R, int SHIFT, int TY, int KB, int UNROLL, int PF = 1, int OCC = 1, int TX = 64, int ST = 1>
 __global__ __launch_bounds__(TX * TY, OCC) void project_march_kernel(ProjArgs<R> a)
 {
     __shared__ R lds[ring_size<TileShape<TY, TX>, 3>()];
-    ProjectMarch<R, SHIFT, TY, KB, UNROLL, PF, TX>::run(a, lds);
+    ProjectMarch<R, SHIFT, TY, KB, UNROLL, PF, TX, ST>::run(a, lds);
 }
 
 // ------------------------------------------------- projection + momentum

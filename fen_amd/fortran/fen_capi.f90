@@ -62,7 +62,8 @@ module fen_capi
                 bind(C, name = 'fen_get_scalars')
             import :: c_ptr, c_int, c_double
             type(c_ptr), value :: ctx
-            real(c_double), intent(out) :: maxdiv, maxcfl, dt_o, dt_visc, dt_conv
+            ! an absent argument is passed as NULL (not read back)
+            real(c_double), intent(out), optional :: maxdiv, maxcfl, dt_o, dt_visc, dt_conv
         end function
 
         integer(c_int) function fen_upload(ctx, field, host, count) bind(C, name = 'fen_upload')
@@ -106,7 +107,8 @@ module fen_capi
             type(c_ptr), value :: ctx
             integer(c_int), value :: step
             real(c_double), intent(inout) :: dt
-            real(c_double), intent(out) :: maxdiv, maxcfl
+            ! absent: the step is queued without waiting for its checks
+            real(c_double), intent(out), optional :: maxdiv, maxcfl
         end function
 
         integer(c_int) function fen_ns_run(ctx, step0, nsteps, dt) bind(C, name = 'fen_ns_run')

@@ -697,9 +697,17 @@ module navier_stokes_mod
     !> Arithmetic of the GPU step: FEN_FP64 (default) or FEN_FP32_REFINE.
     integer(c_int) :: fen_precision = FEN_FP64
 
+    !> .true.: navier_stokes_solver queues the step without waiting for its
+    !> checks; maxdiv and maxCFL are then fetched by print_solver_status or
+    !> fen_sync_checks() (no counterpart in the reference, whose values are
+    !> always current).  .false. (default): both are current after each step.
+    logical :: lazy_checks = .false.
+
     type(c_ptr) :: fen_ctx = c_null_ptr
     logical, private :: host_modified = .true.   ! host mirrors hold data the device lacks
     logical, private :: device_ahead = .false.   ! device holds data the host mirrors lack
+    logical, private :: checks_stale = .false.   ! maxdiv, maxCFL not fetched since the last step
+    logical, private :: source_resident = .false.   ! S has been uploaded once
 
 contains
 
@@ -853,6 +861,7 @@ contains
     !> Host mirrors -> HBM (fields, boundary types and values, source term).
     subroutine fen_sync_device()
         integer(c_int) :: fsrc
+        logical :: nonzero
         call upload_bc(p)
         call upload_bc(phi)
         call upload_bc(v%x)
@@ -874,7 +883,14 @@ contains
             call fen_check(fen_ctx, fen_update_ghost_nodes(fen_ctx, FEN_VOF), 'vof ghost nodes')
         end if
 #endif
-        if (any(S%x%f /= 0.0_dp) .or. any(S%y%f /= 0.0_dp)) then
+        ! S is uploaded once it is non-zero in any component, and then every
+        ! time, so that a source switched off reaches the device as zeros
+        nonzero = any(S%x%f /= 0.0_dp) .or. any(S%y%f /= 0.0_dp)
+#if DIM==3
+        nonzero = nonzero .or. any(S%z%f /= 0.0_dp)
+#endif
+        if (nonzero .or. source_resident) then
+            source_resident = .true.
             fsrc = FEN_S_X
             call fen_check(fen_ctx, fen_upload(fen_ctx, fsrc, S%x%f, int(size(S%x%f), c_size_t)), 'S')
             fsrc = FEN_S_Y
@@ -951,11 +967,17 @@ contains
         call push_params()
         call fen_check(fen_ctx, fen_set_dt_o(fen_ctx, dt_o), 'fen_set_dt_o')
         d = dt
-        call fen_check(fen_ctx, fen_ns_step(fen_ctx, int(step, c_int), d, md, mc), 'navier_stokes_solver')
+        if (lazy_checks) then
+            call fen_check(fen_ctx, fen_ns_step(fen_ctx, int(step, c_int), d), 'navier_stokes_solver')
+            checks_stale = .true.
+        else
+            call fen_check(fen_ctx, fen_ns_step(fen_ctx, int(step, c_int), d, md, mc), 'navier_stokes_solver')
+            maxdiv = md
+            maxCFL = mc
+            checks_stale = .false.
+        end if
         dt = d
-        maxdiv = md
-        maxCFL = mc
-        call fen_check(fen_ctx, fen_get_scalars(fen_ctx, md, mc, dto, dtv, dtc), 'fen_get_scalars')
+        call fen_check(fen_ctx, fen_get_scalars(fen_ctx, dt_o=dto, dt_visc=dtv, dt_conv=dtc), 'fen_get_scalars')
         dt_o = dto
         if (constant_CFL) then
             dt_conv = dtc
@@ -969,10 +991,10 @@ contains
         type(grid), intent(in) :: comp_grid
         real(dp), intent(in) :: U
         real(dp), intent(inout) :: dt
-        real(c_double) :: d, md, mc, dto, dtv, dtc
+        real(c_double) :: d, dto, dtv, dtc
         call push_params()
         call fen_check(fen_ctx, fen_set_timestep(fen_ctx, U, d), 'set_timestep')
-        call fen_check(fen_ctx, fen_get_scalars(fen_ctx, md, mc, dto, dtv, dtc), 'fen_get_scalars')
+        call fen_check(fen_ctx, fen_get_scalars(fen_ctx, dt_o=dto, dt_visc=dtv, dt_conv=dtc), 'fen_get_scalars')
         dt = d
         dt_o = dto
         dt_visc = dtv
@@ -980,10 +1002,21 @@ contains
         if (comp_grid%Nx < 0) print *, 'invalid grid'
     end subroutine set_timestep
 
+    !> Fetches maxdiv and maxCFL of the last step (waits for it).
+    subroutine fen_sync_checks()
+        real(c_double) :: md, mc
+        if (.not. checks_stale) return
+        call fen_check(fen_ctx, fen_get_scalars(fen_ctx, maxdiv=md, maxcfl=mc), 'fen_get_scalars')
+        maxdiv = md
+        maxCFL = mc
+        checks_stale = .false.
+    end subroutine fen_sync_checks
+
     subroutine print_navier_stokes_solver_status(log_id, step, time, dt)
         use global_mod, only : myrank
         integer, intent(in) :: log_id, step
         real(dp), intent(in) :: time, dt
+        call fen_sync_checks()
         if (myrank == 0) write(log_id, 10) 'step: ', step, 'time: ', time, 'dt: ', dt, &
              'maxdiv: ', maxdiv, 'maxCFL: ', maxCFL
 10      format(A6,I7,1x,A6,E13.6,1x,A4,E13.6,1x,A8,E13.6,1x,A9,1x,E13.6)
@@ -1008,6 +1041,8 @@ contains
 #endif
         if (c_associated(fen_ctx)) rc = fen_destroy(fen_ctx)
         fen_ctx = c_null_ptr
+        source_resident = .false.
+        checks_stale = .false.
         poisson_shared_ctx = c_null_ptr
         scalar_sync_hook => Null()
     end subroutine destroy_navier_stokes_solver

@@ -10,27 +10,14 @@ libfen on the GPU.
 import numpy as np
 import torch
 
-from fen_amd._lib import DVO_X, DVO_Y, DVO_Z, P, U, V, W
-from tests.common import ghosted
+from fen_amd._lib import U, V, W
+from fen_amd.cases import interior_velocity as final_velocity
+from fen_amd.cases import load_state
 
 
 class Run:
     def __init__(self, solver, state, nsteps, every, dt, dt_o):
         self.solver, self.state, self.nsteps, self.every, self.dt, self.dt_o = solver, state, nsteps, every, dt, dt_o
-
-
-def load_state(solver, state, dt_o):
-    """Upload interior fields (u, v, w, p ghosted with their ghosts filled; dv_o zero unless given)."""
-    for fid in (U, V, W, P):
-        solver.upload(fid, ghosted(np.asfortranarray(state[fid]), 3))
-        solver.update_ghost_nodes(fid)
-    for fid in (DVO_X, DVO_Y, DVO_Z):
-        solver.upload(fid, np.asfortranarray(state.get(fid, np.zeros(solver.n))))
-    solver.set_dt_o(dt_o)
-
-
-def final_velocity(solver):
-    return [np.ascontiguousarray(solver.download(f)[1:-1, 1:-1, 1:-1]) for f in (U, V, W)]
 
 
 class NSRun(torch.autograd.Function):

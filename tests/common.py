@@ -30,14 +30,7 @@ def make_oracle(case):
 
 
 def make_gpu(case, precision=0):
-    from fen_amd import Solver
-    s = Solver(case.dim, case.n, case.L, bc=case.bc, precision=precision)
-
-    def up(fid, arr):
-        s.upload(fid, ghosted(arr, case.dim) if fid <= W else arr)
-
-    case.apply(s, up)
-    return s
+    return cases.make_solver(case, precision)
 
 
 def live(a, dim):
