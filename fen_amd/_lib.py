@@ -81,6 +81,10 @@ def load():
         "fen_set_multiphase": (c_int, [vp, c_double, c_double, c_double, c_double, c_double, c_double]),
         "fen_update_material_properties": (c_int, [vp]),
         "fen_advect_vof": (c_int, [vp, c_double]),
+        "fen_status_text": (c_int, [c_int, c_double, c_double, c_double, c_double, ctypes.c_char_p, c_size_t]),
+        "fen_grid_json_text": (c_int, [c_int, c_int, c_int, dp, c_double, c_double, c_double, ctypes.c_char_p,
+                                       c_size_t]),
+        "fen_case_read": (c_int, [ctypes.c_char_p, vp, ctypes.c_char_p, c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

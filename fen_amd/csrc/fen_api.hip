@@ -283,31 +283,6 @@ void free_all(fen_ctx *c)
 
 void sync(fen_ctx *c) { FEN_HIP_CHECK(hipStreamSynchronize(c->stream)); }
 
-// Fortran Ew.d edit descriptor with d = 6 or 8 (0.ddddddE+xx), right-justified.
-std::string fortran_e(double x, int w, int d)
-{
-    char buf[64];
-    if (std::isnan(x)) { snprintf(buf, sizeof buf, "%*s", w, "NaN"); return buf; }
-    if (std::isinf(x)) { snprintf(buf, sizeof buf, "%*s", w, x > 0 ? "Infinity" : "-Infinity"); return buf; }
-    std::string body;
-    if (x == 0.0) {
-        body = std::string(std::signbit(x) ? "-" : "") + "0." + std::string(d, '0') + "E+00";
-    } else {
-        char tmp[64];
-        snprintf(tmp, sizeof tmp, "%.*E", d - 1, std::fabs(x));   // d.ddddE+xx
-        std::string s(tmp);
-        const size_t epos = s.find('E');
-        const int e = std::atoi(s.c_str() + epos + 1) + 1;
-        std::string digits = s.substr(0, 1) + s.substr(2, epos - 2);
-        char ex[16];
-        if (std::abs(e) <= 99) snprintf(ex, sizeof ex, "E%c%02d", e < 0 ? '-' : '+', std::abs(e));
-        else snprintf(ex, sizeof ex, "%c%03d", e < 0 ? '-' : '+', std::abs(e));
-        body = std::string(x < 0 ? "-" : "") + "0." + digits + ex;
-    }
-    snprintf(buf, sizeof buf, "%*s", w, body.c_str());
-    return buf;
-}
-
 // Whether the solver subtracts the mean of phi: pp, ppn and pn
 // (src/poisson.f90:397-408, 495-503, 1159-1171); npn, nnn and nn do not.
 bool removes_mean(const fen_ctx *c)
@@ -1154,37 +1129,19 @@ int fen_format_status(fen_ctx *c, int step, double time, double dt, char *buf, s
     double md = 0.0, mc = 0.0;
     int rc = fen_get_scalars(c, &md, &mc, nullptr, nullptr, nullptr);
     if (rc != FEN_OK && rc != FEN_ERR_NONFINITE) return rc;
-    // (A6,I7,1x,A6,E13.6,1x,A4,E13.6,1x,A8,E13.6,1x,A9,1x,E13.6), src/navier_stokes.f90:746
-    std::string s = "step: ";
-    char ib[32];
-    snprintf(ib, sizeof ib, "%7d", step);
-    s += ib;
-    s += " time: " + fortran_e(time, 13, 6);
-    s += " dt: " + fortran_e(dt, 13, 6);
-    s += " maxdiv: " + fortran_e(md, 13, 6);
-    s += "  maxCFL:  " + fortran_e(mc, 13, 6);
-    snprintf(buf, len, "%s", s.c_str());
-    return rc;
+    const int rt = fen_status_text(step, time, dt, md, mc, buf, len);
+    return rt != FEN_OK ? rt : rc;
 }
 
 int fen_write_grid_json(const fen_ctx *c, const char *path)
 {
     if (!c || !path) return FEN_ERR_ARG;
+    char text[1024];
+    const int rc = fen_grid_json_text(c->nx, c->ny, c->nz, c->x0, c->lx, c->ly, c->lz, text, sizeof text);
+    if (rc != FEN_OK) return rc;
     FILE *fp = std::fopen(path, "w");
     if (!fp) return FEN_ERR_ARG;
-    // src/grid.f90:246-258
-    std::fprintf(fp, "{\n");
-    std::fprintf(fp, "    \"Grid\": {\n");
-    std::fprintf(fp, "        \"Nx\":  %7d,\n", c->nx);
-    std::fprintf(fp, "        \"Ny\":  %7d,\n", c->ny);
-    std::fprintf(fp, "        \"Nz\":  %7d,\n", c->nz);
-    std::fprintf(fp, "        \"origin\": [ %s,%s,%s],\n", fortran_e(c->x0[0], 16, 8).c_str(),
-                 fortran_e(c->x0[1], 16, 8).c_str(), fortran_e(c->x0[2], 16, 8).c_str());
-    std::fprintf(fp, "        \"Lx\":  %s,\n", fortran_e(c->lx, 16, 8).c_str());
-    std::fprintf(fp, "        \"Ly\":  %s,\n", fortran_e(c->ly, 16, 8).c_str());
-    std::fprintf(fp, "        \"Lz\":  %s\n", fortran_e(c->lz, 16, 8).c_str());
-    std::fprintf(fp, "      }\n");
-    std::fprintf(fp, "}\n");
+    std::fputs(text, fp);
     std::fclose(fp);
     return FEN_OK;
 }

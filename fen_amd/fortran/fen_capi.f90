@@ -13,6 +13,24 @@ module fen_capi
     integer(c_int), parameter :: FEN_VOF = 11, FEN_RHO = 12, FEN_MU = 13
     integer(c_int), parameter :: FEN_BC_PERIODIC = 0, FEN_BC_WALL = 1, FEN_BC_INFLOW = 2, FEN_BC_OUTFLOW = 3
     integer(c_int), parameter :: FEN_FP64 = 0, FEN_FP32_REFINE = 1
+    integer, parameter :: FEN_CASE_MAXLIST = 32, FEN_CASE_PATHLEN = 256
+
+    !> A case file's contents (struct fen_case of include/fen.h, member for member).
+    type, bind(C) :: fen_case
+        integer(c_int) :: dim, n(3), bc(6), constant_cfl, precision, restart, steps, status_every
+        integer(c_int) :: n_save_state, save_state(FEN_CASE_MAXLIST)
+        integer(c_int) :: n_save_fields, save_fields(FEN_CASE_MAXLIST)
+        integer(c_int) :: write_final, multiphase
+        real(c_double) :: origin(3), L(3)
+        real(c_double) :: density, viscosity, g(3), cfl
+        real(c_double) :: dt, U
+        real(c_double) :: rho0, rho1, mu0, mu1, sigma
+        real(c_double) :: iface(3)
+        character(kind=c_char) :: name(64)
+        character(kind=c_char) :: data_dir(FEN_CASE_PATHLEN)
+        character(kind=c_char) :: restart_dir(FEN_CASE_PATHLEN)
+        character(kind=c_char) :: init(FEN_CASE_PATHLEN, 5)   ! u, v, w, p, vof
+    end type fen_case
 
     interface
 
@@ -146,6 +164,14 @@ module fen_capi
         integer(c_int) function fen_synchronize(ctx) bind(C, name = 'fen_synchronize')
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function fen_case_read(path, out, err, errlen) bind(C, name = 'fen_case_read')
+            import :: c_int, c_char, c_size_t, fen_case
+            character(kind=c_char), intent(in) :: path(*)
+            type(fen_case), intent(out) :: out
+            character(kind=c_char), intent(out) :: err(*)
+            integer(c_size_t), value :: errlen
         end function
 
     end interface

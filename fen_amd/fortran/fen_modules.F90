@@ -1233,8 +1233,9 @@ contains
     end subroutine init_solver
 
     !> Cell-centred fields data/vx_<step>.raw, vy_, [vz_,] p_ (raw float64
-    !> interiors; reference src/solver.f90:103-156).
-    subroutine save_fields(step)
+    !> interiors; reference src/solver.f90:103-156).  dir replaces data/
+    !> (no counterpart in the reference, whose folder is fixed).
+    subroutine save_fields(step, dir)
         use scalar_mod, only : scalar
         use fields_mod, only : face_to_center
         use navier_stokes_mod, only : v, p, fen_sync_host
@@ -1242,38 +1243,47 @@ contains
         use navier_stokes_mod, only : vof
 #endif
         integer, intent(in) :: step
+        character(len=*), intent(in), optional :: dir
         type(scalar) :: temp
         character(len=7) :: sn
+        character(len=:), allocatable :: d
+        d = 'data'
+        if (present(dir)) d = trim(dir)
         call fen_sync_host()
         call temp%allocate(v%G, v%x%gl)
         write(sn, '(I0.7)') step
         call face_to_center(v%x, temp, 'x')
-        call temp%write('data/vx_'//sn//'.raw')
+        call temp%write(d//'/vx_'//sn//'.raw')
         call face_to_center(v%y, temp, 'y')
-        call temp%write('data/vy_'//sn//'.raw')
+        call temp%write(d//'/vy_'//sn//'.raw')
 #if DIM==3
         call face_to_center(v%z, temp, 'z')
-        call temp%write('data/vz_'//sn//'.raw')
+        call temp%write(d//'/vz_'//sn//'.raw')
 #endif
-        call p%write('data/p_'//sn//'.raw')
+        call p%write(d//'/p_'//sn//'.raw')
 #ifdef MF
-        call vof%write('data/vof_'//sn//'.raw')
+        call vof%write(d//'/vof_'//sn//'.raw')
 #endif
         call temp%destroy()
     end subroutine save_fields
 
     !> Restart file data/state_<step>.raw: p, u, v, dv_o%x, dv_o%y [, w, dv_o%z],
     !> each the raw float64 interior in Fortran order (src/solver.f90:160-240).
-    subroutine save_state(step)
+    !> dir replaces data/ (no counterpart in the reference).
+    subroutine save_state(step, dir)
         use navier_stokes_mod, only : v, dv_o, p, fen_sync_host
         integer, intent(in) :: step
+        character(len=*), intent(in), optional :: dir
         integer :: u, lo(3), hi(3)
         character(len=7) :: sn
+        character(len=:), allocatable :: d
+        d = 'data'
+        if (present(dir)) d = trim(dir)
         call fen_sync_host()
         lo = p%G%lo
         hi = p%G%hi
         write(sn, '(I0.7)') step
-        open(newunit = u, file = 'data/state_'//sn//'.raw', access = 'stream', form = 'unformatted', &
+        open(newunit = u, file = d//'/state_'//sn//'.raw', access = 'stream', form = 'unformatted', &
              status = 'replace')
         write(u) p%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
         write(u) v%x%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
@@ -1287,16 +1297,20 @@ contains
         close(u)
     end subroutine save_state
 
-    subroutine load_state(step)
+    subroutine load_state(step, dir)
         use navier_stokes_mod, only : v, dv_o, p, fen_sync_host, fen_host_modified
         integer, intent(in) :: step
+        character(len=*), intent(in), optional :: dir
         integer :: u, lo(3), hi(3)
         character(len=7) :: sn
+        character(len=:), allocatable :: d
+        d = 'data'
+        if (present(dir)) d = trim(dir)
         call fen_sync_host()
         lo = p%G%lo
         hi = p%G%hi
         write(sn, '(I0.7)') step
-        open(newunit = u, file = 'data/state_'//sn//'.raw', access = 'stream', form = 'unformatted', &
+        open(newunit = u, file = d//'/state_'//sn//'.raw', access = 'stream', form = 'unformatted', &
              status = 'old')
         read(u) p%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
         read(u) v%x%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
@@ -1323,3 +1337,219 @@ contains
     end subroutine destroy_solver
 
 end module solver_mod
+
+!===================================================================================================
+module case_mod
+    ! Set-up and time loop of a run from a case file (a JSON description read
+    ! by libfen's fen_case_read; format in include/fen.h).  setup_case does
+    ! what a FEN program's own set-up code does: grid%setup, init_solver,
+    ! navier_stokes_mod's module variables, the initial fields or a restart,
+    ! set_timestep.  run_case is the program's time loop: advance_solution,
+    ! print_solver_status, save_state and save_fields at the listed steps.
+    ! No counterpart in the reference, which has no input file.
+    use iso_c_binding, only : c_char, c_null_char, c_size_t, c_int
+    use precision_mod, only : dp
+    use fen_capi, only : fen_case, fen_case_read, FEN_OK, FEN_FP64, FEN_BC_PERIODIC, FEN_BC_WALL, &
+                         FEN_BC_INFLOW, FEN_BC_OUTFLOW
+    implicit none
+    private
+    public :: fen_case, read_case, setup_case, run_case, case_text
+
+    !> The case being set up (the two-phase interface function reads it).
+    type(fen_case), save :: current
+
+contains
+
+    !> A NUL-terminated C text as a Fortran string.
+    function case_text(chars) result(s)
+        character(kind=c_char), intent(in) :: chars(:)
+        character(len=:), allocatable :: s
+        integer :: n, i
+        n = 0
+        do while (n < size(chars))
+            if (chars(n + 1) == c_null_char) exit
+            n = n + 1
+        end do
+        allocate(character(len=n) :: s)
+        do i = 1, n
+            s(i:i) = chars(i)
+        end do
+    end function case_text
+
+    !> Reads a case file; stops with the reader's message if it is not valid.
+    subroutine read_case(path, cs)
+        use IO_mod, only : print_error_message
+        character(len=*), intent(in) :: path
+        type(fen_case), intent(out) :: cs
+        character(kind=c_char) :: err(512)
+        integer(c_int) :: rc
+        rc = fen_case_read(trim(path)//c_null_char, cs, err, int(size(err), c_size_t))
+        if (rc /= FEN_OK) then
+            call print_error_message(case_text(err))
+            error stop 1
+        end if
+    end subroutine read_case
+
+    function bc_name(code) result(s)
+        integer(c_int), intent(in) :: code
+        character(len=:), allocatable :: s
+        select case (code)
+        case (FEN_BC_WALL)
+            s = 'Wall'
+        case (FEN_BC_INFLOW)
+            s = 'Inflow'
+        case (FEN_BC_OUTFLOW)
+            s = 'Outflow'
+        case default
+            s = 'Periodic'
+        end select
+    end function bc_name
+
+#ifdef MF
+    !> Signed distance y - level - amplitude cos(2 pi (x - x0) / wavelength)
+    !> of the case's interface (positive in the phase above it).
+    function case_interface(x, y) result(d)
+        use global_mod, only : pi
+        real(dp), intent(in) :: x, y
+        real(dp) :: d
+        d = y - current%iface(2)*cos(2.0_dp*pi*(x - current%origin(1))/current%iface(3)) - current%iface(1)
+    end function case_interface
+#endif
+
+    !> Sets a run up as the case describes.  On return the fields hold the
+    !> initial state (or the restart state of step step0), dt is the time
+    !> step and dt_o is set.
+    subroutine setup_case(cs, comp_grid, dt, step0)
+        use IO_mod, only : print_error_message
+        use global_mod, only : Ndim
+        use grid_mod, only : grid, bc_type
+        use solver_mod, only : init_solver, load_state
+        use navier_stokes_mod, only : density, viscosity, g, CFL, constant_CFL, fen_precision, v, p, &
+                                      set_timestep, dt_o, fen_host_modified
+#ifdef MF
+        use navier_stokes_mod, only : rho, mu
+        use multiphase_mod, only : rho_0, rho_1, mu_0, mu_1, sigma, update_material_properties
+        use volume_of_fluid_mod, only : distance, vof
+#endif
+        type(fen_case), intent(in) :: cs
+        type(grid), intent(inout), target :: comp_grid
+        real(dp), intent(out) :: dt
+        integer, intent(out) :: step0
+        type(bc_type) :: bc(Ndim*2)
+        integer :: f
+
+        current = cs
+        if (cs%dim /= Ndim) then
+            call print_error_message('case file: its grid is not of this build''s dimension (DIM)')
+            error stop 1
+        end if
+#ifdef MF
+        if (cs%multiphase == 0) then
+            call print_error_message('case file: the two-phase build needs a "Multiphase" object')
+            error stop 1
+        end if
+#else
+        if (cs%multiphase /= 0) then
+            call print_error_message('case file: "Multiphase" needs the two-phase build (-DMF)')
+            error stop 1
+        end if
+#endif
+        do f = 1, Ndim*2
+            bc(f)%s = bc_name(cs%bc(f))
+        end do
+        comp_grid%name = case_text(cs%name)
+        call comp_grid%setup(cs%n(1), cs%n(2), cs%n(3), cs%L(1), cs%L(2), cs%L(3), cs%origin, 1, 1, bc)
+
+        density = cs%density
+        viscosity = cs%viscosity
+        g = cs%g
+        CFL = cs%cfl
+        constant_CFL = cs%constant_cfl /= 0
+        fen_precision = cs%precision
+#ifdef MF
+        rho_0 = cs%rho0
+        rho_1 = cs%rho1
+        mu_0 = cs%mu0
+        mu_1 = cs%mu1
+        sigma = cs%sigma
+        distance => case_interface
+#endif
+        call init_solver(comp_grid)
+
+        step0 = 0
+        if (cs%restart >= 0) then
+            call load_state(cs%restart, case_text(cs%restart_dir))
+            step0 = cs%restart
+        else
+            if (len(case_text(cs%init(:, 1))) > 0) call v%x%read(case_text(cs%init(:, 1)))
+            if (len(case_text(cs%init(:, 2))) > 0) call v%y%read(case_text(cs%init(:, 2)))
+#if DIM==3
+            if (len(case_text(cs%init(:, 3))) > 0) call v%z%read(case_text(cs%init(:, 3)))
+#endif
+            if (len(case_text(cs%init(:, 4))) > 0) call p%read(case_text(cs%init(:, 4)))
+            call p%update_ghost_nodes()
+            call v%update_ghost_nodes()
+            call fen_host_modified()
+        end if
+#ifdef MF
+        if (len(case_text(cs%init(:, 5))) > 0) then
+            call vof%read(case_text(cs%init(:, 5)))
+            call vof%update_ghost_nodes()
+            call fen_host_modified()
+            call update_material_properties(rho, mu, vof)
+        end if
+#endif
+        if (cs%U > 0.0_dp) then
+            call set_timestep(comp_grid, dt, cs%U)
+        else
+            dt = cs%dt
+            dt_o = dt
+        end if
+    end subroutine setup_case
+
+    !> The time loop: steps step0+1 .. step0+steps with the status line every
+    !> status_every steps, save_state and save_fields at the listed steps
+    !> (into the case's data_dir), and the final interiors (u.raw, v.raw,
+    !> [w.raw,] p.raw [, vof.raw]) when write_final is set.  time starts at
+    !> step0 * dt.
+    subroutine run_case(cs, comp_grid, dt, step0)
+        use IO_mod, only : stdout
+        use grid_mod, only : grid
+        use solver_mod, only : advance_solution, print_solver_status, save_state, save_fields
+        use navier_stokes_mod, only : v, p
+#ifdef MF
+        use volume_of_fluid_mod, only : vof
+#endif
+        type(fen_case), intent(in) :: cs
+        type(grid), intent(in) :: comp_grid
+        real(dp), intent(inout) :: dt
+        integer, intent(in) :: step0
+        character(len=:), allocatable :: dir
+        integer :: step
+        real(dp) :: time
+
+        dir = case_text(cs%data_dir)
+        time = step0*dt
+        do step = step0 + 1, step0 + cs%steps
+            time = time + dt
+            call advance_solution(comp_grid, step, dt)
+            if (cs%status_every > 0) then
+                if (mod(step - step0, cs%status_every) == 0) call print_solver_status(stdout, step, time, dt)
+            end if
+            if (any(cs%save_state(1:cs%n_save_state) == step)) call save_state(step, dir)
+            if (any(cs%save_fields(1:cs%n_save_fields) == step)) call save_fields(step, dir)
+        end do
+        if (cs%write_final /= 0) then
+            call v%x%write('u.raw')
+            call v%y%write('v.raw')
+#if DIM==3
+            call v%z%write('w.raw')
+#endif
+            call p%write('p.raw')
+#ifdef MF
+            call vof%write('vof.raw')
+#endif
+        end if
+    end subroutine run_case
+
+end module case_mod

@@ -193,6 +193,78 @@ int fen_format_status(fen_ctx *ctx, int step, double time, double dt, char *buf,
 /* Writes grid.json exactly as grid%print_json (src/grid.f90:233-265). */
 int fen_write_grid_json(const fen_ctx *ctx, const char *path);
 
+/* The same two texts without a context: the status line of the given values
+ * (no newline), and grid.json's whole text (with its newlines).  Return
+ * FEN_ERR_ARG if buf is too small.  fen_format_status and
+ * fen_write_grid_json write these texts. */
+int fen_status_text(int step, double time, double dt, double maxdiv, double maxcfl, char *buf, size_t len);
+int fen_grid_json_text(int nx, int ny, int nz, const double *x0, double lx, double ly, double lz, char *buf,
+                       size_t len);
+
+/* Case file: a JSON description of a run's setup, read by fen_case_read.
+ * FEN has no input file: its programs call grid%setup (src/grid.f90:67),
+ * init_solver (src/solver.f90:34), assign navier_stokes_mod's module
+ * variables (src/navier_stokes.f90:17-42), call set_timestep (:623) and
+ * write the initial fields, then loop over advance_solution
+ * (src/solver.f90:28) with print_solver_status, save_state and save_fields.
+ * The case file holds that setup as data.  Its "Grid" object has the keys of
+ * the grid.json that grid%print_json writes (src/grid.f90:246-258):
+ *
+ *   { "Grid": { "Nx": 256, "Ny": 256, "Nz": 256, "origin": [0, 0, 0],
+ *               "Lx": 6.283185307179586, "Ly": ..., "Lz": ...,
+ *               "boundary_conditions": ["Periodic", ... 4 (2-D) or 6 faces],
+ *               "name": "grid" },
+ *     "Physics":  { "density": 1, "viscosity": 1, "g": [0, 0, 0], "CFL": 1,
+ *                   "constant_CFL": false },
+ *     "Timestep": { "dt": 1e-3 }  or  { "U": 2.0 }   (set_timestep(comp_grid, dt, U)),
+ *     "Precision": "FP64" | "FP32_REFINE",
+ *     "Initial":  { "u": "u.raw", "v": ..., "w": ..., "p": ..., "vof": ... },
+ *     "Restart":  { "step": 5, "dir": "data" },
+ *     "Run":      { "steps": 100, "status_every": 1, "save_state": [50, 100],
+ *                   "save_fields": [100], "data_dir": "data", "write_final": false },
+ *     "Multiphase": { "rho_0": .., "rho_1": .., "mu_0": .., "mu_1": .., "sigma": 0,
+ *                     "interface": { "level": .., "amplitude": .., "wavelength": .. } } }
+ *
+ * "Grid", "Timestep" and "Run" are required; omitted physics take FEN's
+ * module defaults (density 1, viscosity 1, g 0, CFL 1, constant_CFL off);
+ * omitted boundaries are Periodic, as in grid%setup.  Nz == 1 makes the case
+ * 2-D.  Initial files are raw float64 interiors in Fortran order (the format
+ * of scalar%read/write); fields without one start at zero.  Unknown keys,
+ * wrong types and out-of-range values are errors.  A Multiphase interface is
+ * the zero level of y - level - amplitude cos(2 pi (x - x0) / wavelength),
+ * the heavy fluid (rho_1) above it.  No counterpart in the reference (it has
+ * no input file; src/json.f90 is an unused output writer). */
+#define FEN_CASE_MAXLIST 32
+#define FEN_CASE_PATHLEN 256
+typedef struct fen_case {
+    int dim;                                /* 2 (Nz == 1) or 3 */
+    int n[3];                               /* Nx, Ny, Nz */
+    int bc[6];                              /* FEN_BC_* per face */
+    int constant_cfl;
+    int precision;                          /* FEN_FP64 or FEN_FP32_REFINE */
+    int restart;                            /* step of the state to load, or -1 */
+    int steps;
+    int status_every;                       /* 0: no status lines */
+    int n_save_state, save_state[FEN_CASE_MAXLIST];
+    int n_save_fields, save_fields[FEN_CASE_MAXLIST];
+    int write_final;                        /* write u.raw v.raw [w.raw] p.raw [vof.raw] at the end */
+    int multiphase;                         /* the "Multiphase" object is present */
+    double origin[3], L[3];
+    double density, viscosity, g[3], cfl;
+    double dt;                              /* fixed dt, or 0 */
+    double U;                               /* velocity scale of set_timestep, or 0 */
+    double rho0, rho1, mu0, mu1, sigma;
+    double iface[3];                        /* interface level, amplitude, wavelength */
+    char name[64];                          /* grid name: <name>.json */
+    char data_dir[FEN_CASE_PATHLEN];        /* where save_state and save_fields write */
+    char restart_dir[FEN_CASE_PATHLEN];     /* where the restart state is read */
+    char init[5][FEN_CASE_PATHLEN];         /* u, v, w, p, vof ("" = none) */
+} fen_case;
+
+/* Parses a case file into *out.  On failure returns FEN_ERR_ARG and writes
+ * the reason (file, line and column, or the key) into err. */
+int fen_case_read(const char *path, fen_case *out, char *err, size_t errlen);
+
 /* Reports the Poisson solver kind (FEN_SOLVER_*) and the grid spacing. */
 int fen_solver_kind(const fen_ctx *ctx, int *kind);
 int fen_grid_delta(const fen_ctx *ctx, double *delta);

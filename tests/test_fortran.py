@@ -1,6 +1,7 @@
 """FEN's Fortran interface over libfen: the layer compiles for DIM=2 and
 DIM=3 and in the two-phase build (CPU), and FEN-style programs built on it
 reproduce the oracle and the reference's restart criterion on the GPU."""
+import json
 import os
 import re
 import shutil
@@ -67,45 +68,109 @@ def test_fortran_taylor_green_2d_matches_oracle(tmp_path):
         assert rel_err(got, ref) < 1e-12, name
 
 
+def _case(path, grid, steps, **extra):
+    c = {"Grid": grid, "Timestep": extra.pop("timestep", {"U": 2.0}), "Run": {"steps": steps, **extra.pop("run", {})}}
+    c.update(extra)
+    path.write_text(json.dumps(c, indent=2))
+    return path
+
+
+def _raw(a, path):
+    np.asfortranarray(a).ravel(order="F").tofile(path)
+
+
 @pytest.mark.gpu
 def test_fortran_restart_is_bitwise(tmp_path):
+    """The reference's restart criterion (test/small_test/io/run.sh:6-16):
+    a run that saves its state after steps 5 and 10, and a run restarted
+    from step 5 that saves after step 10, write identical state files.
+    Both runs are case files driven by run_case."""
     case = cases.tgv3d(16)
     for fid, name in ((U, "u"), (V, "v"), (W, "w"), (P, "p")):
-        np.asfortranarray(case.fields[fid]).ravel(order="F").tofile(tmp_path / f"init_{name}.raw")
+        _raw(case.fields[fid], tmp_path / f"init_{name}.raw")
+    grid = {"Nx": 16, "Ny": 16, "Nz": 16, "Lx": 2 * np.pi, "Ly": 2 * np.pi, "Lz": 2 * np.pi}
     (tmp_path / "data").mkdir()
-    exe = _bin("restart_3d")
-    subprocess.run([exe, "1"], cwd=tmp_path, check=True, timeout=120)
-    subprocess.run([exe, "2"], cwd=tmp_path, check=True, timeout=120)
+    (tmp_path / "again").mkdir()
+    first = _case(tmp_path / "first.json", grid, 10, Initial={k: f"init_{k}.raw" for k in "uvwp"},
+                  run={"save_state": [5, 10], "status_every": 0})
+    second = _case(tmp_path / "second.json", grid, 5, Restart={"step": 5, "dir": "data"},
+                   run={"save_state": [10], "data_dir": "again", "status_every": 0})
+    exe = _bin("run_case_3d")
+    subprocess.run([exe, str(first)], cwd=tmp_path, check=True, timeout=120)
+    subprocess.run([exe, str(second)], cwd=tmp_path, check=True, timeout=120)
     a = (tmp_path / "data" / "state_0000010.raw").read_bytes()
-    b = (tmp_path / "data" / "state_0000020.raw").read_bytes()
+    b = (tmp_path / "again" / "state_0000010.raw").read_bytes()
     assert len(a) == 7 * 16 ** 3 * 8 and a == b
 
 
 @pytest.mark.gpu
+def test_fortran_case_file_run_matches_oracle(tmp_path):
+    """A 3-D case file through run_case (set-up, status lines, save_fields,
+    final fields) against the oracle; grid%setup's grid.json equals the
+    C-ABI's fen_write_grid_json for the same grid."""
+    from fen_amd import Solver
+    N, steps = 32, 12
+    case = cases.tgv3d(N, steps=steps)
+    for fid, name in ((U, "u"), (V, "v"), (W, "w"), (P, "p")):
+        _raw(case.fields[fid], tmp_path / f"u0_{name}.raw")
+    L = 2 * np.pi
+    grid = {"Nx": N, "Ny": N, "Nz": N, "Lx": L, "Ly": L, "Lz": L, "name": "tgv"}
+    (tmp_path / "out").mkdir()
+    path = _case(tmp_path / "case.json", grid, steps, timestep={"dt": case.dt},
+                 Physics={"density": case.density, "viscosity": case.viscosity, "CFL": case.cfl},
+                 Initial={k: f"u0_{k}.raw" for k in "uvwp"},
+                 run={"status_every": 4, "save_fields": [steps], "data_dir": "out", "write_final": True})
+    out = subprocess.run([_bin("run_case_3d"), str(path)], cwd=tmp_path, check=True, capture_output=True,
+                         text=True, timeout=120).stdout
+    status = [l for l in out.splitlines() if l.startswith("step:")]
+    assert [int(l[6:13]) for l in status] == [4, 8, 12] and all(len(l) == 97 for l in status)
+    o = make_oracle(case)
+    for s in range(1, steps + 1):
+        _, md, mc = o.step(s, case.dt)
+    assert abs(float(status[-1][60:73]) - md) <= 1e-6 * abs(md) + 1e-30
+    for name, fid in (("u", U), ("v", V), ("w", W), ("p", P)):
+        got = np.fromfile(tmp_path / f"{name}.raw").reshape((N, N, N), order="F")
+        assert rel_err(got, o.field(fid)[1:-1, 1:-1, 1:-1]) < 1e-12, name
+    assert (tmp_path / "out" / f"vz_{steps:07d}.raw").stat().st_size == N ** 3 * 8
+    # grid%setup wrote <name>.json (src/grid.f90:198)
+    g = Solver(3, (N, N, N), (L, L, L))
+    g.write_grid_json(str(tmp_path / "c.json"))
+    assert (tmp_path / "c.json").read_bytes() == (tmp_path / "tgv.json").read_bytes()
+    g.close()
+
+
+@pytest.mark.gpu
 def test_fortran_rayleigh_taylor_matches_oracle(tmp_path):
-    """The reference's Rayleigh-Taylor program (two-phase build) at 32 x 128,
-    30 steps, against the oracle's MF steps from the same initial vof.  exp,
-    log, tanh and cosh come from the device's and the host's libm, so the bar
-    is the two-phase one: 1e-9 relative."""
+    """A two-phase Rayleigh-Taylor case file (32 x 128 on [0,1] x [0,4],
+    periodic in x, walls in y, heavy fluid 1.225 above light 0.1694 across
+    y = 2 + 0.05 cos(2 pi x), mu = 3.13e-3, gravity -9.80665, dt from
+    set_timestep with U = 1) run by run_case in the two-phase build, 30
+    steps, against the oracle's MF steps from the same initial vof.  exp,
+    log, tanh and cosh come from the device's and the host's libm, so the
+    bar is the two-phase one: 1e-9 relative."""
     import math
     from fen_amd._lib import BC_PERIODIC, BC_WALL, VOF
     from oracle import oracle as O
     nx, steps = 32, 30
-    (tmp_path / "data").mkdir()
-    out = subprocess.run([_bin("rayleigh_taylor_2d"), str(steps), str(nx)], cwd=tmp_path, check=True,
+    n = (nx, 4 * nx)
+    d = 1.0 / nx
+    grid = {"Nx": nx, "Ny": 4 * nx, "Nz": 1, "Lx": 1.0, "Ly": 4.0,
+            "boundary_conditions": ["Periodic", "Periodic", "Wall", "Wall"]}
+    path = _case(tmp_path / "rt.json", grid, steps, timestep={"U": 1.0},
+                 Physics={"viscosity": 1.0, "g": [0.0, -9.80665, 0.0]},
+                 Multiphase={"rho_0": 0.1694, "rho_1": 1.225, "mu_0": 3.13e-3, "mu_1": 3.13e-3,
+                             "interface": {"level": 2.0, "amplitude": 0.05, "wavelength": 1.0}},
+                 run={"write_final": True})
+    out = subprocess.run([_bin("run_case_2d_mf"), str(path)], cwd=tmp_path, check=True,
                          capture_output=True, text=True, timeout=120).stdout
     status = [l for l in out.splitlines() if l.startswith("step:")]
     assert len(status) == steps
-    n = (nx, 4 * nx)
-    d = 1.0 / nx
     vof0 = cases.vof_from_distance(lambda x, y: y - 0.05 * np.cos(2.0 * math.pi * x) - 2.0, n[0], n[1], d)
-    saved0 = np.fromfile(tmp_path / "data" / "vof_0").reshape(n, order="F")
-    assert np.abs(saved0 - vof0).max() < 1e-15
     o = O.Oracle(2, (n[0], n[1], 1), (1.0, 4.0, d),
                  bc=[BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC])
     o.set_params(density=1.0, viscosity=1.0, g=(0.0, -9.80665, 0.0), cfl=1.0)
     o.set_multiphase(0.1694, 1.225, 3.13e-3, 3.13e-3, 0.0)
-    o.field(VOF)[1:-1, 1:-1, 1] = saved0
+    o.field(VOF)[1:-1, 1:-1, 1] = vof0
     o.update_ghost_nodes(VOF)
     o.update_material_properties()
     dt = o.mf_set_timestep(1.0)
