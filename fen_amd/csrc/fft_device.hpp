@@ -130,6 +130,78 @@ __device__ __forceinline__ void fft_lds(C *line, int t, const C *tw)
     fft_stages<N, 1, SIGN>(line, t, tw);
 }
 
+// The same transform with the line's data in registers at both ends.  The
+// first Stockham stage (radix 8, no twiddles) reads x[t + q N/8], q = 0..7,
+// and the last stage writes X[t + q N/8]: the same eight positions.  So a
+// thread can load its eight inputs straight from global memory into v[q] =
+// x[t + q N/8], and receives v[q] = X[t + q N/8]; only the stages in between
+// exchange through `line` in LDS (two round trips for N = 128..512 instead
+// of four with LDS staging at both ends).  The arithmetic is fft_lds's, so
+// the results are bitwise equal.  `line` is this thread's line in LDS
+// (padded indexing); all threads of the block must call this (barriers),
+// and on return every read of `line` is complete.
+template <int N, int NS, int SIGN, typename C>
+__device__ __forceinline__ void fft_regs_stages(C (&v)[8], C *line, int t, const C *tw)
+{
+    constexpr int REM = N / NS;
+    constexpr int R = REM >= 8 ? 8 : REM;
+    constexpr int T = N / 8;
+    constexpr int NB = 8 / R;
+    constexpr bool FIRST = NS == 1, LAST = NS * R == N;
+    if constexpr (!FIRST) {
+        // this stage's inputs x[j + r N/R], j = t + m T
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+            const int j = t + m * T;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[m * R + r] = line[lds_pad(j + r * (N / R))];
+        }
+        __syncthreads();   // every read of this stage precedes any write below (in place)
+    }
+#pragma unroll
+    for (int m = 0; m < NB; m++) {
+        const int j = t + m * T;
+        const int k = j & (NS - 1);
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                C w = tw[r * k * (N / (NS * R))];
+                if constexpr (SIGN > 0) w.y = -w.y;
+                v[m * R + r] = c_mul(v[m * R + r], w);
+            }
+        }
+        dft_any<R, SIGN>(&v[m * R]);
+    }
+    if constexpr (LAST) {
+        // outputs (j div NS) NS R + k + r NS with j < NS: position t + (m + NB r) N/8
+        C out[8];
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+#pragma unroll
+            for (int r = 0; r < R; r++) out[m + NB * r] = v[m * R + r];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = out[q];
+    } else {
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+            const int j = t + m * T;
+            const int k = j & (NS - 1);
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) line[lds_pad(base + r * NS)] = v[m * R + r];
+        }
+        __syncthreads();
+        fft_regs_stages<N, NS * R, SIGN>(v, line, t, tw);
+    }
+}
+
+template <int N, int SIGN, typename C>
+__device__ __forceinline__ void fft_regs(C (&v)[8], C *line, int t, const C *tw)
+{
+    fft_regs_stages<N, 1, SIGN>(v, line, t, tw);
+    if constexpr (N > 8) __syncthreads();   // the last stage's LDS reads are done
+}
+
 // Copies the twiddle table W_N^m (m < N) into LDS.
 template <int N, typename C>
 __device__ __forceinline__ void load_twiddles(C *tw_lds, const C *tw_glob)

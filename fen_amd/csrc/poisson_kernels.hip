@@ -44,52 +44,48 @@ __global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs<R> a)
     constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), TJ = 2 * L, LL = lds_line<N>();
     __shared__ C buf[L * LL];
     __shared__ C tw[N];
-    load_twiddles<N>(tw, a.tw);
+    load_twiddles<N>(tw, a.tw);   // first read after the first stage's barrier
     const long row0 = (long)blockIdx.x * TJ;
-    // Staging: every thread owns PER elements; their global loads are issued
-    // in batches before any LDS write, so a batch costs one memory latency.
-    constexpr int PER = TJ * N / kBlock, BATCH = 8;
-    static_assert(PER % BATCH == 0, "staging batches");
+    // thread (l, t) forms rows 2l and 2l+1 at x = t + q N/8 straight into
+    // the first FFT stage's registers (the two real rows are the real and
+    // imaginary parts of one complex line)
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    C v[8];
 #pragma unroll
-    for (int b0 = 0; b0 < PER; b0 += BATCH) {
-        double val[BATCH];
+    for (int h = 0; h < 2; h++) {
+        const long r = row0 + 2 * l + h;
+        const bool ok = r < a.nrows;
+        const long rr = ok ? r : row0;   // a valid row: the loads are kept, their values dropped
+        const int j = (int)(rr & (a.b.ny - 1)) + 1, k = (int)(rr >> a.ny_shift) + 1;
 #pragma unroll
-        for (int m = 0; m < BATCH; m++) {
-            const int e = threadIdx.x + (b0 + m) * kBlock;
-            const int row = e / N, i = e % N + 1;
-            const long r = row0 + row;
-            const bool ok = r < a.nrows;
-            const long rr = ok ? r : row0;   // a valid row: the load is kept, its value dropped
-            const int j = (int)(rr & (a.b.ny - 1)) + 1, k = (int)(rr >> a.ny_shift) + 1;
-            double v;
+        for (int q = 0; q < 8; q++) {
+            const int i = t + q * (N / 8) + 1;
+            double val;
             if constexpr (DIRECT) {
-                v = a.u[(i - 1 + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
+                val = a.u[(i - 1 + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
             } else {
                 const long g = a.b.g(i, j, k);
                 double d = ((double)a.u[g] - (double)a.u[g - 1]) * a.id +
                            ((double)a.v[g] - (double)a.v[g - a.b.gx]) * a.id;
                 if constexpr (D3) d = d + ((double)a.w[g] - (double)a.w[g - a.b.gxy]) * a.id;
-                v = d * a.rho / a.dt;
+                val = d * a.rho / a.dt;
             }
-            val[m] = ok ? v : 0.0;
-        }
-#pragma unroll
-        for (int m = 0; m < BATCH; m++) {
-            const int e = threadIdx.x + (b0 + m) * kBlock;
-            const int row = e / N, i = e % N + 1;
-            C *line = buf + (row >> 1) * LL;
-            if (row & 1) line[lds_pad(i - 1)].y = (R)val[m];
-            else line[lds_pad(i - 1)].x = (R)val[m];
+            if (h == 0) v[q].x = ok ? (R)val : R(0);
+            else v[q].y = ok ? (R)val : R(0);
         }
     }
+    C *line = buf + l * LL;
+    fft_regs<N, -1>(v, line, t, tw);
+    // the half spectrum of each row needs bins kx and N - kx: through LDS
+#pragma unroll
+    for (int q = 0; q < 8; q++) line[lds_pad(t + q * (N / 8))] = v[q];
     __syncthreads();
-    fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
     for (int e = threadIdx.x; e < TJ * (N / 2 + 1); e += kBlock) {
         const int row = e % TJ, kx = e / TJ;
         const long r = row0 + row;
         if (r >= a.nrows) continue;
-        const C *line = buf + (row >> 1) * LL;
-        const C zk = line[lds_pad(kx)], zn = line[lds_pad((N - kx) & (N - 1))];
+        const C *ln = buf + (row >> 1) * LL;
+        const C zk = ln[lds_pad(kx)], zn = ln[lds_pad((N - kx) & (N - 1))];
         const R h = R(0.5), post = (R)a.post;   // post is 1 or 1/nx: exact in R
         C X;
         if ((row & 1) == 0) X = C{(zk.x + zn.x) * h, (zk.y - zn.y) * h};
@@ -155,25 +151,30 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs<R> a)
         }
     }
     __syncthreads();
-    fft_lds<N, 1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    C *line = buf + l * LL;
+    C v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = line[lds_pad(t + q * (N / 8))];
+    __syncthreads();   // every thread holds its first-stage inputs before the stage writes
+    fft_regs<N, 1>(v, line, t, tw);
     double acc = 0.0;
-    for (int e = threadIdx.x; e < TJ * N; e += kBlock) {
-        // for N >= 64 a wave writes part of one x line: its row is wave-uniform
-        const int row = N >= 64 ? __builtin_amdgcn_readfirstlane(e / N) : e / N, i = e % N;
-        const long r = row0 + row;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const long r = row0 + 2 * l + h;
         if (r >= a.nrows) continue;
-        const C z = buf[(row >> 1) * LL + lds_pad(i)];
-        const R val = (row & 1) ? z.y : z.x;
         const int j = (int)(r & (a.ny - 1)), k = (int)(r >> a.ny_shift);
-        a.out[(i + a.gl) + a.sx * (j + a.gl) + a.sy * (k + a.gl)] = val;
-        if (a.img) {
-            if constexpr (N >= 64) march_images(a.out, a.b, a.img, i + 1, j + 1, k + 1, val, true);
-            else write_images(a.out, a.b, a.img, i + 1, j + 1, k + 1, val);
+        R *orow = a.out + a.gl + a.sx * (j + a.gl) + a.sy * (k + a.gl);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int i = t + q * (N / 8);
+            const R val = h ? v[q].y : v[q].x;
+            orow[i] = val;
+            if (a.img) write_images(a.out, a.b, a.img, i + 1, j + 1, k + 1, val);
+            acc += (double)val;
         }
-        acc += (double)val;
     }
-    if (a.want_sum) {
-        __syncthreads();                     // every read of buf is done: reuse it
+    if (a.want_sum) {   // fft_regs has finished every read of buf: reuse it
         double *red = reinterpret_cast<double *>(buf);
         red[threadIdx.x] = acc;
         __syncthreads();
@@ -218,49 +219,37 @@ __global__ __launch_bounds__(kBlock) void y_kernel(YArgs<R> a)
     constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), LL = lds_line<N>();
     __shared__ C buf[L * LL];
     __shared__ C tw[N];
-    load_twiddles<N>(tw, a.tw);
-    const long line0 = (long)blockIdx.x * L;
-    constexpr int PER = L * N / kBlock;
-    C v[PER];
+    load_twiddles<N>(tw, a.tw);   // first read after the first stage's barrier
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    const long ln = (long)blockIdx.x * L + l;
+    const bool live = ln < a.nlines;
+    C *row = a.spec + (live ? ln : (long)blockIdx.x * L) * N;   // a dead lane reads the block's first line
+    C v[8];
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kBlock;
-        const int l = e / N, j = e % N;
-        const long ln = line0 + l;
-        const C x = a.spec[(ln < a.nlines ? ln : line0) * N + j];
-        v[m] = ln < a.nlines ? x : C{R(0), R(0)};
+    for (int q = 0; q < 8; q++) {
+        const C x = row[t + q * (N / 8)];
+        v[q] = live ? x : C{R(0), R(0)};
     }
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kBlock;
-        buf[(e / N) * LL + lds_pad(e % N)] = v[m];
-    }
-    __syncthreads();
-    C *line = buf + (threadIdx.x / T) * LL;
-    fft_lds<N, SIGN>(line, threadIdx.x % T, tw);
+    C *line = buf + l * LL;
+    fft_regs<N, SIGN>(v, line, t, tw);
     if constexpr (MODE == 1) {
-        for (int e = threadIdx.x; e < L * N; e += kBlock) {
-            const int l = e / N, j = e % N;
-            const long ln = line0 + l;
-            if (ln >= a.nlines) continue;
-            const int kx = (int)ln / (int)a.nz;   // 32-bit division
-            C X = buf[l * LL + lds_pad(j)];
-            const double s = a.mwx[kx] + a.mwy[j];
-            if (s == 0.0) X = C{R(0), R(0)};
-            else X = scale_div(X, a.inv_n, s);
-            buf[l * LL + lds_pad(j)] = X;
+        const int kx = live ? (int)ln / (int)a.nz : 0;   // 32-bit division
+        const double mx = a.mwx[kx];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const double s = mx + a.mwy[t + q * (N / 8)];
+            v[q] = s == 0.0 ? C{R(0), R(0)} : scale_div(v[q], a.inv_n, s);
         }
-        __syncthreads();
-        fft_lds<N, 1>(line, threadIdx.x % T, tw);
+        fft_regs<N, 1>(v, line, t, tw);
     }
-    for (int e = threadIdx.x; e < L * N; e += kBlock) {
-        const int l = e / N, j = e % N;
-        const long ln = line0 + l;
-        if (ln >= a.nlines) continue;
-        C X = buf[l * LL + lds_pad(j)];
-        X.x = (R)((double)X.x * a.post);
-        X.y = (R)((double)X.y * a.post);
-        a.spec[ln * N + j] = X;
+    if (live) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            C X = v[q];
+            X.x = (R)((double)X.x * a.post);
+            X.y = (R)((double)X.y * a.post);
+            row[t + q * (N / 8)] = X;
+        }
     }
 }
 
@@ -282,51 +271,38 @@ template <class R, int N, int NT>
 __global__ __launch_bounds__(NT) void z_solve_kernel(ZArgs<R> a)
 {
     using C = typename Cplx<R>::type;
-    constexpr int T = threads_per_line<N>(), L = NT / (N / 8), LL = lds_line<N>();
+    constexpr int L = NT / (N / 8), LL = lds_line<N>();
     __shared__ C buf[L * LL];
     __shared__ C tw[N];
-    load_twiddles<N>(tw, a.tw);
+    load_twiddles<N>(tw, a.tw);   // first read after the first stage's barrier
     const int kx = blockIdx.y;
     const int j0 = blockIdx.x * L;
-    C *base = a.spec + kx * a.kxs + j0;
-    constexpr int PER = L * N / NT;
-    static_assert(NT % L == 0, "a thread keeps its line in the divide loop");
-    const int lt = threadIdx.x % L;
-    const double mxy = j0 + lt < a.ny ? a.mwx[kx] + a.mwy[j0 + lt] : 0.0;
-    C v[PER];
-    double mz[PER];   // z wavenumbers of this thread's divide-loop elements, loaded with the line
+    // adjacent threads own adjacent j lines: every k row of the block is one
+    // contiguous run of L elements
+    const int l = threadIdx.x % L, t = threadIdx.x / L;
+    const bool live = j0 + l < a.ny;
+    C *col = a.spec + kx * a.kxs + j0 + (live ? l : 0);   // a dead lane reads column j0, drops it
+    C v[8];
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * NT;
-        const int l = e % L, k = e / L;
-        const C x = base[(long)k * a.ny + (j0 + l < a.ny ? l : 0)];
-        v[m] = (j0 + l < a.ny) ? x : C{R(0), R(0)};
-        mz[m] = a.mwz[k];
+    for (int q = 0; q < 8; q++) {
+        const C x = col[(long)(t + q * (N / 8)) * a.ny];
+        v[q] = live ? x : C{R(0), R(0)};
     }
+    double mz[8];
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * NT;
-        buf[(e % L) * LL + lds_pad(e / L)] = v[m];
-    }
-    __syncthreads();
-    C *line = buf + (threadIdx.x / T) * LL;
-    fft_lds<N, -1>(line, threadIdx.x % T, tw);
+    for (int q = 0; q < 8; q++) mz[q] = a.mwz[t + q * (N / 8)];
+    const double mxy = live ? a.mwx[kx] + a.mwy[j0 + l] : 0.0;
+    C *line = buf + l * LL;
+    fft_regs<N, -1>(v, line, t, tw);
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * NT;
-        const int l = e % L, k = e / L;
-        if (j0 + l >= a.ny) continue;
-        C X = buf[l * LL + lds_pad(k)];
-        const double s = mxy + mz[m];
-        if (s == 0.0) X = C{R(0), R(0)};
-        else X = scale_div(X, a.inv_n, s);
-        buf[l * LL + lds_pad(k)] = X;
+    for (int q = 0; q < 8; q++) {
+        const double s = mxy + mz[q];
+        v[q] = s == 0.0 ? C{R(0), R(0)} : scale_div(v[q], a.inv_n, s);
     }
-    __syncthreads();
-    fft_lds<N, 1>(line, threadIdx.x % T, tw);
-    for (int e = threadIdx.x; e < L * N; e += NT) {
-        const int l = e % L, k = e / L;
-        if (j0 + l < a.ny) base[(long)k * a.ny + l] = buf[l * LL + lds_pad(k)];
+    fft_regs<N, 1>(v, line, t, tw);
+    if (live) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) col[(long)(t + q * (N / 8)) * a.ny] = v[q];
     }
 }
 
