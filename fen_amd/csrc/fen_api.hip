@@ -444,7 +444,8 @@ void advance_mf(fen_ctx *c, double dt)
     c->last_dt = dt;
     c->step_count++;
     mf_advect_vof(c, dt);          // :82
-    mf_update_fields(c, true);     // :85 and p_hat = 2 p - p_o (:91), with ghosts
+    // :85 and p_hat (:88-92), with ghosts; the constant-CFL form extrapolates with dt_o
+    mf_update_fields(c, true, c->constant_cfl ? dt : 0.0);
     mf_momentum(c, dt);            // predicted_velocity_field with MF (:140-213)
     if (const int rest = remaining_groups(c, c->img_vel)) {
         GhostDesc<double> g[2] = {make_desc<double>(c, FEN_U, c->d_us[0]), make_desc<double>(c, FEN_V, c->d_us[1])};
@@ -778,6 +779,8 @@ int fen_set_timestep(fen_ctx *c, double U, double *dt)
             const double pi = std::acos(-1.0);
             const double dt_surf = std::sqrt(0.5 * (m.rho0 + m.rho1) * (d * d * d) / (pi * m.sigma + 1.0e-16));
             *dt = std::fmin(*dt, dt_surf);
+            c->mf->dt_surf = dt_surf;
+            c->mf->have_dt_surf = true;
         }
     }
     c->dt_o = *dt;
@@ -992,7 +995,11 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
     (void)step;
     if (!c || !dt) return FEN_ERR_ARG;
     if (!c->have_dt_o) return fail(c, FEN_ERR_STATE, "dt_o is undefined: call fen_set_timestep or fen_set_dt_o");
-    if (c->mf && c->constant_cfl) return fail(c, FEN_ERR_UNSUPPORTED, "constant CFL with the two-phase mode");
+    // the reference leaves dt_surf unset unless sigma > 0 (src/navier_stokes.f90:29, 657) and
+    // takes min(dt_conv, dt_visc, dt_surf) with it (:723)
+    if (c->mf && c->constant_cfl && !c->mf->have_dt_surf)
+        return fail(c, FEN_ERR_UNSUPPORTED,
+                    "constant CFL in the two-phase mode needs sigma > 0 and fen_set_timestep (dt_surf)");
     FEN_TRY(c, {
         if (c->constant_cfl) {
             // update_timestep (src/navier_stokes.f90:670-730): the max of
@@ -1013,6 +1020,7 @@ int fen_ns_step(fen_ctx *c, int step, double *dt, double *maxdiv, double *maxcfl
             const double mv = c->h_scal[1];
             c->dt_conv = mv > 0.0 ? c->cfl * c->delta / mv : 1.0;
             *dt = c->dt_conv < c->dt_visc ? c->dt_conv : c->dt_visc;
+            if (c->mf && c->mf->dt_surf < *dt) *dt = c->mf->dt_surf;   // MF build (:723)
             if (*dt > 1.1 * c->dt_o) *dt = 1.1 * c->dt_o;
         }
         const double dtv = *dt;

@@ -161,6 +161,8 @@ struct AdjointTape;
 // ghosted, the host's (nx+2)(ny+2)(nz+2) layout).
 struct MFState {
     double rho0 = 1, rho1 = 1, mu0 = 1, mu1 = 1, sigma = 0, beta = 1, rhomin = 1, irhomin = 1;
+    double dt_surf = 0;          // set by set_timestep when sigma > 0 (src/navier_stokes.f90:657)
+    bool have_dt_surf = false;
     int x_first = 1;
     double *vof = nullptr, *vof1 = nullptr, *vof2 = nullptr, *h = nullptr, *dd = nullptr, *curv = nullptr;
     double *nrx = nullptr, *nry = nullptr, *lx = nullptr, *ly = nullptr;
@@ -283,7 +285,8 @@ template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long
 template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out, bool store = true);
 bool fused_z_walls(const fen_ctx *c);
 void mf_advect_vof(fen_ctx *c, double dt);
-void mf_update_fields(fen_ctx *c, bool with_phat);
+// p_hat = 2 p - p_o, or with `dt` > 0 (constant CFL) p_o + (dt + dt_o)(p - p_o)/dt_o.
+void mf_update_fields(fen_ctx *c, bool with_phat, double dt = 0.0);
 void mf_momentum(fen_ctx *c, double dt);
 void mf_ghost_field(fen_ctx *c, double *f);
 template <class R>

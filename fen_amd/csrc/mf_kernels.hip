@@ -190,17 +190,22 @@ __global__ __launch_bounds__(kBX *kBY) void vof_finish_kernel(Box b, double delt
 }
 
 // rho, mu from the vof over the whole array (src/multiphase.f90:133-134), and
-// p_hat = 2 p - p_o (src/navier_stokes.f90:91) in the same pass.
+// p_hat in the same pass: 2 p - p_o at fixed dt (src/navier_stokes.f90:91),
+// p_o + (dt + dt_o)(p - p_o)/dt_o with constant CFL (:89; `dt` > 0).
 __global__ __launch_bounds__(256) void mf_fields_kernel(long n, const double *vof, double rho0, double rho1,
                                                         double mu0, double mu1, double *rho, double *mu,
-                                                        const double *p, const double *p_o, double *p_hat)
+                                                        const double *p, const double *p_o, double *p_hat,
+                                                        double dt, double dt_o)
 {
     const long q = (long)blockIdx.x * 256 + threadIdx.x;
     if (q >= n) return;
     const double f = vof[q];
     rho[q] = rho1 * f + rho0 * (1.0 - f);
     mu[q] = mu1 * f + mu0 * (1.0 - f);
-    if (p_hat) p_hat[q] = 2.0 * p[q] - p_o[q];
+    if (p_hat) {
+        if (dt > 0.0) p_hat[q] = p_o[q] + (dt + dt_o) * (p[q] - p_o[q]) / dt_o;
+        else p_hat[q] = 2.0 * p[q] - p_o[q];
+    }
 }
 
 struct MFMomArgs {
@@ -377,13 +382,14 @@ void mf_advect_vof(fen_ctx *c, double dt)
 }
 
 // update_material_properties (and p_hat when `with_phat`).
-void mf_update_fields(fen_ctx *c, bool with_phat)
+void mf_update_fields(fen_ctx *c, bool with_phat, double dt)
 {
     MFState &m = *c->mf;
     const long n = (long)c->n_ghost;
     hipLaunchKernelGGL(mf_fields_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
                        (const double *)m.vof, m.rho0, m.rho1, m.mu0, m.mu1, m.rho, m.mu,
-                       (const double *)as<double>(c->d_p), (const double *)m.p_o, with_phat ? m.p_hat : nullptr);
+                       (const double *)as<double>(c->d_p), (const double *)m.p_o, with_phat ? m.p_hat : nullptr,
+                       dt, c->dt_o);
     double *f[3] = {m.rho, m.mu, m.p_hat};
     mf_ghost(c, f, with_phat ? 3 : 2);
 }

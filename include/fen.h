@@ -162,8 +162,12 @@ int fen_ns_run(fen_ctx *ctx, int step0, int nsteps, double *dt);
  * material properties, p_hat, momentum with variable rho and mu and CSF
  * surface tension, the Poisson solve with rho_min, projection with 1/rho_min,
  * p_o = p.  fen_set_timestep takes the MF branch (:653-660).  Faces must be
- * Periodic or Wall (the reference stops otherwise); constant CFL is not
- * supported in this mode.  Returns FEN_ERR_UNSUPPORTED for 3-D or FP32. */
+ * Periodic or Wall (the reference stops otherwise).  With constant CFL the
+ * step takes update_timestep's MF branch, min(dt_conv, dt_visc, dt_surf)
+ * (:723), and extrapolates p_hat with dt_o (:88-89); the reference sets
+ * dt_surf only when sigma > 0 (:657), so constant CFL needs sigma > 0 and a
+ * fen_set_timestep call after this one (FEN_ERR_UNSUPPORTED otherwise).
+ * Returns FEN_ERR_UNSUPPORTED for 3-D or FP32. */
 int fen_set_multiphase(fen_ctx *ctx, double rho0, double rho1, double mu0, double mu1, double sigma,
                        double beta);
 

@@ -64,6 +64,8 @@ struct orc_ctx {
     /* two-phase (MF) state: multiphase_mod and volume_of_fluid_mod */
     int mf, x_first;
     double rho0, rho1, mu0, mu1, sigma, beta, rhomin, irhomin;
+    double dt_surf;      /* set by set_timestep when sigma > 0 (src/navier_stokes.f90:29,657) */
+    int have_dt_surf;
     ofield vof, h, dd, curv, nrx, nry, lx2, ly2, rho, mu, p_hat, p_o;
 };
 
@@ -1087,13 +1089,16 @@ static double max_velocity_sum(const orc_ctx *c)
     return mv;
 }
 
-/* update_timestep, src/navier_stokes.f90:670-730 (single-phase branch). */
+/* update_timestep, src/navier_stokes.f90:670-730: dt = min(dt_conv, dt_visc)
+ * (:725), or min(dt_conv, dt_visc, dt_surf) in the MF build (:723); then
+ * the 1.1 dt_o cap (:728). */
 static void update_timestep(orc_ctx *c, double *dt)
 {
     c->dt_o = *dt;
     const double mv = max_velocity_sum(c);
     c->dt_conv = mv > 0.0 ? c->cfl * c->delta / mv : 1.0;
     *dt = c->dt_conv < c->dt_visc ? c->dt_conv : c->dt_visc;
+    if (c->mf && c->dt_surf < *dt) *dt = c->dt_surf;
     if (*dt > 1.1 * c->dt_o) *dt = 1.1 * c->dt_o;
 }
 
@@ -1102,7 +1107,8 @@ static void mf_step(orc_ctx *c, double dt, double *maxdiv, double *maxcfl);
 void orc_ns_step(orc_ctx *c, int step, double *dt_io, double *maxdiv, double *maxcfl)
 {
     (void)step;
-    if (c->mf) {   /* the MF build's step, fixed dt */
+    if (c->mf) {   /* the MF build's step */
+        if (c->constant_cfl) update_timestep(c, dt_io);                        /* :78 */
         mf_step(c, *dt_io, maxdiv, maxcfl);
         return;
     }
@@ -1424,6 +1430,8 @@ double orc_mf_set_timestep(orc_ctx *c, double U)
         const double dt_surf = sqrt(0.5 * (c->rho0 + c->rho1) * (c->delta * c->delta * c->delta) /
                                     (pi * c->sigma + 1.0e-16));
         dt = fmin(dt, dt_surf);
+        c->dt_surf = dt_surf;
+        c->have_dt_surf = 1;
     }
     c->dt_o = dt;
     return dt;
@@ -1439,7 +1447,12 @@ static void mf_step(orc_ctx *c, double dt, double *maxdiv, double *maxcfl)
 
     orc_advect_vof(c, dt);                                                      /* :82 */
     orc_update_material_properties(c);                                          /* :85 */
-    for (size_t q = 0; q < ng; q++) c->p_hat.f[q] = 2.0 * c->p.f[q] - c->p_o.f[q];   /* :91 */
+    if (c->constant_cfl) {                                                      /* :88 */
+        for (size_t q = 0; q < ng; q++)
+            c->p_hat.f[q] = c->p_o.f[q] + (dt + c->dt_o) * (c->p.f[q] - c->p_o.f[q]) / c->dt_o;
+    } else {
+        for (size_t q = 0; q < ng; q++) c->p_hat.f[q] = 2.0 * c->p.f[q] - c->p_o.f[q];   /* :91 */
+    }
     ghost_fill(c, &c->p_hat);
 
     /* predicted_velocity_field (:140-213) */

@@ -70,7 +70,11 @@ void orc_ns_step(orc_ctx *c, int step, double *dt, double *maxdiv, double *maxcf
  * 174-184, 526-531).  orc_set_multiphase switches the context to it (returns 0, or -1
  * for 3-D or for faces other than Periodic/Wall); after the vof is written,
  * orc_update_material_properties forms rho and mu as init_solver does
- * (src/solver.f90:85-93).  orc_ns_step then runs the MF step at fixed dt. */
+ * (src/solver.f90:85-93).  orc_ns_step then runs the MF step; with constant_CFL
+ * it takes update_timestep's MF branch (src/navier_stokes.f90:723) and the
+ * p_hat extrapolation with dt_o (:88-89).  The reference leaves dt_surf unset
+ * when sigma == 0 (:29, 657), so dt would become min(..., 0) there: that
+ * combination is left to the caller (the GPU path refuses it). */
 int orc_set_multiphase(orc_ctx *c, double rho0, double rho1, double mu0, double mu1, double sigma,
                        double beta);
 void orc_update_material_properties(orc_ctx *c);

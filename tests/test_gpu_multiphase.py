@@ -126,3 +126,47 @@ def test_rayleigh_taylor_with_walls_matches_oracle():
     print(f"Rayleigh-Taylor 32x64, 40 steps: rel err {errs}, maxdiv {mdg:.2e}")
     for f, e in errs.items():
         assert e < 1e-9, (f, e)
+
+
+def test_rayleigh_taylor_constant_cfl_matches_oracle():
+    """Constant CFL in the two-phase mode: update_timestep's MF branch with
+    dt_surf (src/navier_stokes.f90:723) and p_hat extrapolated with dt_o
+    (:88-89), 40 steps in which the 1.1 dt_o cap and then dt_conv bind."""
+    n = (32, 64)
+    L = (1.0, 2.0)
+    wave = lambda x, y: (y - 1.0 - 0.05 * np.cos(2 * math.pi * x))
+    vof = vof_from_distance(wave, n, L[0] / n[0])
+    bc = [BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC]
+    params = dict(density=1.0, viscosity=0.01, g=(0.0, -1.0, 0.0), cfl=0.05)
+    o, g = _pair(n, L, bc, (1.0, 3.0, 0.01, 0.01, 0.05), vof, params=params)
+    o.update_material_properties()
+    g.update_material_properties()
+    dto = o.mf_set_timestep(1.0)
+    dtg = g.set_timestep(1.0)
+    assert dto == dtg
+    for s in (o, g):
+        s.set_params(**params, constant_cfl=True)
+    for s in range(1, 41):
+        dto, _, _ = o.step(s, dto)
+        dtg, mdg, _ = g.navier_stokes_solver(s, dtg)
+        assert abs(dtg - dto) <= 1e-9 * dto, (s, dtg, dto)
+    errs = {f: _rel(g.download(f)[1:-1, 1:-1, 1], o.field(f)[1:-1, 1:-1, 1]) for f in (U, V, P, VOF)}
+    print(f"Rayleigh-Taylor 32x64 constant CFL, 40 steps: dt {dtg:.6e}, rel err {errs}, maxdiv {mdg:.2e}")
+    for f, e in errs.items():
+        assert e < 1e-9, (f, e)
+
+
+def test_constant_cfl_without_surface_tension_is_refused():
+    """The reference sets dt_surf only when sigma > 0 (:657) and uses it in
+    update_timestep (:723): the combination is refused, not run with an
+    undefined dt_surf."""
+    from fen_amd import FenError
+    n = (16, 32)
+    vof = vof_from_distance(lambda x, y: y - 1.0, n, 1.0 / n[0])
+    bc = [BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC]
+    _, g = _pair(n, (1.0, 2.0), bc, (1.0, 3.0, 0.01, 0.01, 0.0), vof)
+    g.update_material_properties()
+    dt = g.set_timestep(1.0)
+    g.set_params(density=1.0, viscosity=0.01, cfl=0.5, constant_cfl=True)
+    with pytest.raises(FenError, match="sigma > 0"):
+        g.navier_stokes_solver(1, dt)

@@ -196,3 +196,76 @@ def test_static_drop_laplace_pressure_jump(rho_out):
     assert abs(jump - sigma / R) < 0.03 * sigma / R
     assert umax < 1e-2
     assert abs(md) < 1e-12
+
+
+def _rt_oracle(n, L, sigma, cfl, constant_cfl):
+    from fen_amd._lib import BC_PERIODIC, BC_WALL, VOF
+    wave = lambda x, y: (y - 0.5 * L[1] - 0.05 * np.cos(2 * math.pi * x))
+    vof = vof_from_distance(wave, n, L[0] / n[0])
+    bc = [BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC]
+    o = O.Oracle(2, (n[0], n[1], 1), (L[0], L[1], L[0] / n[0]), bc=bc)
+    o.set_params(density=1.0, viscosity=0.01, g=(0.0, -1.0, 0.0), cfl=cfl)
+    o.set_multiphase(1.0, 3.0, 0.01, 0.01, sigma)
+    o.field(VOF)[1:-1, 1:-1, 1] = vof
+    o.update_ghost_nodes(VOF)
+    o.update_material_properties()
+    dt = o.mf_set_timestep(1.0)
+    o.set_params(density=1.0, viscosity=0.01, g=(0.0, -1.0, 0.0), cfl=cfl, constant_cfl=constant_cfl)
+    return o, dt
+
+
+def test_mf_constant_cfl_dt_sequence():
+    """update_timestep's MF branch (src/navier_stokes.f90:670-730): before
+    every step dt_o <- dt, dt = min(CFL delta / max(|u|+|v|), dt_visc,
+    dt_surf), capped at 1.1 dt_o; the max is recomputed here from the
+    oracle's velocity, dt_visc and dt_surf from set_timestep's closed forms
+    (:653-660).  A small CFL makes dt_conv bind once the flow starts, the
+    1.1 dt_o cap binds first (set_timestep's dt is for U = 1, the shear flow
+    given here moves at 0.5)."""
+    from fen_amd._lib import U, V
+    n, L, sigma, cfl = (16, 32), (1.0, 2.0), 0.05, 0.02
+    o, dt = _rt_oracle(n, L, sigma, cfl, True)
+    d = L[0] / n[0]
+    y = (np.arange(1, n[1] + 1) - 0.5) * d
+    o.field(U)[1:-1, 1:-1, 1] = 0.5 * np.sin(math.pi * y / L[1])[None, :]   # u(y), v = 0: divergence-free
+    o.update_ghost_nodes(U)
+    dt_visc = 0.125 * d * d * min(1.0 / 0.01, 3.0 / 0.01)
+    dt_surf = math.sqrt(0.5 * (1.0 + 3.0) * d ** 3 / (math.pi * sigma + 1.0e-16))
+    assert dt == min(cfl * d / 1.0, dt_visc, dt_surf)
+    bound = set()
+    for s in range(1, 41):
+        u, v = o.field(U)[1:-1, 1:-1, 1], o.field(V)[1:-1, 1:-1, 1]
+        mv = float((np.abs(u) + np.abs(v)).max())
+        dt_conv = cfl * d / mv if mv > 0 else 1.0
+        want = min(dt_conv, dt_visc, dt_surf)
+        cap = 1.1 * dt
+        if want > cap:
+            want = cap
+        bound.add("cap" if want == cap else ("conv" if want == dt_conv else ("surf" if want == dt_surf else "visc")))
+        dt_new, _, _ = o.step(s, dt)
+        assert dt_new == want, (s, dt_new, want)
+        assert o.dt_limits()[1] == dt_conv
+        dt = dt_new
+    assert {"cap", "conv"} <= bound, bound
+    assert dt < 0.5 * dt_surf
+
+
+def test_mf_constant_cfl_with_dt_fixed_by_dt_surf_is_the_fixed_dt_run():
+    """When dt_surf binds on every step, dt stays equal to dt_o and the
+    constant-CFL extrapolation p_o + (dt + dt_o)(p - p_o)/dt_o (:89) is
+    2 p - p_o (:91) up to rounding: the run equals the fixed-dt run at that
+    dt to round-off (a wrong dt or dt_o in the formula would not)."""
+    from fen_amd._lib import P, U, V, VOF
+    n, L = (16, 32), (1.0, 2.0)
+    a, dta = _rt_oracle(n, L, 50.0, 1.0, True)
+    b, dtb = _rt_oracle(n, L, 50.0, 1.0, False)
+    assert dta == dtb
+    for s in range(1, 11):
+        dta, _, _ = a.step(s, dta)
+        b.step(s, dtb)
+        assert dta == dtb
+        if s in (2, 10):   # one rounding of p_hat apart after step 2, grown by the flow after 10
+            tol = 1e-14 if s == 2 else 1e-11
+            for f in (U, V, P, VOF):
+                x, y = a.field(f)[1:-1, 1:-1, 1], b.field(f)[1:-1, 1:-1, 1]
+                assert np.abs(x - y).max() <= tol * np.abs(y).max(), (s, f)
