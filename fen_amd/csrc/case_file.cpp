@@ -52,8 +52,10 @@ class Parser {
     }
 
   private:
+    static constexpr int kMaxDepth = 64;   // case files nest 3 deep; this bounds the recursion
     const std::string &s;
     size_t p = 0;
+    int depth = 0;
 
     [[noreturn]] void fail(const std::string &what) const
     {
@@ -81,6 +83,12 @@ class Parser {
     {
         ws();
         if (p >= s.size()) fail("unexpected end of file");
+        if (depth >= kMaxDepth) fail("nesting deeper than " + std::to_string(kMaxDepth) + " levels");
+        struct Nest {
+            int &d;
+            explicit Nest(int &x) : d(x) { d++; }
+            ~Nest() { d--; }
+        } nest(depth);
         Value v;
         const char ch = s[p];
         if (ch == '{') {

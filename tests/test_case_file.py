@@ -111,6 +111,7 @@ def test_invalid_cases_are_rejected(tmp_path, mutate, msg):
     ('{"a": "\\u0041"}', "unsupported escape"),
     ('{} x', "unexpected text after the top-level value"),
     ('', "unexpected end of file"),
+    ('[' * 100 + ']' * 100, "nesting deeper than 64 levels"),
 ])
 def test_malformed_json_names_the_place(tmp_path, text, msg):
     with pytest.raises(case_file.CaseError, match=msg.replace("(", r"\(").replace("[", r"\[").replace("]", r"\]")):
