@@ -63,25 +63,31 @@ def test_channel_512_full_size_matches_oracle():
     for s in range(1, case.steps + 1):
         _, mdo, _ = o.step(s, dt)
     t1 = time.perf_counter()
-    errs = {f: rel_err(g.download(f)[1:-1, 1:-1, 1:-1], o.field(f)[1:-1, 1:-1, 1:-1]) for f in (U, V, W)}
-    # ppn subtracts the interior mean of phi every step (src/poisson.f90:1151-1171)
-    # and p accumulates phi, so p also accumulates the difference of the two
-    # sides' 67 M-term sums: the reference's sequential `sum` (the oracle's
-    # order) against the GPU's block tree.  That difference is the same at
-    # every cell, a constant offset of p, which the velocity never sees
-    # (only grad phi enters it).  p is compared with the offset removed,
-    # and the offset is bounded by the summation-order error of 100 means.
-    pg, po = g.download(P)[1:-1, 1:-1, 1:-1], o.field(P)[1:-1, 1:-1, 1:-1]
-    diff = pg - po
-    offset = float(diff.mean())
-    pscale = float(np.abs(po).max())
-    errs[P] = float(np.abs(diff - offset).max()) / pscale
-    print(f"channel 512x512x256 100 steps: oracle {t1 - t0:.1f} s; rel err {errs}; p offset {offset / pscale:.2e} "
-          f"(max-norm p error with it {rel_err(pg, po):.2e}); maxdiv gpu {md:.2e} oracle {mdo:.2e}")
+    errs = {f: rel_err(g.download(f)[1:-1, 1:-1, 1:-1], o.field(f)[1:-1, 1:-1, 1:-1]) for f in (U, V, W, P)}
+    # The pressure accumulates phi, whose lowest wavenumbers the solve
+    # amplifies most (ppn: the (kx, ky) = (1, 0) column has eigenvalues down
+    # to (2 pi / 4)^2 against 4 / delta^2): two FP64 solves that round
+    # differently differ there by ~N/(2 pi) times what the velocity shows
+    # (the velocity sees grad phi).  The bar for p is therefore the oracle's
+    # own spread under round-off: the same 100 steps with the velocity
+    # changed by one ulp at every cell before every step (a stand-in for two
+    # implementations whose roundings differ at every step), at least 1e-12.
+    o2 = make_oracle(case)
+    rng = np.random.default_rng(11)
+    for s in range(1, case.steps + 1):
+        for fid in (U, V, W):
+            f = o2.field(fid)[1:-1, 1:-1, 1:-1]
+            f *= 1.0 + np.finfo(float).eps * rng.choice([-1.0, 1.0], size=f.shape)
+            o2.update_ghost_nodes(fid)
+        o2.step(s, dt)
+    spread = {f: rel_err(o2.field(f)[1:-1, 1:-1, 1:-1], o.field(f)[1:-1, 1:-1, 1:-1]) for f in (U, V, W, P)}
+    o2.close()
+    print(f"channel 512x512x256 100 steps: oracle {t1 - t0:.1f} s; rel err gpu vs oracle {errs}; "
+          f"oracle vs oracle with 1-ulp changes every step {spread}; maxdiv gpu {md:.2e} oracle {mdo:.2e}")
     o.close()
-    for f, e in errs.items():
-        assert e < 1e-12, (f, e)
-    assert abs(offset) / pscale < 1e-11
+    for f in (U, V, W):
+        assert errs[f] < 1e-12, (f, errs[f])
+    assert errs[P] < max(1e-12, 2.0 * spread[P]), (errs[P], spread[P])
     u = g.download(U)[1:-1, 1:-1, 1:-1]
     w = g.download(W)[1:-1, 1:-1, 1:-1]
     scale = float(np.abs(u).max())
