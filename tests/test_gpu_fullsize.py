@@ -62,6 +62,8 @@ def test_channel_512_full_size_matches_oracle():
     o = make_oracle(case)
     for s in range(1, case.steps + 1):
         _, mdo, _ = o.step(s, dt)
+        if s % 10 == 0:
+            print(f"channel oracle step {s}", flush=True)
     t1 = time.perf_counter()
     errs = {f: rel_err(g.download(f)[1:-1, 1:-1, 1:-1], o.field(f)[1:-1, 1:-1, 1:-1]) for f in (U, V, W, P)}
     # The pressure accumulates phi, whose lowest wavenumbers the solve
@@ -80,6 +82,8 @@ def test_channel_512_full_size_matches_oracle():
             f *= 1.0 + np.finfo(float).eps * rng.choice([-1.0, 1.0], size=f.shape)
             o2.update_ghost_nodes(fid)
         o2.step(s, dt)
+        if s % 10 == 0:
+            print(f"channel perturbed oracle step {s}", flush=True)
     spread = {f: rel_err(o2.field(f)[1:-1, 1:-1, 1:-1], o.field(f)[1:-1, 1:-1, 1:-1]) for f in (U, V, W, P)}
     o2.close()
     print(f"channel 512x512x256 100 steps: oracle {t1 - t0:.1f} s; rel err gpu vs oracle {errs}; "
