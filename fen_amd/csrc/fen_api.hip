@@ -461,12 +461,14 @@ void advance_mf(fen_ctx *c, double dt)
 
 // Whether a step may end with its projection pending, to be fused into the
 // next step's momentum (launch_project_momentum): 3-D, every field periodic
-// in every direction (ppp) or x, y periodic with no-slip walls at z (ppn).
+// in every direction (ppp), x, y periodic with no-slip walls at z (ppn), or
+// npn/nnn in FP64 with the standard wall rules.
 bool deferrable(const fen_ctx *c)
 {
     if (!c->fused_run || c->dim != 3 || c->mf) return false;
     if (c->solver == FEN_SOLVER_PPP) return c->img_vel == 7 && c->img_p == 7 && c->img_phi == 7;
-    return c->solver == FEN_SOLVER_PPN && fen::fused_z_walls(c);
+    if (c->solver == FEN_SOLVER_PPN) return fen::fused_z_walls(c);
+    return fen::fused_general_walls(c);   // npn, nnn (FP64)
 }
 
 // One step that ends pending (fen_ctx::pending).  If the previous step is

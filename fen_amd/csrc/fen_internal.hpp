@@ -289,6 +289,9 @@ void mf_advect_vof(fen_ctx *c, double dt);
 void mf_update_fields(fen_ctx *c, bool with_phat, double dt = 0.0);
 void mf_momentum(fen_ctx *c, double dt);
 void mf_ghost_field(fen_ctx *c, double *f);
+// npn and nnn (FP64) whose state fields keep the standard wall rules: the fused
+// kernel's general-wall form applies.
+bool fused_general_walls(const fen_ctx *c);
 template <class R>
 void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
                              const void *p_in, void *const *us_out, void *p_out, double *checks_out);

@@ -53,55 +53,54 @@ __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
     constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), TJ = 2 * L, LL = lds_line<N>();
     __shared__ double2 buf[L * LL];
     __shared__ double2 tw[N];
-    load_twiddles<N>(tw, a.tw);
+    load_twiddles<N>(tw, a.tw);   // first read after the first stage's barrier
     const long row0 = (long)blockIdx.x * TJ;
-    // staging: the loads of BATCH elements are issued before their LDS writes
-    constexpr int PER = TJ * N / kNB, BATCH = PER < 8 ? PER : 8;
-    static_assert(PER * kNB == TJ * N && PER % BATCH == 0, "staging batches");
+    // thread (l, t) forms Makhoul's reordered rows 2l, 2l+1 at the first FFT
+    // stage's positions p = t + q N/8 straight into registers:
+    // v_p = x_{2p} (p < N/2), x_{2(N-1-p)+1} (p >= N/2)
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    double2 v[8];
 #pragma unroll
-    for (int b0 = 0; b0 < PER; b0 += BATCH) {
-        double vals[BATCH];
+    for (int h = 0; h < 2; h++) {
+        const long r0 = row0 + 2 * l + h;
+        const bool ok = r0 < a.nrows;
+        const long r = ok ? r0 : row0;   // a valid row: the loads are kept, the value dropped
+        // rows number ny*nz < 2^31: 32-bit division (a 64-bit one is a long routine)
+        const int ri = (int)r, j = ri % a.b.ny + 1, k = ri / a.b.ny + 1;
 #pragma unroll
-        for (int m = 0; m < BATCH; m++) {
-            const int e = threadIdx.x + (b0 + m) * kNB;
-            const int row = e / N, i = e % N;
-            const long r0 = row0 + row;
-            const bool ok = r0 < a.nrows;
-            const long r = ok ? r0 : row0;   // a valid row: the loads are kept, the value dropped
+        for (int q = 0; q < 8; q++) {
+            const int p = t + q * (N / 8);
+            const int i = p < N / 2 ? 2 * p : 2 * (N - 1 - p) + 1;
             double val;
             if constexpr (SRC == 2) {
                 val = a.u[r * N + i];
+            } else if constexpr (SRC == 1) {
+                val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
             } else {
-                // rows number ny*nz < 2^31: 32-bit division (a 64-bit one is a long routine)
-                const int ri = (int)r, j = ri % a.b.ny + 1, k = ri / a.b.ny + 1;
-                if constexpr (SRC == 1) {
-                    val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
-                } else {
-                    const long g = a.b.g(i + 1, j, k);
-                    double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
-                    if (a.d3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
-                    val = d * a.rho / a.dt;
-                }
+                const long g = a.b.g(i + 1, j, k);
+                double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
+                if (a.d3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
+                val = d * a.rho / a.dt;
             }
-            vals[m] = ok ? val : 0.0;
-        }
-#pragma unroll
-        for (int m = 0; m < BATCH; m++) {
-            const int e = threadIdx.x + (b0 + m) * kNB;
-            const int row = e / N, i = e % N;
-            double2 *line = buf + (row >> 1) * LL;
-            if (row & 1) line[lds_pad(mk_pos(i, N))].y = vals[m];
-            else line[lds_pad(mk_pos(i, N))].x = vals[m];
+            if (h == 0) v[q].x = ok ? val : 0.0;
+            else v[q].y = ok ? val : 0.0;
         }
     }
+    double2 *line = buf + l * LL;
+    fft_regs<N, -1>(v, line, t, tw);
+    // y_k needs bins k and N - k of the line: through LDS
+#pragma unroll
+    for (int q = 0; q < 8; q++) line[lds_pad(t + q * (N / 8))] = v[q];
     __syncthreads();
-    fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
+    // adjacent threads write adjacent addresses: rows (x pass, orr = 1) or
+    // k (y pass, ok = 1)
+    const bool k_fast = a.ok == 1;
     for (int e = threadIdx.x; e < TJ * N; e += kNB) {
-        const int row = e % TJ, k = e / TJ;
+        const int row = k_fast ? e / N : e % TJ, k = k_fast ? e % N : e / TJ;
         const long r = row0 + row;
         if (r >= a.nrows) continue;
-        const double2 *line = buf + (row >> 1) * LL;
-        const double2 zk = line[lds_pad(k)], zn = line[lds_pad((N - k) & (N - 1))];
+        const double2 *ln = buf + (row >> 1) * LL;
+        const double2 zk = ln[lds_pad(k)], zn = ln[lds_pad((N - k) & (N - 1))];
         const double c = a.mk[k].x, s = a.mk[k].y;
         double y;
         if ((row & 1) == 0) y = c * (zk.x + zn.x) + s * (zk.y - zn.y);
@@ -128,52 +127,85 @@ struct Dct3Args {
 template <int N, int DST>
 __global__ __launch_bounds__(kNB) void dct3_rows_kernel(Dct3Args a)
 {
-    constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), TJ = 2 * L, LL = lds_line<N>();
+    constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), LL = lds_line<N>();
     __shared__ double2 buf[L * LL];
     __shared__ double2 tw[N];
     load_twiddles<N>(tw, a.tw);
-    const long row0 = (long)blockIdx.x * TJ;
-    // staging: all of a thread's loads are issued before its LDS writes
-    constexpr int PER = L * N / kNB;
-    static_assert(PER * kNB == L * N, "staging");
-    double a1[PER], b1[PER], a2[PER], b2[PER];   // x_k and x_{N-k} of both rows
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kNB;
-        const int l = e % L, k = e / L;
-        const long r = row0 + 2 * l;
-        a1[m] = 0.0; b1[m] = 0.0; a2[m] = 0.0; b2[m] = 0.0;
-        if (r < a.nrows) {
-            a1[m] = a.in[k * a.ik + r * a.ir];
-            if (k > 0) b1[m] = a.in[(N - k) * a.ik + r * a.ir];
-        }
-        if (r + 1 < a.nrows) {
-            a2[m] = a.in[k * a.ik + (r + 1) * a.ir];
-            if (k > 0) b2[m] = a.in[(N - k) * a.ik + (r + 1) * a.ir];
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kNB;
-        const int l = e % L, k = e / L;
+    const long row0 = (long)blockIdx.x * 2 * L;
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    double2 *line = buf + l * LL;
+    // U_k of both rows packed, (x_k - i x_{N-k}) exp(i pi k/(2N)) (x_N = 0)
+    auto packed = [&](double a1, double b1, double a2, double b2, int k) {
         const double c = a.mk[k].x, s = a.mk[k].y;
-        const double2 U1 = double2{a1[m] * c + b1[m] * s, a1[m] * s - b1[m] * c};
-        const double2 U2 = double2{a2[m] * c + b2[m] * s, a2[m] * s - b2[m] * c};
-        buf[l * LL + lds_pad(k)] = double2{U1.x - U2.y, U1.y + U2.x};
+        const double2 U1 = double2{a1 * c + b1 * s, a1 * s - b1 * c};
+        const double2 U2 = double2{a2 * c + b2 * s, a2 * s - b2 * c};
+        return double2{U1.x - U2.y, U1.y + U2.x};
+    };
+    double2 v[8];
+    if constexpr (DST == 1) {
+        // y pass: k is the contiguous direction (ik = 1), so the first
+        // stage's inputs k = t + q N/8 load straight into registers
+        const long r1 = row0 + 2 * l, r2 = r1 + 1;
+        const bool ok1 = r1 < a.nrows, ok2 = r2 < a.nrows;
+        const double *in1 = a.in + (ok1 ? r1 : row0) * a.ir, *in2 = a.in + (ok2 ? r2 : row0) * a.ir;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int k = t + q * (N / 8);
+            const int kn = k > 0 ? N - k : 0;
+            const double a1 = in1[k * a.ik], b1 = in1[kn * a.ik], a2 = in2[k * a.ik], b2 = in2[kn * a.ik];
+            v[q] = packed(ok1 ? a1 : 0.0, ok1 && k > 0 ? b1 : 0.0, ok2 ? a2 : 0.0, ok2 && k > 0 ? b2 : 0.0, k);
+        }
+    } else {
+        // x pass: rows are the contiguous direction (ir = 1): staged through
+        // LDS with adjacent threads on adjacent rows; all loads issue first
+        constexpr int PER = L * N / kNB;
+        static_assert(PER * kNB == L * N, "staging");
+        double a1[PER], b1[PER], a2[PER], b2[PER];   // x_k and x_{N-k} of both rows
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int e = threadIdx.x + m * kNB;
+            const int ls = e % L, k = e / L;
+            const long r = row0 + 2 * ls;
+            a1[m] = 0.0; b1[m] = 0.0; a2[m] = 0.0; b2[m] = 0.0;
+            if (r < a.nrows) {
+                a1[m] = a.in[k * a.ik + r * a.ir];
+                if (k > 0) b1[m] = a.in[(N - k) * a.ik + r * a.ir];
+            }
+            if (r + 1 < a.nrows) {
+                a2[m] = a.in[k * a.ik + (r + 1) * a.ir];
+                if (k > 0) b2[m] = a.in[(N - k) * a.ik + (r + 1) * a.ir];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int e = threadIdx.x + m * kNB;
+            const int ls = e % L, k = e / L;
+            buf[ls * LL + lds_pad(k)] = packed(a1[m], b1[m], a2[m], b2[m], k);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = line[lds_pad(t + q * (N / 8))];
+        __syncthreads();   // every thread holds its first-stage inputs before the stage writes
     }
-    __syncthreads();
-    fft_lds<N, 1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
-    for (int e = threadIdx.x; e < TJ * N; e += kNB) {
-        const int row = e / N, i = e % N;
-        const long r = row0 + row;
+    fft_regs<N, 1>(v, line, t, tw);
+    // y_{2m} = u_m, y_{2m+1} = u_{N-1-m}: position p holds y_i,
+    // i = 2p (p < N/2), 2(N-1-p)+1 (p >= N/2)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const long r = row0 + 2 * l + h;
         if (r >= a.nrows) continue;
-        const double2 z = buf[(row >> 1) * LL + lds_pad(mk_pos(i, N))];
-        const double val = ((row & 1) ? z.y : z.x) * a.post;
+        double *orow;
         if constexpr (DST == 0) {
             const int ri = (int)r, j = ri % (int)a.ny, k = ri / (int)a.ny;   // 32-bit division
-            a.out[(i + a.gl) + a.sx * (j + a.gl) + a.sy * (k + a.gl)] = val;
+            orow = a.out + a.gl + a.sx * (j + a.gl) + a.sy * (k + a.gl);
         } else {
-            a.out[r * N + i] = val;
+            orow = a.out + r * N;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int p = t + q * (N / 8);
+            const int i = p < N / 2 ? 2 * p : 2 * (N - 1 - p) + 1;
+            orow[i] = (h ? v[q].y : v[q].x) * a.post;
         }
     }
 }
@@ -199,34 +231,31 @@ __global__ __launch_bounds__(kNB) void rfft_rows_kernel(RfftArgs a)
     __shared__ double2 tw[N];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
-    // staging: every load reads a valid row (past the end: the block's
-    // first, value dropped), so all of a thread's loads issue together
-    constexpr int PER = TJ * N / kNB;
-    static_assert(PER * kNB == TJ * N, "staging");
-    double vals[PER];
+    // rows 2l, 2l+1 at the first stage's positions straight into registers;
+    // past the end a row reads the block's first (value dropped)
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    const long r1 = row0 + 2 * l, r2 = r1 + 1;
+    const bool ok1 = r1 < a.nrows, ok2 = r2 < a.nrows;
+    const double *in1 = a.re + (ok1 ? r1 : row0) * N, *in2 = a.re + (ok2 ? r2 : row0) * N;
+    double2 v[8];
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kNB;
-        const long r = row0 + e / N;
-        vals[m] = a.re[(r < a.nrows ? r : row0) * N + e % N];
+    for (int q = 0; q < 8; q++) {
+        const int j = t + q * (N / 8);
+        const double x1 = in1[j], x2 = in2[j];
+        v[q] = double2{ok1 ? x1 : 0.0, ok2 ? x2 : 0.0};
     }
+    double2 *line = buf + l * LL;
+    fft_regs<N, -1>(v, line, t, tw);
+    // bins k and N - k of the packed line: through LDS
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kNB;
-        const int row = e / N, j = e % N;
-        const double val = row0 + row < a.nrows ? vals[m] : 0.0;
-        double2 *line = buf + (row >> 1) * LL;
-        if (row & 1) line[lds_pad(j)].y = val;
-        else line[lds_pad(j)].x = val;
-    }
+    for (int q = 0; q < 8; q++) line[lds_pad(t + q * (N / 8))] = v[q];
     __syncthreads();
-    fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
     for (int e = threadIdx.x; e < TJ * H; e += kNB) {
         const int row = e / H, k = e % H;
         const long r = row0 + row;
         if (r >= a.nrows) continue;
-        const double2 *line = buf + (row >> 1) * LL;
-        const double2 zk = line[lds_pad(k)], zn = line[lds_pad((N - k) & (N - 1))];
+        const double2 *ln = buf + (row >> 1) * LL;
+        const double2 zk = ln[lds_pad(k)], zn = ln[lds_pad((N - k) & (N - 1))];
         double2 X;
         if ((row & 1) == 0) X = double2{(zk.x + zn.x) * 0.5, (zk.y - zn.y) * 0.5};
         else X = double2{(zk.y + zn.y) * 0.5, (zn.x - zk.x) * 0.5};
@@ -240,45 +269,34 @@ __global__ __launch_bounds__(kNB) void irfft_rows_kernel(RfftArgs a)
     constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), LL = lds_line<N>(), H = N / 2 + 1;
     __shared__ double2 buf[L * LL];
     __shared__ double2 tw[N];
-    load_twiddles<N>(tw, a.tw);
+    load_twiddles<N>(tw, a.tw);   // first read after the first stage's barrier
     const long row0 = (long)blockIdx.x * 2 * L;
-    // staging: every load reads a valid row (values past the end dropped),
-    // so all of a thread's loads issue together
-    constexpr int PER = (L * H + kNB - 1) / kNB;
-    double2 V1[PER], V2[PER];
+    // the packed line of rows 2l, 2l+1 at the first stage's positions p:
+    // bin k = p (p <= N/2) or its mirror N - p, loaded straight from the
+    // half spectra (contiguous in k); past the end a row reads the block's first
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    const long r1 = row0 + 2 * l, r2 = r1 + 1;
+    const bool ok1 = r1 < a.nrows, ok2 = r2 < a.nrows;
+    const double2 *c1 = a.cx + (ok1 ? r1 : row0) * H, *c2 = a.cx + (ok2 ? r2 : row0) * H;
+    double2 v[8];
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kNB;
-        const int ee = e < L * H ? e : 0;
-        const int l = ee / H, k = ee % H;
-        const long r = row0 + 2 * l;
-        V1[m] = a.cx[(r < a.nrows ? r : row0) * H + k];
-        V2[m] = a.cx[(r + 1 < a.nrows ? r + 1 : row0) * H + k];
+    for (int q = 0; q < 8; q++) {
+        const int p = t + q * (N / 8);
+        const int k = p <= N / 2 ? p : N - p;
+        const double2 y1 = c1[k], y2 = c2[k];
+        const double2 X1 = ok1 ? y1 : double2{0.0, 0.0};
+        const double2 X2 = ok2 ? y2 : double2{0.0, 0.0};
+        if (p == 0 || p == N / 2) v[q] = double2{X1.x, X2.x};   // Im of bins 0 and N/2 ignored
+        else if (p < N / 2) v[q] = double2{X1.x - X2.y, X1.y + X2.x};
+        else v[q] = double2{X1.x + X2.y, X2.x - X1.y};
     }
+    double2 *line = buf + l * LL;
+    fft_regs<N, 1>(v, line, t, tw);
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kNB;
-        if (e >= L * H) continue;
-        const int l = e / H, k = e % H;
-        const long r = row0 + 2 * l;
-        const double2 X1 = r < a.nrows ? V1[m] : double2{0.0, 0.0};
-        const double2 X2 = r + 1 < a.nrows ? V2[m] : double2{0.0, 0.0};
-        double2 *line = buf + l * LL;
-        if (k == 0 || k == N / 2) {
-            line[lds_pad(k)] = double2{X1.x, X2.x};
-        } else {
-            line[lds_pad(k)] = double2{X1.x - X2.y, X1.y + X2.x};
-            line[lds_pad(N - k)] = double2{X1.x + X2.y, X2.x - X1.y};
-        }
-    }
-    __syncthreads();
-    fft_lds<N, 1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
-    for (int e = threadIdx.x; e < 2 * L * N; e += kNB) {
-        const int row = e / N, j = e % N;
-        const long r = row0 + row;
-        if (r >= a.nrows) continue;
-        const double2 z = buf[(row >> 1) * LL + lds_pad(j)];
-        a.re[r * N + j] = (row & 1) ? z.y : z.x;
+    for (int q = 0; q < 8; q++) {
+        const int j = t + q * (N / 8);
+        if (ok1) a.re[r1 * N + j] = v[q].x;
+        if (ok2) a.re[r2 * N + j] = v[q].y;
     }
 }
 

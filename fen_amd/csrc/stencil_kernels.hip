@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <type_traits>
 
 #include "fen_internal.hpp"
 #include "stencil_march.hpp"
@@ -529,8 +530,29 @@ bool fused_z_walls(const fen_ctx *c)
     return true;
 }
 
+// The general-wall form of the fused kernel (3-D FP64, no pending mean
+// shift; the Neumann-direction grids npn and nnn): every axis of every
+// state field is periodic on both faces, or Dirichlet/Neumann on both
+// (the tile formation evaluates the ghost fill's rule chain).  Grids the
+// two dedicated forms take are excluded.
+bool fused_general_walls(const fen_ctx *c)
+{
+    if (c->box.dim != 3 || c->precision != FEN_FP64) return false;
+    if (c->solver != FEN_SOLVER_NPN && c->solver != FEN_SOLVER_NNN) return false;
+    for (int ax = 0; ax < 3; ax++) {
+        const bool per = c->periodic[ax] != 0;
+        for (int f : {FEN_U, FEN_V, FEN_W, FEN_P, FEN_PHI})
+            for (int side = 0; side < 2; side++) {
+                const int t = c->bc_type[f][2 * ax + side];
+                if (per ? t != 0 : (t != 1 && t != 2)) return false;
+            }
+    }
+    return true;
+}
+
 // Step n's projection + checks fused with step n+1's momentum (a grid
-// periodic in every direction, or x, y periodic with z walls).
+// periodic in every direction, x, y periodic with z walls, or the general
+// walls of fused_general_walls).
 // us_in/p_in: u*, v*, w* and p of step n;
 // us_out/p_out: u*, v*, w* of step n+1 and p after step n (distinct arrays).
 template <class R>
@@ -539,9 +561,25 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
 {
     const bool periodic = c->img_vel == 7 && c->img_p == 7 && c->img_phi == 7 && !c->shift_active;
     const bool zwalls = fused_z_walls(c);
-    if (c->box.dim != 3 || !(periodic || zwalls))
-        throw std::runtime_error("fused projection + momentum needs a periodic grid or x, y periodic with z walls");
+    const bool general = !periodic && !zwalls && std::is_same<R, double>::value && fused_general_walls(c) &&
+                         !c->shift_active;
+    if (c->box.dim != 3 || !(periodic || zwalls || general))
+        throw std::runtime_error("fused projection + momentum needs a periodic grid, x, y periodic with z walls, "
+                                 "or the npn/nnn walls");
     PMArgs<R> a{};
+    if (general) {
+        a.wall = (c->periodic[0] ? 0 : 1) | (c->periodic[1] ? 0 : 2) | (c->periodic[2] ? 0 : 4);
+        // slots of bc_type / d_plane are the field ids FEN_P .. FEN_W
+        const int fid[4] = {FEN_U, FEN_V, FEN_W, FEN_P};
+        const int loc[4] = {'x', 'y', 'z', 'c'};
+        for (int q = 0; q < 4; q++) {
+            for (int f = 0; f < 6; f++) {
+                a.gd[q].plane[f] = as<R>(c->d_plane[fid[q]][f]);
+                a.gd[q].type[f] = c->bc_type[fid[q]][f];
+            }
+            a.gd[q].loc = loc[q];
+        }
+    }
     if (zwalls) {
         a.shift = c->shift_active ? c->d_shift : nullptr;
         for (int d = 0; d < 3; d++) {
@@ -580,9 +618,11 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
     hipLaunchKernelGGL((project_momentum_kernel<R, SRC, ZW, SH::FTY, SH::FOCC, SH::FTX>), grid, block, 0, c->stream, a)
     if (c->has_source) {
         if (zwalls) FEN_PM(1, 1);
+        else if (general) FEN_PM(1, 2);
         else FEN_PM(1, 0);
     } else {
         if (zwalls) FEN_PM(0, 1);
+        else if (general) FEN_PM(0, 2);
         else FEN_PM(0, 0);
     }
 #undef FEN_PM

@@ -331,6 +331,14 @@ __global__ __launch_bounds__(TX * TY, OCC) void project_march_kernel(ProjArgs<R>
 // planes are indexed at the cell's own, unwrapped (i, j)), w = bc (and the
 // last interior w is overwritten with bc), p = the neighbouring interior
 // value; a pending mean shift of phi is subtracted as the projection does.
+// Ghost rules of one state field for the general-wall form (ZW == 2): the
+// bc type and value plane of each face and the field's location.
+template <class R> struct PMGhost {
+    const R *plane[6];
+    int type[6];
+    int loc;   // 'c', 'x', 'y', 'z'
+};
+
 template <class R> struct PMArgs {
     MomArgs<R> m;                  // momentum of step n+1 (m.u, m.v, m.w, m.p unused)
     const R *__restrict__ us;      // u*, v*, w* of step n
@@ -345,7 +353,62 @@ template <class R> struct PMArgs {
     double pid, pdt, prhof, pirhof;
     int prpow2;
     int kb;                        // planes marched per workgroup
+    PMGhost<R> gd[4];              // ZW == 2: ghost rules of u, v, w, p
+    int wall;                      // ZW == 2: non-periodic axes (bit 0 x, 1 y, 2 z)
 };
+
+// The general-wall form's ghost value of one state field at ghosted
+// position (gi, gj, kk), given v = the field's projected value at the
+// interior cell the ghost fill reads, (ix, jy, kz) per axis: the ghost
+// kernel's x -> y -> z composition (S1, S2, S3) applied to that value, each
+// level with the value plane indexed at that level's coordinates, and the
+// high-face normal overwrite f(N) = bc.  Periodic axes only move the index.
+template <class R>
+__device__ __forceinline__ R pm_ghost_chain(const PMGhost<R> &d, const Box &b, int wall, int gi, int gj, int kk,
+                                            int jy, int kz, R v)
+{
+    const R two = R(2.0);
+    if (wall & 1) {   // S1(gi, jy, kz)
+        const long pj = jy + (long)(b.ny + 2) * kz;
+        if (gi == 0) {
+            if (d.type[0] == 1) v = d.loc == 'x' ? d.plane[0][pj] : two * d.plane[0][pj] - v;
+        } else if (gi == b.nx + 1) {
+            if (d.type[1] == 1) v = d.loc == 'x' ? d.plane[1][pj] : two * d.plane[1][pj] - v;
+        } else if (gi == b.nx && d.type[1] == 1 && d.loc == 'x') {
+            v = d.plane[1][pj];
+        }
+    }
+    if (wall & 2) {   // S2(gi, gj, kz)
+        const long pi = gi + (long)(b.nx + 2) * kz;
+        if (gj == 0) {
+            if (d.type[2] == 1) v = d.loc == 'y' ? d.plane[2][pi] : two * d.plane[2][pi] - v;
+        } else if (gj == b.ny + 1) {
+            if (d.type[3] == 1) v = d.loc == 'y' ? d.plane[3][pi] : two * d.plane[3][pi] - v;
+        } else if (gj == b.ny && d.type[3] == 1 && d.loc == 'y') {
+            v = d.plane[3][pi];
+        }
+    }
+    if (wall & 4) {   // S3(gi, gj, kk)
+        const long pij = gi + (long)(b.nx + 2) * gj;
+        if (kk == 0) {
+            if (d.type[4] == 1) v = d.loc == 'z' ? d.plane[4][pij] : two * d.plane[4][pij] - v;
+        } else if (kk == b.nz + 1) {
+            if (d.type[5] == 1) v = d.loc == 'z' ? d.plane[5][pij] : two * d.plane[5][pij] - v;
+        } else if (kk == b.nz && d.type[5] == 1 && d.loc == 'z') {
+            v = d.plane[5][pij];
+        }
+    }
+    return v;
+}
+
+// Source index of a ghosted coordinate along one axis: periodic wraps,
+// a wall axis reads the adjacent interior cell.
+__device__ __forceinline__ int pm_src(int g, int n, bool wall)
+{
+    if (g == 0) return wall ? 1 : n;
+    if (g == n + 1) return wall ? n : 1;
+    return g;
+}
 
 template <class R, int SRC, int TY, int ZW, int TX = 64>
 struct ProjectMomentumMarch {
@@ -361,7 +424,7 @@ struct ProjectMomentumMarch {
     __device__ __forceinline__ static PlaneIdx plane_index(const Box &b, int i0, int j0)
     {
         PlaneIdx pi{};
-        if constexpr (ZW) {
+        if constexpr (ZW == 1) {
 #pragma unroll
             for (int q = 0; q < S::PER; q++) {
                 const int e = threadIdx.x + q * S::NT;
@@ -376,7 +439,7 @@ struct ProjectMomentumMarch {
 
     // Tile element offsets at k = 0 with every ghost index wrapped to its
     // periodic interior image (elements past the grid are clamped first).
-    __device__ __forceinline__ static TileIdx<S> index(const Box &b, int i0, int j0)
+    __device__ __forceinline__ static TileIdx<S> index(const Box &b, int i0, int j0, int wall)
     {
         TileIdx<S> t;
 #pragma unroll
@@ -385,8 +448,13 @@ struct ProjectMomentumMarch {
             const int ri = e % S::LX, rj = e / S::LX;
             int gi = min(i0 - 1 + ri, b.nx + 1), gj = min(j0 - 1 + rj, b.ny + 1);
             if (e >= S::N) { gi = i0; gj = j0; }
-            gi = gi == 0 ? b.nx : (gi == b.nx + 1 ? 1 : gi);
-            gj = gj == 0 ? b.ny : (gj == b.ny + 1 ? 1 : gj);
+            if constexpr (ZW == 2) {
+                gi = pm_src(gi, b.nx, wall & 1);
+                gj = pm_src(gj, b.ny, wall & 2);
+            } else {
+                gi = gi == 0 ? b.nx : (gi == b.nx + 1 ? 1 : gi);
+                gj = gj == 0 ? b.ny : (gj == b.ny + 1 ? 1 : gj);
+            }
             t.off[q] = gi + (int)b.gx * gj;
         }
         return t;
@@ -396,8 +464,9 @@ struct ProjectMomentumMarch {
     {
         const Box &b = a.m.b;
         // the interior plane the state at plane kk is formed from
-        const int kw = ZW ? (kk == 0 ? 1 : (kk == b.nz + 1 ? b.nz : kk))
-                          : (kk == 0 ? b.nz : (kk == b.nz + 1 ? 1 : kk));
+        const int kw = ZW == 1   ? (kk == 0 ? 1 : (kk == b.nz + 1 ? b.nz : kk))
+                       : ZW == 2 ? pm_src(kk, b.nz, a.wall & 4)
+                                 : (kk == 0 ? b.nz : (kk == b.nz + 1 ? 1 : kk));
         const long kofs = (long)kw * b.gxy, sj = b.gx, sk = b.gxy;
 #pragma unroll
         for (int q = 0; q < S::PER; q++) {
@@ -417,7 +486,7 @@ struct ProjectMomentumMarch {
     // rules when ZW) into ring slot `slot`.
     template <class SL>
     __device__ __forceinline__ static void commit(R *lds, SL slot, const Raw &r, const PMArgs<R> &a,
-                                                  const PlaneIdx &pi, int kk, double sh)
+                                                  const PlaneIdx &pi, int kk, double sh, int i0, int j0)
     {
         auto dr = [&](double x) { return over_rho(x, a.prhof, a.pirhof, a.prpow2); };
         const int nz = a.m.b.nz;
@@ -425,14 +494,30 @@ struct ProjectMomentumMarch {
         for (int q = 0; q < S::PER; q++) {
             const int e = threadIdx.x + q * S::NT;
             double f0 = (double)r.v[4][q], fx = (double)r.v[5][q], fy = (double)r.v[6][q], fz = (double)r.v[7][q];
-            if constexpr (ZW) {
+            if constexpr (ZW == 1) {
                 f0 = f0 - sh; fx = fx - sh; fy = fy - sh; fz = fz - sh;
             }
             R un = (R)((double)r.v[0][q] - dr((fx - f0) * a.pid * a.pdt));
             R vn = (R)((double)r.v[1][q] - dr((fy - f0) * a.pid * a.pdt));
             R wn = (R)((double)r.v[2][q] - dr((fz - f0) * a.pid * a.pdt));
-            const R pn = (R)((double)r.v[3][q] + f0);
-            if constexpr (ZW) {
+            R pn = (R)((double)r.v[3][q] + f0);
+            if constexpr (ZW == 2) {
+                // ghost and overwritten positions of a wall axis: the ghost
+                // fill's rule chain on the projected source values
+                const Box &b = a.m.b;
+                const int ri = e % S::LX, rj = e / S::LX;
+                const int gi = min(i0 - 1 + ri, b.nx + 1), gj = min(j0 - 1 + rj, b.ny + 1);
+                const int w = a.wall;
+                if (((w & 1) && (gi == 0 || gi >= b.nx)) || ((w & 2) && (gj == 0 || gj >= b.ny)) ||
+                    ((w & 4) && (kk == 0 || kk >= b.nz))) {
+                    const int jy = pm_src(gj, b.ny, w & 2), kz = pm_src(kk, b.nz, w & 4);
+                    un = pm_ghost_chain(a.gd[0], b, w, gi, gj, kk, jy, kz, un);
+                    vn = pm_ghost_chain(a.gd[1], b, w, gi, gj, kk, jy, kz, vn);
+                    wn = pm_ghost_chain(a.gd[2], b, w, gi, gj, kk, jy, kz, wn);
+                    pn = pm_ghost_chain(a.gd[3], b, w, gi, gj, kk, jy, kz, pn);
+                }
+            }
+            if constexpr (ZW == 1) {
                 const R two = R(2.0);
                 if (kk == 0) {
                     un = two * a.zlo[0][pi.v[q]] - un;
@@ -460,10 +545,10 @@ struct ProjectMomentumMarch {
     __device__ __forceinline__ static void plane(const PMArgs<R> &a, R *lds, Raw &r, const TileIdx<S> &t,
                                                  const PlaneIdx &pi, double sh, int i, int j, int k, int kend,
                                                  bool active, int c, R (&o)[3], double &md, double &mv, double &bad,
-                                                 bool xedge, SM sm, SC sc, SN sn)
+                                                 bool xedge, SM sm, SC sc, SN sn, int i0, int j0)
     {
         const MomArgs<R> &m = a.m;
-        commit(lds, sn, r, a, pi, k + 1, sh);   // state at plane k+1
+        commit(lds, sn, r, a, pi, k + 1, sh, i0, j0);   // state at plane k+1
         __syncthreads();
         if (k + 2 <= kend + 1) fetch(r, a, t, k + 2);
         R on[3] = {R(0), R(0), R(0)};
@@ -516,14 +601,14 @@ struct ProjectMomentumMarch {
         const int i = i0 + tx, j = j0 + ty;
         const bool active = i <= b.nx && j <= b.ny;
         const int c = (ty + 1) * S::LX + tx + 1;
-        const TileIdx<S> t = index(b, i0, j0);
+        const TileIdx<S> t = index(b, i0, j0, a.wall);
         const PlaneIdx pi = plane_index(b, i0, j0);
-        const double sh = (ZW && a.shift) ? a.shift[0] : 0.0;
+        const double sh = (ZW == 1 && a.shift) ? a.shift[0] : 0.0;
         Raw r;
         fetch(r, a, t, k0 - 1);
-        commit(lds, IC<0>{}, r, a, pi, k0 - 1, sh);
+        commit(lds, IC<0>{}, r, a, pi, k0 - 1, sh, i0, j0);
         fetch(r, a, t, k0);
-        commit(lds, IC<1>{}, r, a, pi, k0, sh);
+        commit(lds, IC<1>{}, r, a, pi, k0, sh, i0, j0);
         fetch(r, a, t, k0 + 1);
         R o[3] = {R(0), R(0), R(0)};
         if (active) {
@@ -533,7 +618,7 @@ struct ProjectMomentumMarch {
         double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
         int sm = 0, sc = 1, sn = 2;
         for (int k = k0; k <= kend; k++) {
-            plane(a, lds, r, t, pi, sh, i, j, k, kend, active, c, o, md, mv, bad, xedge, sm, sc, sn);
+            plane(a, lds, r, t, pi, sh, i, j, k, kend, active, c, o, md, mv, bad, xedge, sm, sc, sn, i0, j0);
             const int tmp = sm;
             sm = sc; sc = sn; sn = tmp;
         }

@@ -35,11 +35,23 @@ def _same_state(a, b):
     assert sa["maxdiv"] == sb["maxdiv"] and sa["maxcfl"] == sb["maxcfl"]
 
 
-@pytest.mark.parametrize("precision", [0, FP32_REFINE])
-@pytest.mark.parametrize("name", ["tgv", "channel"])
+def _case(name, steps):
+    if name == "tgv":
+        return cases.tgv3d(32, steps=steps)
+    if name == "channel":
+        return cases.channel(32, 16, 24, steps=steps)
+    if name == "cavity":   # nnn: walls on all six faces, moving lid
+        return cases.cavity3d(32, 16, 24, steps=steps)
+    if name == "duct":     # npn: walls at x and z
+        return cases.duct3d(32, 16, 24, steps=steps)
+    return cases.duct3d(32, 16, 24, steps=steps, z_io=True)   # npn with Inflow/Outflow at z
+
+
+@pytest.mark.parametrize("precision,name", [(0, "tgv"), (0, "channel"), (0, "cavity"), (0, "duct"), (0, "duct_io"),
+                                            (FP32_REFINE, "tgv"), (FP32_REFINE, "channel")])
 def test_queued_steps_equal_whole_steps(name, precision, monkeypatch):
     print(f"dropin queued {name} precision {precision}", flush=True)
-    case = cases.tgv3d(32, steps=6) if name == "tgv" else cases.channel(32, 16, 24, steps=6)
+    case = _case(name, 6)
     a, b = _pair(case, monkeypatch, precision)
     for s in range(1, 7):
         a.navier_stokes_solver(s, case.dt, want_checks=False)
@@ -47,11 +59,11 @@ def test_queued_steps_equal_whole_steps(name, precision, monkeypatch):
     _same_state(a, b)
 
 
-@pytest.mark.parametrize("name", ["tgv", "channel"])
+@pytest.mark.parametrize("name", ["tgv", "channel", "cavity", "duct_io"])
 def test_checks_read_every_step(name, monkeypatch):
     """maxdiv and maxCFL of every step from the pending state's checks equal
     those of the whole step."""
-    case = cases.tgv3d(32, steps=5) if name == "tgv" else cases.channel(32, 16, 24, steps=5)
+    case = _case(name, 5)
     a, b = _pair(case, monkeypatch)
     for s in range(1, 6):
         ra = a.navier_stokes_solver(s, case.dt)
