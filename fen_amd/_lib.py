@@ -63,6 +63,7 @@ def load():
         "fen_ns_step": (c_int, [vp, c_int, dp, dp, dp]),
         "fen_ns_run": (c_int, [vp, c_int, c_int, dp]),
         "fen_poisson_solve": (c_int, [vp, dp, c_int]),
+        "fen_poisson_solve_noalloc": (c_int, [vp, dp, c_int]),
         "fen_add_advection": (c_int, [vp, dp, dp, dp]),
         "fen_format_status": (c_int, [vp, c_int, c_double, c_double, ctypes.c_char_p, c_size_t]),
         "fen_write_grid_json": (c_int, [vp, ctypes.c_char_p]),

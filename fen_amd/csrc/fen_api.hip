@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -27,6 +28,16 @@
 using fen::Box;
 using fen::GhostDesc;
 using fen::as;
+
+// ---------------------------------------------------------------------------
+// Scratch of fen_poisson_solve_noalloc: the c2r block sums and the subtracted
+// mean of one solve.  kNoAllocParts block sums cover ny*nz/(4096/nx) <= 32768
+// (every grid up to 512 x 512 x 256, 256^3, 1024 x 1024 x 128).  The array
+// exists once per GPU for the life of the process; calls on one GPU share it.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocParts = 32768;
+constexpr size_t kNoAllocDoubles = kNoAllocParts + 32;   // + the mean, on its own 256-byte line
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
 
 namespace fen {
 
@@ -149,6 +160,25 @@ std::vector<double2> dct_twiddles(int n)
 
 // Element size of the context's fields.
 size_t elem(const fen_ctx *c) { return c->precision == FEN_FP64 ? sizeof(double) : sizeof(float); }
+
+// Device address of `scratch` on the current device, looked up once per
+// device (hipGetSymbolAddress with that device current) and kept: in host
+// code the array's name is not a device address.
+double *noalloc_scratch()
+{
+    static std::mutex mu;
+    static std::vector<double *> addr;
+    int dev = 0;
+    FEN_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev >= addr.size()) addr.resize((size_t)dev + 1, nullptr);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        FEN_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        addr[dev] = static_cast<double *>(p);
+    }
+    return addr[dev];
+}
 
 void *dev_alloc_bytes(size_t bytes)
 {
@@ -357,7 +387,8 @@ void solve_to_x_spectrum(fen_ctx *c, const R *const *vel, const R *direct, long 
 
 // x backward transform of the spectrum into the interior of phi; sums the
 // interior for the mean when `mean`; writes the periodic images in `img`.
-template <class R> void x_backward(fen_ctx *c, R *phi, int gl, long sx, long sy, bool mean, int img)
+template <class R> void x_backward(fen_ctx *c, R *phi, int gl, long sx, long sy, bool mean, int img,
+                                   double *part = nullptr, size_t n_part = 0, double *shift = nullptr)
 {
     if constexpr (std::is_same_v<R, double>) {
         if (x_cosine(c)) {
@@ -366,7 +397,7 @@ template <class R> void x_backward(fen_ctx *c, R *phi, int gl, long sx, long sy,
             return;
         }
     }
-    fen::launch_c2r<R>(c, phi, gl, sx, sy, mean, img);
+    fen::launch_c2r<R>(c, phi, gl, sx, sy, mean, img, part, n_part, shift);
 }
 
 // K2..K6 of a step: the Poisson solve on u*, v*, w* (`us`) up to phi with
@@ -657,6 +688,7 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
     c->hx = nx / 2 + 1;
     try {
         FEN_HIP_CHECK(hipStreamCreate(&c->stream));
+        c->d_noalloc = noalloc_scratch();
         {
             int dev = 0;
             FEN_HIP_CHECK(hipGetDevice(&dev));
@@ -1050,6 +1082,43 @@ int fen_ns_run(fen_ctx *c, int step0, int nsteps, double *dt)
     return FEN_OK;
 }
 
+namespace {
+
+// The whole-array semantics of the reference's solvers that the device
+// solve leaves to the host: the x backward transform writes the interior
+// only, so with ghosts (gl = 1) ppn/npn/nnn leave them 0 (phi%f = 0 first,
+// src/poisson.f90:1151, 1297, 1436) and nn divides them by 2nx with the rest
+// (:594); the mean s is subtracted over the whole array (pp :503, ppn
+// :1171) or the interior (pn :408).
+void poisson_host_fixups(const fen_ctx *c, double *f, size_t n, int gl, bool mean, double s)
+{
+    const size_t gx = c->nx + 2, gy = c->ny + 2;
+    auto interior = [&](size_t q) {
+        const size_t i = q % gx, j = (q / gx) % gy, k = q / (gx * gy);
+        return i >= 1 && i <= (size_t)c->nx && j >= 1 && j <= (size_t)c->ny && k >= 1 && k <= (size_t)c->nz;
+    };
+    const int sv = c->solver;
+    if (gl && (sv == FEN_SOLVER_PPN || sv == FEN_SOLVER_NPN || sv == FEN_SOLVER_NNN)) {
+        for (size_t q = 0; q < n; q++)
+            if (!interior(q)) f[q] = 0.0;
+    }
+    if (gl && sv == FEN_SOLVER_NN) {
+        const double s2 = (double)(float)(2 * c->nx);
+        for (size_t q = 0; q < n; q++)
+            if (!interior(q)) f[q] = f[q] / s2;
+    }
+    if (mean) {
+        if (sv == FEN_SOLVER_PN) {
+            for (size_t q = 0; q < n; q++)
+                if (!gl || interior(q)) f[q] = f[q] - s;
+        } else {
+            for (size_t q = 0; q < n; q++) f[q] = f[q] - s;
+        }
+    }
+}
+
+}  // namespace
+
 int fen_poisson_solve(fen_ctx *c, double *f, int gl)
 {
     if (!c || !f || (gl != 0 && gl != 1)) return FEN_ERR_ARG;
@@ -1075,40 +1144,45 @@ int fen_poisson_solve(fen_ctx *c, double *f, int gl)
             solve_to_x_spectrum<R>(c, nullptr, as<R>(buf), sx, sy, gl, 1.0, 1.0);
             x_backward<R>(c, as<R>(buf), gl, sx, sy, mean, 0);
         });
-        std::vector<double> host(n);
-        copy_out(c, host.data(), buf, n);
+        copy_out(c, f, buf, n);
         if (mean) {
             FEN_HIP_CHECK(hipMemcpyAsync(c->h_scal + 3, c->d_shift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
             sync(c);
         }
-        // ghosts: the x backward transform writes the interior only
-        const size_t gx = c->nx + 2, gy = c->ny + 2;
-        auto interior = [&](size_t q) {
-            const size_t i = q % gx, j = (q / gx) % gy, k = q / (gx * gy);
-            return i >= 1 && i <= (size_t)c->nx && j >= 1 && j <= (size_t)c->ny && k >= 1 && k <= (size_t)c->nz;
-        };
-        const int sv = c->solver;
-        if (gl && (sv == FEN_SOLVER_PPN || sv == FEN_SOLVER_NPN || sv == FEN_SOLVER_NNN)) {
-            // phi%f = 0 over the whole array before the x backward transform
-            // (src/poisson.f90:1151, 1297, 1436)
-            for (size_t q = 0; q < n; q++)
-                if (!interior(q)) host[q] = 0.0;
-        }
-        if (gl && sv == FEN_SOLVER_NN) {   // whole array / float(nx*2) (:594)
-            const double s2 = (double)(float)(2 * c->nx);
-            for (size_t q = 0; q < n; q++)
-                if (!interior(q)) host[q] = host[q] / s2;
-        }
-        if (mean) {
-            const double s = c->h_scal[3];
-            if (sv == FEN_SOLVER_PN) {   // interior only (:408)
-                for (size_t q = 0; q < n; q++)
-                    if (!gl || interior(q)) host[q] = host[q] - s;
-            } else {
-                for (size_t q = 0; q < n; q++) host[q] = host[q] - s;   // whole array (:503, :1171)
-            }
-        }
-        std::memcpy(f, host.data(), n * sizeof(double));
+        poisson_host_fixups(c, f, n, gl, mean, mean ? c->h_scal[3] : 0.0);
+    });
+    return FEN_OK;
+}
+
+int fen_poisson_solve_noalloc(fen_ctx *c, double *f, int gl)
+{
+    if (!c || !f || (gl != 0 && gl != 1)) return FEN_ERR_ARG;
+    if (c->precision != FEN_FP64)
+        return fail(c, FEN_ERR_UNSUPPORTED, "fen_poisson_solve_noalloc is for FP64 contexts");
+    if (!c->d_noalloc) return fail(c, FEN_ERR_STATE, "no scratch address for this device");
+    const bool mean = removes_mean(c);
+    if (mean && (size_t)fen::c2r_blocks(c) > kNoAllocParts)
+        return fail(c, FEN_ERR_UNSUPPORTED,
+                    "grid too large for fen_poisson_solve_noalloc: the mean needs " +
+                        std::to_string(fen::c2r_blocks(c)) + " block sums, the scratch holds " +
+                        std::to_string(kNoAllocParts) + " (ny*nz/(4096/nx) at most)");
+    const size_t n = gl ? c->n_ghost : c->n_int;
+    const long sx = gl ? c->nx + 2 : c->nx;
+    const long sy = gl ? (long)(c->nx + 2) * (c->ny + 2) : (long)c->nx * c->ny;
+    double *part = c->d_noalloc, *shift = c->d_noalloc + kNoAllocParts;
+    FEN_TRY(c, {
+        // a pending step is completed first (its projection reads u* in the
+        // buffer used below); phi's own pending mean stays where it is
+        flush_pending(c);
+        double *buf = as<double>(c->d_us[0]);   // context memory between steps
+        FEN_HIP_CHECK(hipMemcpyAsync(buf, f, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        solve_to_x_spectrum<double>(c, nullptr, buf, sx, sy, gl, 1.0, 1.0);
+        x_backward<double>(c, buf, gl, sx, sy, mean, 0, part, kNoAllocParts, shift);
+        FEN_HIP_CHECK(hipMemcpyAsync(f, buf, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        double s = 0.0;
+        if (mean) FEN_HIP_CHECK(hipMemcpyAsync(&s, shift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        poisson_host_fixups(c, f, n, gl, mean, s);
     });
     return FEN_OK;
 }

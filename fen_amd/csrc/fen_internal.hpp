@@ -219,6 +219,7 @@ struct fen_ctx {
 
     // reductions
     double *d_part = nullptr;    // per-block partials
+    double *d_noalloc = nullptr; // device address of the allocation-free entry point's scratch (this device)
     size_t n_part = 0;
     double *d_scal = nullptr;    // [0] maxdiv [1] max|u|+|v|+|w| [2] nonfinite flag [3] mean shift
     double *h_scal = nullptr;    // pinned mirror
@@ -279,7 +280,11 @@ template <class R> void launch_y_fft(fen_ctx *c, int sign, double post_scale);
 template <class R> void launch_y_solve_2d(fen_ctx *c, double inv_n);
 template <class R> void launch_z_solve(fen_ctx *c, double inv_n);
 template <class R> void launch_thomas(fen_ctx *c);
-template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img);
+long c2r_blocks(const fen_ctx *c);
+// part/n_part/shift: where the block sums and the mean go (default: the
+// context's d_part, n_part, d_shift).
+template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img,
+                                   double *part = nullptr, size_t n_part = 0, double *shift = nullptr);
 // store == false: the checks of the projection only (max div, max |u|+|v|+|w|
 // of the projected velocity), nothing written but the reduction (3-D).
 template <class R> void launch_project(fen_ctx *c, double dt, double *checks_out, bool store = true);

@@ -595,8 +595,21 @@ template <class R> void launch_thomas(fen_ctx *c)
     prof_end(c, K_THOMAS, 2.0 * spec_bytes<R>(c) + 8.0 * n_el + 32.0 * n_el);
 }
 
-template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img)
+// Workgroups of the x c2r (each writes one block sum when the mean is taken).
+long c2r_blocks(const fen_ctx *c)
 {
+    const int tj = 2 * (kBlock / (c->nx / 8));
+    return ((long)c->ny * c->nz + tj - 1) / tj;
+}
+
+template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long sy, bool want_sum, int img,
+                                   double *part, size_t n_part, double *shift)
+{
+    if (!part) {
+        part = c->d_part;
+        n_part = c->n_part;
+        shift = c->d_shift;
+    }
     C2RArgs<R> a{};
     a.spec = as<CplxT<R>>(c->d_spec);
     a.tw = as<CplxT<R>>(c->d_twx);
@@ -609,12 +622,11 @@ template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long
     a.gl = gl_out;
     a.ny_shift = ilog2(c->ny);
     a.want_sum = want_sum ? 1 : 0;
-    a.part = c->d_part;
+    a.part = part;
     a.b = c->box;
     a.img = img;
-    const int tj = 2 * (kBlock / (c->nx / 8));
-    const long nblk = (a.nrows + tj - 1) / tj;
-    if ((size_t)nblk > c->n_part) throw std::runtime_error("partial-sum buffer too small");
+    const long nblk = c2r_blocks(c);
+    if (want_sum && (size_t)nblk > n_part) throw std::runtime_error("partial-sum buffer too small");
     prof_begin(c, K_C2R);
 #define L_C(NN) hipLaunchKernelGGL((c2r_kernel<R, NN>), dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, a);
     FEN_DISPATCH_N(c->nx, L_C)
@@ -623,8 +635,8 @@ template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long
     if (want_sum) {
         const double count = (double)(float)((long)c->nx * c->ny * c->nz);
         prof_begin(c, K_REDUCE);
-        hipLaunchKernelGGL(mean_shift_kernel, dim3(1), dim3(kShiftThreads), 0, c->stream, (const double *)c->d_part, nblk,
-                           count, c->d_shift);
+        hipLaunchKernelGGL(mean_shift_kernel, dim3(1), dim3(kShiftThreads), 0, c->stream, (const double *)part, nblk,
+                           count, shift);
         prof_end(c, K_REDUCE, 8.0 * nblk);
     }
 }
@@ -643,7 +655,7 @@ template <class R> void launch_apply_shift(fen_ctx *c, R *f, size_t n)
     template void launch_y_solve_2d<R>(fen_ctx *, double);                                                     \
     template void launch_z_solve<R>(fen_ctx *, double);                                                        \
     template void launch_thomas<R>(fen_ctx *);                                                                 \
-    template void launch_c2r<R>(fen_ctx *, R *, int, long, long, bool, int);                                   \
+    template void launch_c2r<R>(fen_ctx *, R *, int, long, long, bool, int, double *, size_t, double *);       \
     template void launch_apply_shift<R>(fen_ctx *, R *, size_t);
 FEN_INSTANTIATE(double)
 FEN_INSTANTIATE(float)

@@ -193,6 +193,12 @@ class Solver:
         self._check(self.lib.fen_poisson_solve(self.h, _dp(a), gl))
         return a
 
+    def solve_poisson_noalloc(self, f, gl):
+        """fen_poisson_solve_noalloc: the same solve, allocating nothing."""
+        a = np.asfortranarray(f, dtype=np.float64).copy(order="F")
+        self._check(self.lib.fen_poisson_solve_noalloc(self.h, _dp(a), gl))
+        return a
+
     def add_advection(self, R=None):
         if R is None:
             R = [np.zeros(self.n, order="F") for _ in range(3)]

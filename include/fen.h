@@ -185,6 +185,20 @@ int fen_advect_vof(fen_ctx *ctx, double dt);
  * with the same whole-array semantics as the selected reference solver. */
 int fen_poisson_solve(fen_ctx *ctx, double *f, int gl);
 
+/* fen_poisson_solve without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight loops, harnesses that forbid allocation).  Same arguments,
+ * results and whole-array semantics as fen_poisson_solve; the block sums and
+ * the mean of the pp/pn/ppn solvers go to a fixed device array of 256 KiB
+ * (32768 block sums: ny*nz/(4096/nx) must not exceed that, else
+ * FEN_ERR_UNSUPPORTED), of which each GPU holds one copy until the process
+ * ends; nothing is allocated on the device or the host.  The context's
+ * fields and spectrum (allocated by fen_create) are used as by
+ * fen_poisson_solve, except that phi's pending mean is left alone.  FP64
+ * contexts only.  Calls that overlap on one GPU (two host threads, or
+ * contexts on different streams) share that array and race. */
+int fen_poisson_solve_noalloc(fen_ctx *ctx, double *f, int gl);
+
 /* Replaces add_advection(comp_grid, RHS) (src/navier_stokes.f90:261-353) on
  * the resident velocity: rx/ry/rz are interior-sized host arrays updated as
  * RHS = RHS - (advective terms). rz may be NULL in 2-D. */
