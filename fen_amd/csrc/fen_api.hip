@@ -881,6 +881,22 @@ int fen_set_dt_o(fen_ctx *c, double dt_o)
 
 int fen_get_scalars(fen_ctx *c, double *maxdiv, double *maxcfl, double *dt_o, double *dt_visc, double *dt_conv)
 {
+int fen_set_vof_x_first(fen_ctx *c, int x_first)
+{
+    if (!c) return FEN_ERR_ARG;
+    if (!c->mf) return fail(c, FEN_ERR_STATE, "not in two-phase mode");
+    c->mf->x_first = x_first != 0;
+    return FEN_OK;
+}
+
+int fen_get_vof_x_first(fen_ctx *c, int *x_first)
+{
+    if (!c || !x_first) return FEN_ERR_ARG;
+    if (!c->mf) return fail(c, FEN_ERR_STATE, "not in two-phase mode");
+    *x_first = c->mf->x_first ? 1 : 0;
+    return FEN_OK;
+}
+
     if (!c) return FEN_ERR_ARG;
     FEN_TRY(c, {
         if (dt_o) *dt_o = c->dt_o;

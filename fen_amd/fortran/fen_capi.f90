@@ -161,9 +161,33 @@ module fen_capi
             real(c_double), value :: dt
         end function
 
+        integer(c_int) function fen_set_vof_x_first(ctx, x_first) bind(C, name = 'fen_set_vof_x_first')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: x_first
+        end function
+
+        integer(c_int) function fen_get_vof_x_first(ctx, x_first) bind(C, name = 'fen_get_vof_x_first')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(out) :: x_first
+        end function
+
         integer(c_int) function fen_synchronize(ctx) bind(C, name = 'fen_synchronize')
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function fen_save_state(ctx, path) bind(C, name = 'fen_save_state')
+            import :: c_ptr, c_int, c_char
+            type(c_ptr), value :: ctx
+            character(kind=c_char), intent(in) :: path(*)
+        end function
+
+        integer(c_int) function fen_load_state(ctx, path) bind(C, name = 'fen_load_state')
+            import :: c_ptr, c_int, c_char
+            type(c_ptr), value :: ctx
+            character(kind=c_char), intent(in) :: path(*)
         end function
 
         integer(c_int) function fen_case_read(path, out, err, errlen) bind(C, name = 'fen_case_read')

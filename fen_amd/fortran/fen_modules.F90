@@ -688,6 +688,11 @@ module navier_stokes_mod
 #ifdef MF
     !> Volume fraction of phase 1 (re-exported by volume_of_fluid_mod).
     type(scalar) :: vof
+    !> Order of the vof's directional sweeps, x first when .true.
+    !> (src/volume_of_fluid.f90:30; re-exported by volume_of_fluid_mod).  Each
+    !> advection flips it; the device's copy is set from it before each step
+    !> and read back after.
+    logical :: x_first = .true.
     logical, private :: multiphase = .false.
 #endif
 
@@ -948,7 +953,22 @@ contains
         mu%fid = FEN_MU
         multiphase = .true.
         host_modified = .true.
+        x_first = .true.
     end subroutine enable_multiphase
+
+    !> x_first to the device before an advection, back after it.
+    subroutine push_x_first()
+        integer(c_int) :: xf
+        xf = 0
+        if (x_first) xf = 1
+        call fen_check(fen_ctx, fen_set_vof_x_first(fen_ctx, xf), 'fen_set_vof_x_first')
+    end subroutine push_x_first
+
+    subroutine pull_x_first()
+        integer(c_int) :: xf
+        call fen_check(fen_ctx, fen_get_vof_x_first(fen_ctx, xf), 'fen_get_vof_x_first')
+        x_first = xf /= 0
+    end subroutine pull_x_first
 #endif
 
     !> Declares that user code changed host mirrors (fields, bc, S) mid-run.
@@ -967,6 +987,9 @@ contains
         call push_params()
         call fen_check(fen_ctx, fen_set_dt_o(fen_ctx, dt_o), 'fen_set_dt_o')
         d = dt
+#ifdef MF
+        if (multiphase) call push_x_first()
+#endif
         if (lazy_checks) then
             call fen_check(fen_ctx, fen_ns_step(fen_ctx, int(step, c_int), d), 'navier_stokes_solver')
             checks_stale = .true.
@@ -977,6 +1000,9 @@ contains
             checks_stale = .false.
         end if
         dt = d
+#ifdef MF
+        if (multiphase) call pull_x_first()
+#endif
         call fen_check(fen_ctx, fen_get_scalars(fen_ctx, dt_o=dto, dt_visc=dtv, dt_conv=dtc), 'fen_get_scalars')
         dt_o = dto
         if (constant_CFL) then
@@ -1058,10 +1084,10 @@ module volume_of_fluid_mod
     use precision_mod, only : dp
     use scalar_mod
     use vector_mod
-    use navier_stokes_mod, only : vof
+    use navier_stokes_mod, only : vof, x_first
     implicit none
     private
-    public :: vof, beta, distance, distance_from_interface, get_vof_from_distance, advect_vof
+    public :: vof, x_first, beta, distance, distance_from_interface, get_vof_from_distance, advect_vof
 
     !> Interface sharpness of the MTHINC profile (src/volume_of_fluid.f90:24).
     real(dp) :: beta = 1.0_dp
@@ -1119,12 +1145,14 @@ contains
     !> (src/volume_of_fluid.f90:420-544); v must be navier_stokes_mod's v.
     subroutine advect_vof(v, dt)
         use fen_capi
-        use navier_stokes_mod, only : fen_ctx, fen_push_host, fen_device_modified
+        use navier_stokes_mod, only : fen_ctx, fen_push_host, fen_device_modified, push_x_first, pull_x_first
         type(vector), intent(in) :: v
         real(dp), intent(in) :: dt
         if (.not. associated(v%G)) return
         call fen_push_host()
+        call push_x_first()
         call fen_check(fen_ctx, fen_advect_vof(fen_ctx, dt), 'advect_vof')
+        call pull_x_first()
         call fen_device_modified()
     end subroutine advect_vof
 
@@ -1267,64 +1295,42 @@ contains
         call temp%destroy()
     end subroutine save_fields
 
-    !> Restart file data/state_<step>.raw: p, u, v, dv_o%x, dv_o%y [, w, dv_o%z],
-    !> each the raw float64 interior in Fortran order (src/solver.f90:160-240).
+    !> Restart file data/state_<step>.raw: p, u, v, dv_o%x, dv_o%y [, w, dv_o%z]
+    !> [, vof, rho, mu, p_o in the two-phase build], each the raw float64
+    !> interior in Fortran order (src/solver.f90:160-329), written and read by
+    !> the library from the resident fields (fen_save_state, fen_load_state).
     !> dir replaces data/ (no counterpart in the reference).
     subroutine save_state(step, dir)
-        use navier_stokes_mod, only : v, dv_o, p, fen_sync_host
+        use iso_c_binding, only : c_null_char
+        use fen_capi, only : fen_save_state, fen_check
+        use navier_stokes_mod, only : fen_ctx, fen_push_host
         integer, intent(in) :: step
         character(len=*), intent(in), optional :: dir
-        integer :: u, lo(3), hi(3)
         character(len=7) :: sn
         character(len=:), allocatable :: d
         d = 'data'
         if (present(dir)) d = trim(dir)
-        call fen_sync_host()
-        lo = p%G%lo
-        hi = p%G%hi
         write(sn, '(I0.7)') step
-        open(newunit = u, file = d//'/state_'//sn//'.raw', access = 'stream', form = 'unformatted', &
-             status = 'replace')
-        write(u) p%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        write(u) v%x%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        write(u) v%y%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        write(u) dv_o%x%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        write(u) dv_o%y%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-#if DIM==3
-        write(u) v%z%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        write(u) dv_o%z%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-#endif
-        close(u)
+        call fen_push_host()
+        call fen_check(fen_ctx, fen_save_state(fen_ctx, d//'/state_'//sn//'.raw'//c_null_char), 'save_state')
     end subroutine save_state
 
     subroutine load_state(step, dir)
-        use navier_stokes_mod, only : v, dv_o, p, fen_sync_host, fen_host_modified
+        use iso_c_binding, only : c_null_char
+        use fen_capi, only : fen_load_state, fen_check
+        use navier_stokes_mod, only : fen_ctx, fen_push_host, fen_device_modified
         integer, intent(in) :: step
         character(len=*), intent(in), optional :: dir
-        integer :: u, lo(3), hi(3)
         character(len=7) :: sn
         character(len=:), allocatable :: d
         d = 'data'
         if (present(dir)) d = trim(dir)
-        call fen_sync_host()
-        lo = p%G%lo
-        hi = p%G%hi
         write(sn, '(I0.7)') step
-        open(newunit = u, file = d//'/state_'//sn//'.raw', access = 'stream', form = 'unformatted', &
-             status = 'old')
-        read(u) p%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        read(u) v%x%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        read(u) v%y%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        read(u) dv_o%x%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        read(u) dv_o%y%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-#if DIM==3
-        read(u) v%z%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-        read(u) dv_o%z%f(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3))
-#endif
-        close(u)
-        call p%update_ghost_nodes()
-        call v%update_ghost_nodes()
-        call fen_host_modified()
+        ! what the host holds goes first, so the loaded state is not overwritten by it
+        call fen_push_host()
+        call fen_check(fen_ctx, fen_load_state(fen_ctx, d//'/state_'//sn//'.raw'//c_null_char), 'load_state')
+        ! the ghost fills of p and v (and the two-phase fields) ran on the device
+        call fen_device_modified()
     end subroutine load_state
 
     subroutine destroy_solver
@@ -1429,7 +1435,7 @@ contains
 #ifdef MF
         use navier_stokes_mod, only : rho, mu
         use multiphase_mod, only : rho_0, rho_1, mu_0, mu_1, sigma, update_material_properties
-        use volume_of_fluid_mod, only : distance, vof
+        use volume_of_fluid_mod, only : distance, vof, x_first
 #endif
         type(fen_case), intent(in) :: cs
         type(grid), intent(inout), target :: comp_grid
@@ -1480,6 +1486,10 @@ contains
         if (cs%restart >= 0) then
             call load_state(cs%restart, case_text(cs%restart_dir))
             step0 = cs%restart
+#ifdef MF
+            ! each advection flips the sweep order, which starts x first
+            x_first = mod(step0, 2) == 0
+#endif
         else
             if (len(case_text(cs%init(:, 1))) > 0) call v%x%read(case_text(cs%init(:, 1)))
             if (len(case_text(cs%init(:, 2))) > 0) call v%y%read(case_text(cs%init(:, 2)))
@@ -1492,7 +1502,7 @@ contains
             call fen_host_modified()
         end if
 #ifdef MF
-        if (len(case_text(cs%init(:, 5))) > 0) then
+        if (cs%restart < 0 .and. len(case_text(cs%init(:, 5))) > 0) then
             call vof%read(case_text(cs%init(:, 5)))
             call vof%update_ghost_nodes()
             call fen_host_modified()

@@ -17,6 +17,7 @@ Arrays are numpy float64 in Fortran order with FEN's ghosted shapes.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -184,6 +185,18 @@ class Solver:
     def advect_vof(self, dt):
         self._check(self.lib.fen_advect_vof(self.h, dt))
 
+    @property
+    def x_first(self):
+        """volume_of_fluid_mod's x_first (src/volume_of_fluid.f90:30): the
+        vof's x sweep runs first at the next advection; each advection flips it."""
+        v = ctypes.c_int(0)
+        self._check(self.lib.fen_get_vof_x_first(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
+    @x_first.setter
+    def x_first(self, flag):
+        self._check(self.lib.fen_set_vof_x_first(self.h, 1 if flag else 0))
+
     def synchronize(self):
         self._check(self.lib.fen_synchronize(self.h))
 
@@ -212,6 +225,14 @@ class Solver:
         if rc not in (_lib.FEN_OK, -4):
             self._check(rc)
         return buf.value.decode()
+
+    def save_state(self, path):
+        """fen_save_state: the restart file of the resident state."""
+        self._check(self.lib.fen_save_state(self.h, os.fsencode(path)))
+
+    def load_state(self, path):
+        """fen_load_state: the resident state from a restart file."""
+        self._check(self.lib.fen_load_state(self.h, os.fsencode(path)))
 
     def write_grid_json(self, path):
         self._check(self.lib.fen_write_grid_json(self.h, path.encode()))

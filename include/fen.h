@@ -180,6 +180,16 @@ int fen_update_material_properties(fen_ctx *ctx);
  * velocity, as the reference's volume-of-fluid tests call it. */
 int fen_advect_vof(fen_ctx *ctx, double dt);
 
+/* The order of the vof's directional sweeps: volume_of_fluid_mod's x_first
+ * (src/volume_of_fluid.f90:30), x sweep first when 1.  Every advection
+ * (fen_ns_step, fen_ns_run, fen_advect_vof) flips it (:488, :532); it starts
+ * at 1 with fen_set_multiphase and is not part of the restart file, so a
+ * restarted run sets it as the reference's restart test does
+ * (test/small_test/io/test_MF.f90:75-80).  FEN_ERR_STATE outside the
+ * two-phase mode. */
+int fen_set_vof_x_first(fen_ctx *ctx, int x_first);
+int fen_get_vof_x_first(fen_ctx *ctx, int *x_first);
+
 /* Replaces the solve_poisson procedure pointer (src/poisson.f90:21-29):
  * solves lap(phi) = f in place on a host array with ghost level gl (0 or 1),
  * with the same whole-array semantics as the selected reference solver. */
@@ -203,6 +213,16 @@ int fen_poisson_solve_noalloc(fen_ctx *ctx, double *f, int gl);
  * the resident velocity: rx/ry/rz are interior-sized host arrays updated as
  * RHS = RHS - (advective terms). rz may be NULL in 2-D. */
 int fen_add_advection(fen_ctx *ctx, double *rx, double *ry, double *rz);
+
+/* Replace save_state / load_state (src/solver.f90:160-329): the restart
+ * file at `path` holds the interiors of p, u, v, dv_o%x, dv_o%y, then in 3-D
+ * w, dv_o%z, then in the two-phase mode vof, rho, mu, p_o, each as raw
+ * float64 in Fortran order (the reference's order and format).  Loading
+ * replaces those interiors and refills the ghosts of p and v (and of the
+ * two-phase fields), as the reference does; a file whose size does not
+ * match this context's grid and mode is refused with FEN_ERR_ARG. */
+int fen_save_state(fen_ctx *ctx, const char *path);
+int fen_load_state(fen_ctx *ctx, const char *path);
 
 /* Replaces print_navier_stokes_solver_status (src/navier_stokes.f90:734-748):
  * formats the status line (no newline) into buf. */

@@ -186,3 +186,52 @@ def test_fortran_rayleigh_taylor_matches_oracle(tmp_path):
         err = rel_err(got, ref)
         print(f"Fortran Rayleigh-Taylor {n[0]}x{n[1]}, {steps} steps: {name} rel err {err:.2e}")
         assert err < 1e-9, name
+
+
+@pytest.mark.gpu
+def test_fortran_two_phase_restart_is_bitwise(tmp_path):
+    """The two-phase restart criterion (test/small_test/io/test_MF.f90 with
+    run.sh): 10 steps saving the state after steps 5 and 10, and a run
+    restarted from step 5 that saves after step 10, write identical files
+    (p, u, v, dv_o, vof, rho, mu, p_o).  The file does not hold the vof's
+    sweep order; the restarted case sets it from the step count."""
+    nx = 16
+    grid = {"Nx": nx, "Ny": 4 * nx, "Nz": 1, "Lx": 1.0, "Ly": 4.0,
+            "boundary_conditions": ["Periodic", "Periodic", "Wall", "Wall"]}
+    common = dict(timestep={"U": 1.0}, Physics={"viscosity": 1.0, "g": [0.0, -9.80665, 0.0]},
+                  Multiphase={"rho_0": 0.1694, "rho_1": 1.225, "mu_0": 3.13e-3, "mu_1": 3.13e-3,
+                              "interface": {"level": 2.0, "amplitude": 0.05, "wavelength": 1.0}})
+    (tmp_path / "data").mkdir()
+    (tmp_path / "again").mkdir()
+    first = _case(tmp_path / "first.json", grid, 10, run={"save_state": [5, 10], "status_every": 0}, **common)
+    second = _case(tmp_path / "second.json", grid, 5, Restart={"step": 5, "dir": "data"},
+                   run={"save_state": [10], "data_dir": "again", "status_every": 0}, **common)
+    exe = _bin("run_case_2d_mf")
+    subprocess.run([exe, str(first)], cwd=tmp_path, check=True, timeout=120)
+    subprocess.run([exe, str(second)], cwd=tmp_path, check=True, timeout=120)
+    a = (tmp_path / "data" / "state_0000010.raw").read_bytes()
+    b = (tmp_path / "again" / "state_0000010.raw").read_bytes()
+    assert len(a) == 9 * nx * 4 * nx * 8 and a == b
+
+
+@pytest.mark.gpu
+def test_fortran_stops_on_nonfinite(tmp_path):
+    """A NaN in the initial velocity reaches the first step's checks; the
+    Fortran layer maps FEN_ERR_NONFINITE to the library's message on stderr
+    and an error stop (a non-zero exit), as its error path does for every
+    failed call."""
+    import math
+    N = 16
+    case = cases.tgv3d(N, steps=2)
+    u = case.fields[U].copy(order="F")
+    u[5, 6, 7] = math.nan
+    for name, arr in (("u", u), ("v", case.fields[V]), ("w", case.fields[W]), ("p", case.fields[P])):
+        _raw(arr, tmp_path / f"u0_{name}.raw")
+    L = 2 * np.pi
+    path = _case(tmp_path / "case.json", {"Nx": N, "Ny": N, "Nz": N, "Lx": L, "Ly": L, "Lz": L}, 2,
+                 timestep={"dt": case.dt}, Initial={k: f"u0_{k}.raw" for k in "uvwp"})
+    r = subprocess.run([_bin("run_case_3d"), str(path)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "navier_stokes_solver failed with status -4" in r.stderr, r.stderr
+    assert "non-finite velocity or divergence" in r.stderr, r.stderr
+    assert not [l for l in r.stdout.splitlines() if l.startswith("step:")]   # no status line was printed
