@@ -298,6 +298,8 @@ void free_all(fen_ctx *c)
     f(c->d_mwx); f(c->d_mwy); f(c->d_mwz); f(c->d_ta); f(c->d_tb); f(c->d_tc);
     f(c->d_twx); f(c->d_twy); f(c->d_twz); f(c->d_spec2); f(c->d_mkx); f(c->d_mky); f(c->d_part); f(c->d_scal);
     for (int s = 0; s < 5; s++) for (int p = 0; p < 6; p++) f(c->d_plane[s][p]);
+    f(c->d_plane_tab);
+    f(c->d_wallbuf);
     if (c->mf) {
         MFState &m = *c->mf;
         for (double *q : {m.vof, m.vof1, m.vof2, m.h, m.dd, m.curv, m.nrx, m.nry, m.lx, m.ly, m.rho, m.mu, m.p_hat,
@@ -704,6 +706,14 @@ int fen_create(fen_ctx **out, int dim, int nx, int ny, int nz, double lx, double
         c->d_phi = dev_alloc_bytes(c->n_ghost * es);
         for (int s = 0; s < 5; s++)
             for (int f = 0; f < 6; f++) c->d_plane[s][f] = dev_alloc_bytes(plane_count(c, f) * es);
+        {   // the planes of u, v, w, p in a device table (the general-wall fused kernel reads them there)
+            void *tab[24];
+            const int fid[4] = {FEN_U, FEN_V, FEN_W, FEN_P};
+            for (int q = 0; q < 4; q++)
+                for (int f = 0; f < 6; f++) tab[6 * q + f] = c->d_plane[fid[q]][f];
+            c->d_plane_tab = static_cast<void **>(dev_alloc_bytes(sizeof tab));
+            FEN_HIP_CHECK(hipMemcpy(c->d_plane_tab, tab, sizeof tab, hipMemcpyHostToDevice));
+        }
         const size_t nspec = (size_t)c->hx * ny * nz;
         c->d_spec = dev_alloc_bytes(nspec * 2 * es);
         const size_t hy = (size_t)ny / 2 + 1;
@@ -871,16 +881,6 @@ int fen_advect_vof(fen_ctx *c, double dt)
     return FEN_OK;
 }
 
-int fen_set_dt_o(fen_ctx *c, double dt_o)
-{
-    if (!c) return FEN_ERR_ARG;
-    c->dt_o = dt_o;
-    c->have_dt_o = true;
-    return FEN_OK;
-}
-
-int fen_get_scalars(fen_ctx *c, double *maxdiv, double *maxcfl, double *dt_o, double *dt_visc, double *dt_conv)
-{
 int fen_set_vof_x_first(fen_ctx *c, int x_first)
 {
     if (!c) return FEN_ERR_ARG;
@@ -897,6 +897,16 @@ int fen_get_vof_x_first(fen_ctx *c, int *x_first)
     return FEN_OK;
 }
 
+int fen_set_dt_o(fen_ctx *c, double dt_o)
+{
+    if (!c) return FEN_ERR_ARG;
+    c->dt_o = dt_o;
+    c->have_dt_o = true;
+    return FEN_OK;
+}
+
+int fen_get_scalars(fen_ctx *c, double *maxdiv, double *maxcfl, double *dt_o, double *dt_visc, double *dt_conv)
+{
     if (!c) return FEN_ERR_ARG;
     FEN_TRY(c, {
         if (dt_o) *dt_o = c->dt_o;

@@ -189,6 +189,8 @@ struct fen_ctx {
     // bc types and device bc planes (element type R) of p, phi, u, v, w
     int bc_type[5][6] = {};
     void *d_plane[5][6] = {};
+    void **d_plane_tab = nullptr;   // d_plane of u, v, w, p (6 each) in device memory
+    void *d_wallbuf = nullptr;      // the general-wall fused kernel's face buffer (allocated on first use)
 
     // resident fields, element type R (fen::as<R>(...) at the use site)
     void *d_u[3] = {nullptr, nullptr, nullptr};    // velocity (state)

@@ -53,45 +53,52 @@ __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
     constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), TJ = 2 * L, LL = lds_line<N>();
     __shared__ double2 buf[L * LL];
     __shared__ double2 tw[N];
-    load_twiddles<N>(tw, a.tw);   // first read after the first stage's barrier
+    load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
-    // thread (l, t) forms Makhoul's reordered rows 2l, 2l+1 at the first FFT
-    // stage's positions p = t + q N/8 straight into registers:
-    // v_p = x_{2p} (p < N/2), x_{2(N-1-p)+1} (p >= N/2)
-    const int l = threadIdx.x / T, t = threadIdx.x % T;
-    double2 v[8];
+    // staging: the loads of BATCH elements are issued before their LDS
+    // writes; adjacent threads read adjacent x, so every load is coalesced
+    // (Makhoul's reorder puts even and odd x in different halves of the
+    // line, so the FFT's first stage cannot take them straight from memory)
+    constexpr int PER = TJ * N / kNB, BATCH = PER < 8 ? PER : 8;
+    static_assert(PER * kNB == TJ * N && PER % BATCH == 0, "staging batches");
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const long r0 = row0 + 2 * l + h;
-        const bool ok = r0 < a.nrows;
-        const long r = ok ? r0 : row0;   // a valid row: the loads are kept, the value dropped
-        // rows number ny*nz < 2^31: 32-bit division (a 64-bit one is a long routine)
-        const int ri = (int)r, j = ri % a.b.ny + 1, k = ri / a.b.ny + 1;
+    for (int b0 = 0; b0 < PER; b0 += BATCH) {
+        double vals[BATCH];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int p = t + q * (N / 8);
-            const int i = p < N / 2 ? 2 * p : 2 * (N - 1 - p) + 1;
+        for (int m = 0; m < BATCH; m++) {
+            const int e = threadIdx.x + (b0 + m) * kNB;
+            const int row = e / N, i = e % N;
+            const long r0 = row0 + row;
+            const bool ok = r0 < a.nrows;
+            const long r = ok ? r0 : row0;   // a valid row: the loads are kept, the value dropped
             double val;
             if constexpr (SRC == 2) {
                 val = a.u[r * N + i];
-            } else if constexpr (SRC == 1) {
-                val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
             } else {
-                const long g = a.b.g(i + 1, j, k);
-                double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
-                if (a.d3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
-                val = d * a.rho / a.dt;
+                // rows number ny*nz < 2^31: 32-bit division (a 64-bit one is a long routine)
+                const int ri = (int)r, j = ri % a.b.ny + 1, k = ri / a.b.ny + 1;
+                if constexpr (SRC == 1) {
+                    val = a.u[(i + a.gl) + a.sx * (j - 1 + a.gl) + a.sy * (k - 1 + a.gl)];
+                } else {
+                    const long g = a.b.g(i + 1, j, k);
+                    double d = (a.u[g] - a.u[g - 1]) * a.id + (a.v[g] - a.v[g - a.b.gx]) * a.id;
+                    if (a.d3) d = d + (a.w[g] - a.w[g - a.b.gxy]) * a.id;
+                    val = d * a.rho / a.dt;
+                }
             }
-            if (h == 0) v[q].x = ok ? val : 0.0;
-            else v[q].y = ok ? val : 0.0;
+            vals[m] = ok ? val : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < BATCH; m++) {
+            const int e = threadIdx.x + (b0 + m) * kNB;
+            const int row = e / N, i = e % N;
+            double2 *line = buf + (row >> 1) * LL;
+            if (row & 1) line[lds_pad(mk_pos(i, N))].y = vals[m];
+            else line[lds_pad(mk_pos(i, N))].x = vals[m];
         }
     }
-    double2 *line = buf + l * LL;
-    fft_regs<N, -1>(v, line, t, tw);
-    // y_k needs bins k and N - k of the line: through LDS
-#pragma unroll
-    for (int q = 0; q < 8; q++) line[lds_pad(t + q * (N / 8))] = v[q];
     __syncthreads();
+    fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
     // adjacent threads write adjacent addresses: rows (x pass, orr = 1) or
     // k (y pass, ok = 1)
     const bool k_fast = a.ok == 1;
@@ -99,8 +106,8 @@ __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
         const int row = k_fast ? e / N : e % TJ, k = k_fast ? e % N : e / TJ;
         const long r = row0 + row;
         if (r >= a.nrows) continue;
-        const double2 *ln = buf + (row >> 1) * LL;
-        const double2 zk = ln[lds_pad(k)], zn = ln[lds_pad((N - k) & (N - 1))];
+        const double2 *line = buf + (row >> 1) * LL;
+        const double2 zk = line[lds_pad(k)], zn = line[lds_pad((N - k) & (N - 1))];
         const double c = a.mk[k].x, s = a.mk[k].y;
         double y;
         if ((row & 1) == 0) y = c * (zk.x + zn.x) + s * (zk.y - zn.y);

@@ -569,16 +569,16 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
     PMArgs<R> a{};
     if (general) {
         a.wall = (c->periodic[0] ? 0 : 1) | (c->periodic[1] ? 0 : 2) | (c->periodic[2] ? 0 : 4);
-        // slots of bc_type / d_plane are the field ids FEN_P .. FEN_W
+        // packed per field: face f's bc type in bits 2f..2f+1, location in bits 12..13
         const int fid[4] = {FEN_U, FEN_V, FEN_W, FEN_P};
-        const int loc[4] = {'x', 'y', 'z', 'c'};
+        const int loc[4] = {1, 2, 3, 0};
         for (int q = 0; q < 4; q++) {
-            for (int f = 0; f < 6; f++) {
-                a.gd[q].plane[f] = as<R>(c->d_plane[fid[q]][f]);
-                a.gd[q].type[f] = c->bc_type[fid[q]][f];
-            }
-            a.gd[q].loc = loc[q];
+            a.gtype[q] = loc[q] << 12;
+            for (int f = 0; f < 6; f++) a.gtype[q] |= c->bc_type[fid[q]][f] << (2 * f);
         }
+        a.gplane = reinterpret_cast<const R *const *>(c->d_plane_tab);
+        if (!c->d_wallbuf) FEN_HIP_CHECK(hipMalloc(&c->d_wallbuf, 4 * pm_wall_layout(c->box).per_field * sizeof(R)));
+        a.wbuf = as<R>(c->d_wallbuf);
     }
     if (zwalls) {
         a.shift = c->shift_active ? c->d_shift : nullptr;
@@ -612,6 +612,11 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
     nbz = (c->box.nz + a.kb - 1) / a.kb;
     const long nblk = (long)a.m.nbx * a.m.nby * nbz;
     if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
+    if (general) {
+        // the state at the wall positions, read by the march below
+        const long nw = pm_wall_layout(c->box).per_field;
+        hipLaunchKernelGGL(wall_state_kernel<R>, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream, a);
+    }
     prof_begin(c, K_PROJ_MOM);
     const dim3 grid((unsigned)nblk), block(SH::FTX * SH::FTY);
 #define FEN_PM(SRC, ZW) \

@@ -331,14 +331,6 @@ __global__ __launch_bounds__(TX * TY, OCC) void project_march_kernel(ProjArgs<R>
 // planes are indexed at the cell's own, unwrapped (i, j)), w = bc (and the
 // last interior w is overwritten with bc), p = the neighbouring interior
 // value; a pending mean shift of phi is subtracted as the projection does.
-// Ghost rules of one state field for the general-wall form (ZW == 2): the
-// bc type and value plane of each face and the field's location.
-template <class R> struct PMGhost {
-    const R *plane[6];
-    int type[6];
-    int loc;   // 'c', 'x', 'y', 'z'
-};
-
 template <class R> struct PMArgs {
     MomArgs<R> m;                  // momentum of step n+1 (m.u, m.v, m.w, m.p unused)
     const R *__restrict__ us;      // u*, v*, w* of step n
@@ -353,49 +345,92 @@ template <class R> struct PMArgs {
     double pid, pdt, prhof, pirhof;
     int prpow2;
     int kb;                        // planes marched per workgroup
-    PMGhost<R> gd[4];              // ZW == 2: ghost rules of u, v, w, p
-    int wall;                      // ZW == 2: non-periodic axes (bit 0 x, 1 y, 2 z)
+    // ZW == 2 (general walls): non-periodic axes (bit 0 x, 1 y, 2 z); per
+    // state field u, v, w, p the bc type of face f in bits 2f..2f+1 and the
+    // location (0 centre, 1 x, 2 y, 3 z face) in bits 12..13; and the bc
+    // value planes, 6 per field in that order, in device memory
+    int wall;
+    int gtype[4];
+    const R *const *gplane;
+    R *wbuf;   // ZW == 2: the state at the wall positions (pm_wall_layout)
 };
 
-// The general-wall form's ghost value of one state field at ghosted
-// position (gi, gj, kk), given v = the field's projected value at the
-// interior cell the ghost fill reads, (ix, jy, kz) per axis: the ghost
-// kernel's x -> y -> z composition (S1, S2, S3) applied to that value, each
-// level with the value plane indexed at that level's coordinates, and the
-// high-face normal overwrite f(N) = bc.  Periodic axes only move the index.
+// The general-wall form (ZW == 2).  Along one wall axis a ghosted position
+// is the low ghost (0), the high ghost (n + 1) or the high face's
+// overwritten normal (n, f(N) = bc): pm_wall_pos gives 0, 1, 2 for these
+// and -1 elsewhere.
+__device__ __forceinline__ int pm_wall_pos(int g, int n)
+{
+    return g == 0 ? 0 : (g == n + 1 ? 1 : (g == n ? 2 : -1));
+}
+
+// The state at every wall position is formed before the march by
+// wall_state_kernel into a face buffer, one block per field of
+// pm_wall_layout().per_field values: x section [nz+2][ny+2][3] (kk, gj,
+// position px fastest: px = 1 and 2 read the same source cell, so a wave
+// reads each of its lines once), y section [3][nz+2][nx+2] (py, kk, gi),
+// z section [3][ny+2][nx+2] (pz, gj, gi).  Positions on several walls
+// (edges, corners) hold the same value in each section that has them.
+struct PMWallLayout {
+    long xs, ys, zs;   // section offsets
+    long xp, yp, zp;   // words per position plane
+    long per_field;
+};
+__host__ __device__ __forceinline__ PMWallLayout pm_wall_layout(const Box &b)
+{
+    PMWallLayout w;
+    w.xp = (long)(b.ny + 2) * (b.nz + 2);
+    w.yp = (long)(b.nx + 2) * (b.nz + 2);
+    w.zp = (long)(b.nx + 2) * (b.ny + 2);
+    w.xs = 0;
+    w.ys = 3 * w.xp;
+    w.zs = w.ys + 3 * w.yp;
+    w.per_field = w.zs + 3 * w.zp;
+    return w;
+}
+
+// The ghost value of one state field at ghosted position (gi, gj, kk),
+// given v = the field's projected value at the interior cell the ghost fill
+// reads, (ix, jy, kz) per axis: the ghost kernel's x -> y -> z composition
+// (S1, S2, S3) applied to that value, each level with the value plane
+// indexed at that level's coordinates, and the high-face normal overwrite
+// f(N) = bc.  Periodic axes only move the index.  gt: packed types and
+// location (PMArgs::gtype), pl: the field's 6 planes.
 template <class R>
-__device__ __forceinline__ R pm_ghost_chain(const PMGhost<R> &d, const Box &b, int wall, int gi, int gj, int kk,
-                                            int jy, int kz, R v)
+__device__ __forceinline__ R pm_ghost_chain(int gt, const R *const *pl, const Box &b, int wall, int gi, int gj,
+                                            int kk, int jy, int kz, R v)
 {
     const R two = R(2.0);
+    const int loc = (gt >> 12) & 3;   // 1 x, 2 y, 3 z, 0 centre
+    auto ty = [&](int f) { return (gt >> (2 * f)) & 3; };
     if (wall & 1) {   // S1(gi, jy, kz)
         const long pj = jy + (long)(b.ny + 2) * kz;
         if (gi == 0) {
-            if (d.type[0] == 1) v = d.loc == 'x' ? d.plane[0][pj] : two * d.plane[0][pj] - v;
+            if (ty(0) == 1) v = loc == 1 ? pl[0][pj] : two * pl[0][pj] - v;
         } else if (gi == b.nx + 1) {
-            if (d.type[1] == 1) v = d.loc == 'x' ? d.plane[1][pj] : two * d.plane[1][pj] - v;
-        } else if (gi == b.nx && d.type[1] == 1 && d.loc == 'x') {
-            v = d.plane[1][pj];
+            if (ty(1) == 1) v = loc == 1 ? pl[1][pj] : two * pl[1][pj] - v;
+        } else if (gi == b.nx && ty(1) == 1 && loc == 1) {
+            v = pl[1][pj];
         }
     }
     if (wall & 2) {   // S2(gi, gj, kz)
         const long pi = gi + (long)(b.nx + 2) * kz;
         if (gj == 0) {
-            if (d.type[2] == 1) v = d.loc == 'y' ? d.plane[2][pi] : two * d.plane[2][pi] - v;
+            if (ty(2) == 1) v = loc == 2 ? pl[2][pi] : two * pl[2][pi] - v;
         } else if (gj == b.ny + 1) {
-            if (d.type[3] == 1) v = d.loc == 'y' ? d.plane[3][pi] : two * d.plane[3][pi] - v;
-        } else if (gj == b.ny && d.type[3] == 1 && d.loc == 'y') {
-            v = d.plane[3][pi];
+            if (ty(3) == 1) v = loc == 2 ? pl[3][pi] : two * pl[3][pi] - v;
+        } else if (gj == b.ny && ty(3) == 1 && loc == 2) {
+            v = pl[3][pi];
         }
     }
     if (wall & 4) {   // S3(gi, gj, kk)
         const long pij = gi + (long)(b.nx + 2) * gj;
         if (kk == 0) {
-            if (d.type[4] == 1) v = d.loc == 'z' ? d.plane[4][pij] : two * d.plane[4][pij] - v;
+            if (ty(4) == 1) v = loc == 3 ? pl[4][pij] : two * pl[4][pij] - v;
         } else if (kk == b.nz + 1) {
-            if (d.type[5] == 1) v = d.loc == 'z' ? d.plane[5][pij] : two * d.plane[5][pij] - v;
-        } else if (kk == b.nz && d.type[5] == 1 && d.loc == 'z') {
-            v = d.plane[5][pij];
+            if (ty(5) == 1) v = loc == 3 ? pl[5][pij] : two * pl[5][pij] - v;
+        } else if (kk == b.nz && ty(5) == 1 && loc == 3) {
+            v = pl[5][pij];
         }
     }
     return v;
@@ -416,6 +451,13 @@ struct ProjectMomentumMarch {
     static constexpr int NT = S::NT, NRAW = 8;
     struct Raw {
         R v[NRAW][S::PER];
+        int wl;   // ZW == 2: bit q set when element q took its state from the face buffer
+    };
+    // ZW == 2: per tile element, its face buffer offset at plane 0 and the
+    // stride per plane when it sits on an x or y wall position (ofs < 0
+    // otherwise), and its offset in a z position plane
+    struct WallIdx {
+        int ofs[S::PER], str[S::PER], zo[S::PER];
     };
     struct PlaneIdx {   // ZW: bc plane index i + (nx+2) j of each tile element
         int v[S::PER];
@@ -460,7 +502,8 @@ struct ProjectMomentumMarch {
         return t;
     }
 
-    __device__ __forceinline__ static void fetch(Raw &r, const PMArgs<R> &a, const TileIdx<S> &t, int kk)
+    __device__ __forceinline__ static void fetch(Raw &r, const PMArgs<R> &a, const TileIdx<S> &t,
+                                                 const WallIdx &wi, int kk)
     {
         const Box &b = a.m.b;
         // the interior plane the state at plane kk is formed from
@@ -468,13 +511,32 @@ struct ProjectMomentumMarch {
                        : ZW == 2 ? pm_src(kk, b.nz, a.wall & 4)
                                  : (kk == 0 ? b.nz : (kk == b.nz + 1 ? 1 : kk));
         const long kofs = (long)kw * b.gxy, sj = b.gx, sk = b.gxy;
+        // ZW == 2: elements on a wall position load u, v, w, p of the face
+        // buffer instead (a z wall plane: all of them)
+        [[maybe_unused]] int pz = -1;
+        [[maybe_unused]] PMWallLayout L{};
+        if constexpr (ZW == 2) {
+            pz = (a.wall & 4) ? pm_wall_pos(kk, b.nz) : -1;
+            L = pm_wall_layout(b);
+            r.wl = 0;
+        }
 #pragma unroll
         for (int q = 0; q < S::PER; q++) {
             const long o = t.off[q] + kofs;
-            r.v[0][q] = a.us[o];
-            r.v[1][q] = a.vs[o];
-            r.v[2][q] = a.ws[o];
-            r.v[3][q] = a.p_in[o];
+            const R *su = a.us + o, *sv = a.vs + o, *sw = a.ws + o, *sp = a.p_in + o;
+            if constexpr (ZW == 2) {
+                const bool wl = pz >= 0 || wi.ofs[q] >= 0;
+                const long ow = pz >= 0 ? L.zs + pz * L.zp + wi.zo[q] : wi.ofs[q] + (long)wi.str[q] * kk;
+                if (wl) {
+                    const R *w0 = a.wbuf + ow;
+                    su = w0; sv = w0 + L.per_field; sw = w0 + 2 * L.per_field; sp = w0 + 3 * L.per_field;
+                    r.wl |= 1 << q;
+                }
+            }
+            r.v[0][q] = *su;
+            r.v[1][q] = *sv;
+            r.v[2][q] = *sw;
+            r.v[3][q] = *sp;
             r.v[4][q] = a.phi[o];
             r.v[5][q] = a.phi[o + 1];
             r.v[6][q] = a.phi[o + sj];
@@ -486,7 +548,7 @@ struct ProjectMomentumMarch {
     // rules when ZW) into ring slot `slot`.
     template <class SL>
     __device__ __forceinline__ static void commit(R *lds, SL slot, const Raw &r, const PMArgs<R> &a,
-                                                  const PlaneIdx &pi, int kk, double sh, int i0, int j0)
+                                                  const PlaneIdx &pi, int kk, double sh)
     {
         auto dr = [&](double x) { return over_rho(x, a.prhof, a.pirhof, a.prpow2); };
         const int nz = a.m.b.nz;
@@ -502,19 +564,8 @@ struct ProjectMomentumMarch {
             R wn = (R)((double)r.v[2][q] - dr((fz - f0) * a.pid * a.pdt));
             R pn = (R)((double)r.v[3][q] + f0);
             if constexpr (ZW == 2) {
-                // ghost and overwritten positions of a wall axis: the ghost
-                // fill's rule chain on the projected source values
-                const Box &b = a.m.b;
-                const int ri = e % S::LX, rj = e / S::LX;
-                const int gi = min(i0 - 1 + ri, b.nx + 1), gj = min(j0 - 1 + rj, b.ny + 1);
-                const int w = a.wall;
-                if (((w & 1) && (gi == 0 || gi >= b.nx)) || ((w & 2) && (gj == 0 || gj >= b.ny)) ||
-                    ((w & 4) && (kk == 0 || kk >= b.nz))) {
-                    const int jy = pm_src(gj, b.ny, w & 2), kz = pm_src(kk, b.nz, w & 4);
-                    un = pm_ghost_chain(a.gd[0], b, w, gi, gj, kk, jy, kz, un);
-                    vn = pm_ghost_chain(a.gd[1], b, w, gi, gj, kk, jy, kz, vn);
-                    wn = pm_ghost_chain(a.gd[2], b, w, gi, gj, kk, jy, kz, wn);
-                    pn = pm_ghost_chain(a.gd[3], b, w, gi, gj, kk, jy, kz, pn);
+                if (r.wl & (1 << q)) {   // the face buffer's final state
+                    un = r.v[0][q]; vn = r.v[1][q]; wn = r.v[2][q]; pn = r.v[3][q];
                 }
             }
             if constexpr (ZW == 1) {
@@ -541,16 +592,49 @@ struct ProjectMomentumMarch {
         }
     }
 
+    // ZW == 2: the face buffer offsets of this thread's tile elements.
+    __device__ __forceinline__ static WallIdx wall_index(const PMArgs<R> &a, int i0, int j0)
+    {
+        WallIdx wi{};
+        if constexpr (ZW == 2) {
+            const Box &b = a.m.b;
+            const PMWallLayout L = pm_wall_layout(b);
+#pragma unroll
+            for (int q = 0; q < S::PER; q++) {
+                const int e = threadIdx.x + q * S::NT;
+                const int ri = e % S::LX, rj = e / S::LX;
+                int gi = min(i0 - 1 + ri, b.nx + 1), gj = min(j0 - 1 + rj, b.ny + 1);
+                if (e >= S::N) { gi = i0; gj = j0; }
+                const int px = (a.wall & 1) ? pm_wall_pos(gi, b.nx) : -1;
+                const int py = (a.wall & 2) ? pm_wall_pos(gj, b.ny) : -1;
+                wi.zo[q] = gi + (b.nx + 2) * gj;
+                if (px >= 0) {
+                    wi.ofs[q] = (int)(L.xs + px + 3 * gj);
+                    wi.str[q] = 3 * (b.ny + 2);
+                } else if (py >= 0) {
+                    wi.ofs[q] = (int)(L.ys + py * L.yp + gi);
+                    wi.str[q] = b.nx + 2;
+                } else {
+                    wi.ofs[q] = -1;
+                    wi.str[q] = 0;
+                }
+            }
+        }
+        return wi;
+    }
+
     template <class SM, class SC, class SN>
     __device__ __forceinline__ static void plane(const PMArgs<R> &a, R *lds, Raw &r, const TileIdx<S> &t,
                                                  const PlaneIdx &pi, double sh, int i, int j, int k, int kend,
                                                  bool active, int c, R (&o)[3], double &md, double &mv, double &bad,
-                                                 bool xedge, SM sm, SC sc, SN sn, int i0, int j0)
+                                                 bool xedge, SM sm, SC sc, SN sn, const WallIdx &wi)
     {
         const MomArgs<R> &m = a.m;
-        commit(lds, sn, r, a, pi, k + 1, sh, i0, j0);   // state at plane k+1
+        commit(lds, sn, r, a, pi, k + 1, sh);   // state at plane k+1
         __syncthreads();
-        if (k + 2 <= kend + 1) fetch(r, a, t, k + 2);
+        if (k + 2 <= kend + 1) {
+            fetch(r, a, t, wi, k + 2);
+        }
         R on[3] = {R(0), R(0), R(0)};
         const long q = active ? m.b.q(i, j, k) : 0;
         const long qs = (long)m.b.nx * m.b.ny;
@@ -605,11 +689,12 @@ struct ProjectMomentumMarch {
         const PlaneIdx pi = plane_index(b, i0, j0);
         const double sh = (ZW == 1 && a.shift) ? a.shift[0] : 0.0;
         Raw r;
-        fetch(r, a, t, k0 - 1);
-        commit(lds, IC<0>{}, r, a, pi, k0 - 1, sh, i0, j0);
-        fetch(r, a, t, k0);
-        commit(lds, IC<1>{}, r, a, pi, k0, sh, i0, j0);
-        fetch(r, a, t, k0 + 1);
+        const WallIdx wi = wall_index(a, i0, j0);
+        fetch(r, a, t, wi, k0 - 1);
+        commit(lds, IC<0>{}, r, a, pi, k0 - 1, sh);
+        fetch(r, a, t, wi, k0);
+        commit(lds, IC<1>{}, r, a, pi, k0, sh);
+        fetch(r, a, t, wi, k0 + 1);
         R o[3] = {R(0), R(0), R(0)};
         if (active) {
             const long q = b.q(i, j, k0);
@@ -618,7 +703,7 @@ struct ProjectMomentumMarch {
         double md = -HUGE_VAL, mv = 0.0, bad = 0.0;
         int sm = 0, sc = 1, sn = 2;
         for (int k = k0; k <= kend; k++) {
-            plane(a, lds, r, t, pi, sh, i, j, k, kend, active, c, o, md, mv, bad, xedge, sm, sc, sn, i0, j0);
+            plane(a, lds, r, t, pi, sh, i, j, k, kend, active, c, o, md, mv, bad, xedge, sm, sc, sn, wi);
             const int tmp = sm;
             sm = sc; sc = sn; sn = tmp;
         }
@@ -631,6 +716,48 @@ __global__ __launch_bounds__(TX * TY, OCC) void project_momentum_kernel(PMArgs<R
 {
     __shared__ R lds[ring_size<TileShape<TY, TX>, 4>()];
     ProjectMomentumMarch<R, SRC, TY, ZW, TX>::run(a, lds);
+}
+
+// The general-wall form's face buffer: u, v, w, p of step n at every wall
+// position (pm_wall_layout), one thread per position.  Each is the
+// projection of the interior cell the ghost fill reads (commit()'s
+// arithmetic, so interior positions such as the overwritten face n match
+// the march's own values bit for bit), then the ghost fill's rule chain.
+template <class R>
+__global__ __launch_bounds__(256) void wall_state_kernel(PMArgs<R> a)
+{
+    const Box &b = a.m.b;
+    const PMWallLayout L = pm_wall_layout(b);
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= L.per_field) return;
+    auto coord = [](int pos, int n) { return pos == 0 ? 0 : (pos == 1 ? n + 1 : n); };
+    int gi, gj, kk;
+    if (idx < L.ys) {
+        if (!(a.wall & 1)) return;
+        const long pos = idx % 3, rem = idx / 3;
+        gi = coord((int)pos, b.nx); gj = (int)(rem % (b.ny + 2)); kk = (int)(rem / (b.ny + 2));
+    } else if (idx < L.zs) {
+        if (!(a.wall & 2)) return;
+        const long pos = (idx - L.ys) / L.yp, rem = (idx - L.ys) % L.yp;
+        gj = coord((int)pos, b.ny); gi = (int)(rem % (b.nx + 2)); kk = (int)(rem / (b.nx + 2));
+    } else {
+        if (!(a.wall & 4)) return;
+        const long pos = (idx - L.zs) / L.zp, rem = (idx - L.zs) % L.zp;
+        kk = coord((int)pos, b.nz); gi = (int)(rem % (b.nx + 2)); gj = (int)(rem / (b.nx + 2));
+    }
+    const int ix = pm_src(gi, b.nx, a.wall & 1), jy = pm_src(gj, b.ny, a.wall & 2), kz = pm_src(kk, b.nz, a.wall & 4);
+    const long o = ix + (long)b.gx * jy + (long)b.gxy * kz;
+    auto dr = [&](double x) { return over_rho(x, a.prhof, a.pirhof, a.prpow2); };
+    const double f0 = (double)a.phi[o], fx = (double)a.phi[o + 1], fy = (double)a.phi[o + b.gx],
+                 fz = (double)a.phi[o + b.gxy];
+    R st[4];
+    st[0] = (R)((double)a.us[o] - dr((fx - f0) * a.pid * a.pdt));
+    st[1] = (R)((double)a.vs[o] - dr((fy - f0) * a.pid * a.pdt));
+    st[2] = (R)((double)a.ws[o] - dr((fz - f0) * a.pid * a.pdt));
+    st[3] = (R)((double)a.p_in[o] + f0);
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+        a.wbuf[f * L.per_field + idx] = pm_ghost_chain(a.gtype[f], a.gplane + 6 * f, b, a.wall, gi, gj, kk, jy, kz, st[f]);
 }
 
 
