@@ -395,7 +395,8 @@ template <class R> void x_backward(fen_ctx *c, R *phi, int gl, long sx, long sy,
     if constexpr (std::is_same_v<R, double>) {
         if (x_cosine(c)) {
             // nn divides by 2nx after the DCT-III (src/poisson.f90:594)
-            fen::launch_dct3_x(c, phi, gl, sx, sy, c->solver == FEN_SOLVER_NN ? 1.0 / (2.0 * (double)c->nx) : 1.0);
+            fen::launch_dct3_x(c, phi, gl, sx, sy, c->solver == FEN_SOLVER_NN ? 1.0 / (2.0 * (double)c->nx) : 1.0,
+                               img != 0);
             return;
         }
     }
@@ -414,8 +415,9 @@ template <class R> void solve_phi(fen_ctx *c, double dt, void *const *us)
     bool dirichlet = false;
     for (int f = 0; f < 6; f++) dirichlet = dirichlet || c->bc_type[FEN_PHI][f] == 1;
     // images written by the c2r only when no shift has to be applied first;
-    // the DCT-III writes none
-    const int img_phi = ((mean && dirichlet) || x_cosine(c)) ? 0 : c->img_phi;
+    // the DCT-III writes every ghost when they are all copies, else none
+    const int img_phi = x_cosine(c) ? (fen::dct3_writes_ghosts(c) ? 7 : 0)
+                                    : ((mean && dirichlet) ? 0 : c->img_phi);
     x_backward<R>(c, as<R>(c->d_phi), 1, c->box.gx, c->box.gxy, mean, img_phi);
     c->shift_active = mean;
     if (mean && dirichlet) {   // the shift does not commute with 2*bc - f: apply it first
@@ -527,10 +529,11 @@ template <class R> void step_pending(fen_ctx *c, double dt)
         for (int d = 0; d < 3; d++) std::swap(c->d_u[d], c->d_us[d]);
         std::swap(c->d_p, c->d_p2);
         c->config_version++;   // a captured step graph holds the old array roles
-        if (const int rest = remaining_groups(c, c->img_vel)) {   // z walls: ghosts of u*, v*, w*
+        // ghosts of u*, v*, w* the momentum did not write as images (walls)
+        if (const int fill = remaining_groups(c, c->img_vel)) {
             GhostDesc<R> g[3] = {make_desc<R>(c, FEN_U, c->d_us[0]), make_desc<R>(c, FEN_V, c->d_us[1]),
                                  make_desc<R>(c, FEN_W, c->d_us[2])};
-            for (auto &d : g) d.groups = rest;
+            for (auto &d : g) d.groups = fill;
             launch_ghost_desc<R>(c, g, 3);
         }
     }

@@ -307,7 +307,10 @@ long stencil_blocks(const Box &b);   // workgroups of the momentum / projection 
 // Neumann-direction solver passes (neumann_kernels.hip, FP64).
 void launch_dct2_x(fen_ctx *c, const double *const *vel, const double *direct, long sx, long sy, int gl, double rho,
                    double dt, double post);
-void launch_dct3_x(fen_ctx *c, double *phi, int gl, long sx, long sy, double post);
+// Whether every ghost of phi is a copy of an interior value (x Neumann;
+// y, z periodic or Neumann), so that the x DCT-III can write them.
+bool dct3_writes_ghosts(const fen_ctx *c);
+void launch_dct3_x(fen_ctx *c, double *phi, int gl, long sx, long sy, double post, bool img = false);
 void launch_dct_y(fen_ctx *c, int inverse, double post);
 void launch_rfft_y(fen_ctx *c, int inverse, double post);
 void launch_thomas_neumann(fen_ctx *c);

@@ -45,6 +45,34 @@ template <int SIGN, int M, typename C> __device__ __forceinline__ C c_w8(C a)
     else return C{h * (a.x - a.y), h * (a.x + a.y)};   // (m==7 fwd) or (m==1 inv)
 }
 
+// Two complex values carried together: two lines transformed by one thread
+// share each LDS access (16 bytes for two float2) and each twiddle.  Every
+// operation acts on each half as on a single value, so the two lines' results
+// are bitwise those of single-line transforms.
+template <typename C> struct CPair {
+    C a, b;
+};
+template <typename C> __device__ __forceinline__ CPair<C> c_add(CPair<C> x, CPair<C> y)
+{
+    return CPair<C>{c_add(x.a, y.a), c_add(x.b, y.b)};
+}
+template <typename C> __device__ __forceinline__ CPair<C> c_sub(CPair<C> x, CPair<C> y)
+{
+    return CPair<C>{c_sub(x.a, y.a), c_sub(x.b, y.b)};
+}
+template <typename C> __device__ __forceinline__ CPair<C> c_mul(CPair<C> x, C w)
+{
+    return CPair<C>{c_mul(x.a, w), c_mul(x.b, w)};
+}
+template <int SIGN, typename C> __device__ __forceinline__ CPair<C> c_rot(CPair<C> x)
+{
+    return CPair<C>{c_rot<SIGN>(x.a), c_rot<SIGN>(x.b)};
+}
+template <int SIGN, int M, typename C> __device__ __forceinline__ CPair<C> c_w8(CPair<C> x)
+{
+    return CPair<C>{c_w8<SIGN, M>(x.a), c_w8<SIGN, M>(x.b)};
+}
+
 template <int SIGN, typename C> __device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3)
 {
     C t0 = c_add(a0, a2), t1 = c_sub(a0, a2);
@@ -139,9 +167,10 @@ __device__ __forceinline__ void fft_lds(C *line, int t, const C *tw)
 // of four with LDS staging at both ends).  The arithmetic is fft_lds's, so
 // the results are bitwise equal.  `line` is this thread's line in LDS
 // (padded indexing); all threads of the block must call this (barriers),
-// and on return every read of `line` is complete.
-template <int N, int NS, int SIGN, typename C>
-__device__ __forceinline__ void fft_regs_stages(C (&v)[8], C *line, int t, const C *tw)
+// and on return every read of `line` is complete.  C may be a CPair of the
+// twiddles' type TW (two lines per thread).
+template <int N, int NS, int SIGN, typename C, typename TW>
+__device__ __forceinline__ void fft_regs_stages(C (&v)[8], C *line, int t, const TW *tw)
 {
     constexpr int REM = N / NS;
     constexpr int R = REM >= 8 ? 8 : REM;
@@ -165,7 +194,7 @@ __device__ __forceinline__ void fft_regs_stages(C (&v)[8], C *line, int t, const
         if constexpr (NS > 1) {
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                C w = tw[r * k * (N / (NS * R))];
+                TW w = tw[r * k * (N / (NS * R))];
                 if constexpr (SIGN > 0) w.y = -w.y;
                 v[m * R + r] = c_mul(v[m * R + r], w);
             }
@@ -195,8 +224,8 @@ __device__ __forceinline__ void fft_regs_stages(C (&v)[8], C *line, int t, const
     }
 }
 
-template <int N, int SIGN, typename C>
-__device__ __forceinline__ void fft_regs(C (&v)[8], C *line, int t, const C *tw)
+template <int N, int SIGN, typename C, typename TW>
+__device__ __forceinline__ void fft_regs(C (&v)[8], C *line, int t, const TW *tw)
 {
     fft_regs_stages<N, 1, SIGN>(v, line, t, tw);
     if constexpr (N > 8) __syncthreads();   // the last stage's LDS reads are done

@@ -127,9 +127,26 @@ struct Dct3Args {
     const double2 *tw, *mk;
     long nrows, ik, ir;
     long sx, sy;
-    int gl, ny;
+    int gl, ny, nz;
+    // x pass with gl == 1: also write the ghosts, when every ghost of phi
+    // is a copy (Neumann x faces; y, z periodic or Neumann): each row's
+    // x ghosts and the ghost rows that copy the row (yper/zper: periodic)
+    int img, yper, zper;
     double post;
 };
+
+// The ghosted indices that hold row index J (interior 1..n) of an axis whose
+// ghosts are copies: J itself, and the ghost rows 0 and n + 1 that copy it
+// (Neumann: 0 copies 1 and n + 1 copies n; periodic: 0 copies n and n + 1
+// copies 1).  Returns how many.
+__device__ __forceinline__ int copy_rows(int J, int n, int per, int (&d)[3])
+{
+    int m = 0;
+    d[m++] = J;
+    if (J == (per ? n : 1)) d[m++] = 0;
+    if (J == (per ? 1 : n)) d[m++] = n + 1;
+    return m;
+}
 
 template <int N, int DST>
 __global__ __launch_bounds__(kNB) void dct3_rows_kernel(Dct3Args a)
@@ -201,18 +218,39 @@ __global__ __launch_bounds__(kNB) void dct3_rows_kernel(Dct3Args a)
     for (int h = 0; h < 2; h++) {
         const long r = row0 + 2 * l + h;
         if (r >= a.nrows) continue;
-        double *orow;
         if constexpr (DST == 0) {
             const int ri = (int)r, j = ri % (int)a.ny, k = ri / (int)a.ny;   // 32-bit division
-            orow = a.out + a.gl + a.sx * (j + a.gl) + a.sy * (k + a.gl);
-        } else {
-            orow = a.out + r * N;
-        }
+            int jd[3], kd[3], nj = 1, nk = 1;
+            jd[0] = j + a.gl;
+            kd[0] = k + a.gl;
+            if (a.img) {
+                nj = copy_rows(j + 1, a.ny, a.yper, jd);
+                nk = copy_rows(k + 1, a.nz, a.zper, kd);
+            }
+            for (int dk = 0; dk < nk; dk++)
+                for (int dj = 0; dj < nj; dj++) {
+                    double *orow = a.out + a.gl + a.sx * jd[dj] + a.sy * kd[dk];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int p = t + q * (N / 8);
-            const int i = p < N / 2 ? 2 * p : 2 * (N - 1 - p) + 1;
-            orow[i] = (h ? v[q].y : v[q].x) * a.post;
+                    for (int q = 0; q < 8; q++) {
+                        const int p = t + q * (N / 8);
+                        const int i = p < N / 2 ? 2 * p : 2 * (N - 1 - p) + 1;
+                        orow[i] = (h ? v[q].y : v[q].x) * a.post;
+                    }
+                    // Neumann x ghosts: phi(0) = phi(1) is y_0 (position 0),
+                    // phi(N+1) = phi(N) is y_{N-1} (position N/2): thread 0's q = 0, 4
+                    if (a.img && t == 0) {
+                        orow[-1] = (h ? v[0].y : v[0].x) * a.post;
+                        orow[N] = (h ? v[4].y : v[4].x) * a.post;
+                    }
+                }
+        } else {
+            double *orow = a.out + r * N;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int p = t + q * (N / 8);
+                const int i = p < N / 2 ? 2 * p : 2 * (N - 1 - p) + 1;
+                orow[i] = (h ? v[q].y : v[q].x) * a.post;
+            }
         }
     }
 }
@@ -553,8 +591,20 @@ void launch_dct2_x(fen_ctx *c, const double *const *vel, const double *direct, l
     prof_end(c, K_DIV_R2C, (vel ? (c->dim == 3 ? 3.0 : 2.0) : 1.0) * 8.0 * c->n_int + 8.0 * c->n_int);
 }
 
-// x DCT-III of A[kx*nrows + r] into the interior rows of phi, times post.
-void launch_dct3_x(fen_ctx *c, double *phi, int gl, long sx, long sy, double post)
+bool dct3_writes_ghosts(const fen_ctx *c)
+{
+    if (c->box.dim != 3 || c->periodic[0]) return false;
+    for (int ax = 0; ax < 3; ax++)
+        for (int side = 0; side < 2; side++) {
+            const int t = c->bc_type[FEN_PHI][2 * ax + side];
+            if (c->periodic[ax] ? t != 0 : t != 2) return false;
+        }
+    return true;
+}
+
+// x DCT-III of A[kx*nrows + r] into the interior rows of phi, times post;
+// with img (gl == 1 and dct3_writes_ghosts) also every ghost of phi.
+void launch_dct3_x(fen_ctx *c, double *phi, int gl, long sx, long sy, double post, bool img)
 {
     Dct3Args a{};
     a.in = as<double>(c->d_spec);
@@ -568,6 +618,11 @@ void launch_dct3_x(fen_ctx *c, double *phi, int gl, long sx, long sy, double pos
     a.sy = sy;
     a.gl = gl;
     a.ny = c->ny;
+    a.nz = c->nz;
+    if (img && (gl != 1 || !dct3_writes_ghosts(c))) throw std::runtime_error("DCT-III ghost writes need gl == 1");
+    a.img = img ? 1 : 0;
+    a.yper = c->periodic[1] ? 1 : 0;
+    a.zper = c->periodic[2] ? 1 : 0;
     a.post = post;
     const unsigned grid = row_blocks(a.nrows, c->nx);
     prof_begin(c, K_C2R);

@@ -306,6 +306,47 @@ __global__ __launch_bounds__(NT) void z_solve_kernel(ZArgs<R> a)
     }
 }
 
+// The FP32 z-solve with two adjacent j lines per thread: each k row element
+// of the pair is one 16-byte load, store and LDS access, so the transforms
+// issue half the memory and LDS instructions of z_solve_kernel (whose
+// FP32 form moves 8 bytes per LDS access).  Same arithmetic per line.
+template <int N>
+__global__ __launch_bounds__(kBlock) void z_solve_pair_kernel(ZArgs<float> a)
+{
+    using P = CPair<float2>;
+    constexpr int L = kBlock / (N / 8), LL = lds_line<N>();
+    __shared__ P buf[L * LL];
+    __shared__ float2 tw[N];
+    load_twiddles<N>(tw, a.tw);   // first read after the first stage's barrier
+    const int kx = blockIdx.y;
+    const int l = threadIdx.x % L, t = threadIdx.x / L;
+    const int j = blockIdx.x * 2 * L + 2 * l;   // lines j, j + 1 (ny is a multiple of 2L)
+    float4 *col = reinterpret_cast<float4 *>(a.spec + kx * a.kxs + j);
+    const long rs = a.ny / 2;   // k row stride in pairs
+    P v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const float4 x = col[(long)(t + q * (N / 8)) * rs];
+        v[q] = P{float2{x.x, x.y}, float2{x.z, x.w}};
+    }
+    double mz[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) mz[q] = a.mwz[t + q * (N / 8)];
+    const double ma = a.mwx[kx] + a.mwy[j], mb = a.mwx[kx] + a.mwy[j + 1];
+    P *line = buf + l * LL;
+    fft_regs<N, -1>(v, line, t, tw);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const double sa = ma + mz[q], sb = mb + mz[q];
+        v[q].a = sa == 0.0 ? float2{0.0f, 0.0f} : scale_div(v[q].a, a.inv_n, sa);
+        v[q].b = sb == 0.0 ? float2{0.0f, 0.0f} : scale_div(v[q].b, a.inv_n, sb);
+    }
+    fft_regs<N, 1>(v, line, t, tw);
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        col[(long)(t + q * (N / 8)) * rs] = float4{v[q].a.x, v[q].a.y, v[q].b.x, v[q].b.y};
+}
+
 // ---------------------------------------------------------------------------
 // K4 (ppn): Thomas sweep along z for every (kx, j) column
 // (src/poisson.f90:1094-1135); one thread per column, adjacent threads own
@@ -562,6 +603,18 @@ template <class R> void launch_z_solve(fen_ctx *c, double inv_n)
     const int L = nt / (c->nz / 8);
     const dim3 grid((unsigned)((c->ny + L - 1) / L), (unsigned)c->hx);
     prof_begin(c, K_Z_SOLVE);
+    if constexpr (sizeof(R) == sizeof(float)) {
+        // two lines per thread: 2 * (kBlock / (nz / 8)) lines per workgroup
+        const int lp = 2 * (kBlock / (c->nz / 8));
+        if (c->ny % lp == 0) {
+            const dim3 gp((unsigned)(c->ny / lp), (unsigned)c->hx);
+#define L_ZP(NN) hipLaunchKernelGGL((z_solve_pair_kernel<NN>), gp, dim3(kBlock), 0, c->stream, a);
+            FEN_DISPATCH_N(c->nz, L_ZP)
+#undef L_ZP
+            prof_end(c, K_Z_SOLVE, 2.0 * spec_bytes<R>(c));
+            return;
+        }
+    }
 #define L_Z(NN)                                                                                       \
     if (nt == 512) hipLaunchKernelGGL((z_solve_kernel<R, NN, 512>), grid, dim3(512), 0, c->stream, a); \
     else hipLaunchKernelGGL((z_solve_kernel<R, NN, kBlock>), grid, dim3(kBlock), 0, c->stream, a);

@@ -25,13 +25,14 @@ import sys
 # demangled kernel name -> the library's profiler name (fen_profile_name)
 _PATTERNS = [
     (r"project_momentum_kernel", "project_momentum"),
+    (r"wall_state_kernel", "wall_state"),
     (r"momentum_march_kernel|momentum_kernel_2d", "momentum"),
     (r"project_march_kernel|project_kernel_2d", "project"),
     (r"div_r2c_kernel", "div_r2c"),
     (r"y_kernel<(?:double|float), \d+, -1, 0>", "y_fwd"),
     (r"y_kernel<(?:double|float), \d+, 1, 0>", "y_inv"),
     (r"y_kernel<(?:double|float), \d+, -?1, 1>", "y_solve"),
-    (r"z_solve_kernel", "z_solve"),
+    (r"z_solve_kernel|z_solve_pair_kernel", "z_solve"),
     (r"thomas_c1_kernel", "thomas_c1"),
     (r"thomas_kernel|thomas_general_kernel|thomas_column_kernel", "thomas"),
     (r"vof_reconstruct_kernel", "vof_reconstruct"),
