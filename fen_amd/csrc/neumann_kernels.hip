@@ -378,14 +378,20 @@ template <> __device__ __forceinline__ double2 zero_<double2>() { return double2
 
 constexpr int kGU = 8;   // rows loaded ahead per batch in the strided sweeps
 
-template <class E, int ASSOC2D>
+// SPLIT (E = double on complex columns): the real and imaginary parts of a
+// column are swept by two adjacent threads.  The coefficients are real, so
+// each part's recurrence is the complex one's for that part, bit for bit,
+// and the sweep runs on twice as many threads (the npn columns number only
+// nx (ny/2+1), half a wave per SIMD at 256^3).
+template <class E, int ASSOC2D, int SPLIT = 0>
 __global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
 {
-    const long t = (long)blockIdx.x * kNB + threadIdx.x;
-    if (t >= a.ncol) return;
+    const long tt = (long)blockIdx.x * kNB + threadIdx.x;
+    if (tt >= (SPLIT ? 2 : 1) * a.ncol) return;
+    const long t = SPLIT ? tt >> 1 : tt;
     const long o = t / a.ni, q = t % a.ni;
-    const long base = o * a.os + q, s = a.s;
-    E *x = static_cast<E *>(a.x) + base;
+    const long base = o * a.os + q, s = SPLIT ? 2 * a.s : a.s, cs = a.s;   // x and c1 strides
+    E *x = SPLIT ? static_cast<E *>(a.x) + 2 * base + (tt & 1) : static_cast<E *>(a.x) + base;
     double *c1 = a.c1 + base;
     const double mx = a.mwx[o], my = ASSOC2D ? 0.0 : a.mwy[q];
     const int n = a.n;
@@ -441,7 +447,7 @@ __global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
 #pragma unroll
         for (int u = 0; u < kGU; u++) {
             dk[u] = x[(long)(k - u) * s];
-            ck[u] = c1[(long)(k - u) * s];
+            ck[u] = c1[(long)(k - u) * cs];
         }
 #pragma unroll
         for (int u = 0; u < kGU; u++) {
@@ -450,7 +456,7 @@ __global__ __launch_bounds__(kNB) void thomas_general_kernel(ThomasGArgs a)
         }
     }
     for (; k >= 0; k--) {
-        xn = sub_(x[(long)k * s], mul_(xn, c1[(long)k * s]));
+        xn = sub_(x[(long)k * s], mul_(xn, c1[(long)k * cs]));
         x[(long)k * s] = xn;
     }
 }
@@ -756,8 +762,12 @@ void launch_thomas_neumann(fen_ctx *c)
             hipLaunchKernelGGL(thomas_column_kernel<double2>, dim3((unsigned)a.ncol), dim3(64), lds, c->stream, a);
         else
             hipLaunchKernelGGL(thomas_column_kernel<double>, dim3((unsigned)a.ncol), dim3(64), lds, c->stream, a);
-    } else if (complex_e && assoc2d) hipLaunchKernelGGL((thomas_general_kernel<double2, 1>), grid, dim3(kNB), 0, c->stream, a);
-    else if (complex_e) hipLaunchKernelGGL((thomas_general_kernel<double2, 0>), grid, dim3(kNB), 0, c->stream, a);
+    } else if (complex_e) {
+        // real and imaginary parts on adjacent threads
+        const dim3 g2((unsigned)((2 * a.ncol + kNB - 1) / kNB));
+        if (assoc2d) hipLaunchKernelGGL((thomas_general_kernel<double, 1, 1>), g2, dim3(kNB), 0, c->stream, a);
+        else hipLaunchKernelGGL((thomas_general_kernel<double, 0, 1>), g2, dim3(kNB), 0, c->stream, a);
+    }
     else if (assoc2d) hipLaunchKernelGGL((thomas_general_kernel<double, 1>), grid, dim3(kNB), 0, c->stream, a);
     else hipLaunchKernelGGL((thomas_general_kernel<double, 0>), grid, dim3(kNB), 0, c->stream, a);
     // columns read and written in each sweep, c1 read in the back substitution (LDS kernel: once each)
