@@ -567,6 +567,17 @@ void pending_checks(fen_ctx *c)
 
 void fen::advance_fp64(fen_ctx *c, double dt) { advance<double>(c, dt); }
 
+void fen::advance_fp64_into(fen_ctx *c, double dt, double *const *in, double *const *out)
+{
+    void *own[3] = {c->d_u[0], c->d_u[1], c->d_u[2]};
+    for (int d = 0; d < 3; d++) c->d_u[d] = in[d];
+    predict<double>(c, dt);
+    solve_phi<double>(c, dt, c->d_us);
+    for (int d = 0; d < 3; d++) c->d_u[d] = out[d];   // the projection writes the new state here
+    finish_step<double>(c, dt);
+    for (int d = 0; d < 3; d++) c->d_u[d] = own[d];
+}
+
 void fen::run_fp64(fen_ctx *c, int nsteps, double dt)
 {
     if (deferrable(c)) {

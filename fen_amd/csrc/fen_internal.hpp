@@ -320,6 +320,10 @@ void launch_advection_only(fen_ctx *c, double *rx, double *ry, double *rz);   //
 
 // One FP64 time step (the step fen_ns_step runs, without the CFL update).
 void advance_fp64(fen_ctx *c, double dt);
+// The same step reading the velocity from `in` (ghosted u, v, w) and
+// writing the new velocity to `out` instead of the context's arrays, which
+// keep their contents (p and dv_o are the context's, updated in place).
+void advance_fp64_into(fen_ctx *c, double dt, double *const *in, double *const *out);
 // nsteps FP64 steps as fen_ns_run takes them (fused when the case allows;
 // bitwise equal to nsteps calls of advance_fp64).
 void run_fp64(fen_ctx *c, int nsteps, double dt);
