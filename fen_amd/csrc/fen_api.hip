@@ -507,8 +507,9 @@ bool deferrable(const fen_ctx *c)
 }
 
 // One step that ends pending (fen_ctx::pending).  If the previous step is
-// pending, its projection, pressure update and checks run fused with this
-// step's momentum; the new u*, v*, w* land in the state arrays and p in the
+// pending, its projection and pressure update run fused with this step's
+// momentum (its checks, superseded by this step's, are not formed); the
+// new u*, v*, w* land in the state arrays and p in the
 // second pressure array, and the arrays then swap roles, so a pending step
 // always leaves u* in d_us and p (before its update) in d_p.  The fused
 // kernel computes the same values as the separate projection and momentum,
@@ -525,7 +526,7 @@ template <class R> void step_pending(fen_ctx *c, double dt)
         c->last_dt = dt;
         c->step_count++;
         void *out[3] = {c->d_u[0], c->d_u[1], c->d_u[2]};
-        launch_project_momentum<R>(c, c->pend_dt, dt, A, B, c->d_us, c->d_p, out, c->d_p2, c->d_scal);
+        launch_project_momentum<R>(c, c->pend_dt, dt, A, B, c->d_us, c->d_p, out, c->d_p2);
         for (int d = 0; d < 3; d++) std::swap(c->d_u[d], c->d_us[d]);
         std::swap(c->d_p, c->d_p2);
         c->config_version++;   // a captured step graph holds the old array roles

@@ -301,7 +301,7 @@ void mf_ghost_field(fen_ctx *c, double *f);
 bool fused_general_walls(const fen_ctx *c);
 template <class R>
 void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
-                             const void *p_in, void *const *us_out, void *p_out, double *checks_out);
+                             const void *p_in, void *const *us_out, void *p_out);
 template <class R> void launch_checks(fen_ctx *c, const R *const *vel, double *out2);
 long stencil_blocks(const Box &b);   // workgroups of the momentum / projection grids
 // Neumann-direction solver passes (neumann_kernels.hip, FP64).

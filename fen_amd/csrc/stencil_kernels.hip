@@ -550,14 +550,15 @@ bool fused_general_walls(const fen_ctx *c)
     return true;
 }
 
-// Step n's projection + checks fused with step n+1's momentum (a grid
-// periodic in every direction, x, y periodic with z walls, or the general
-// walls of fused_general_walls).
+// Step n's projection fused with step n+1's momentum (a grid periodic in
+// every direction, x, y periodic with z walls, or the general walls of
+// fused_general_walls).  Step n's checks are not formed: step n+1 ends
+// pending, so its own checks are what a reader asks for.
 // us_in/p_in: u*, v*, w* and p of step n;
 // us_out/p_out: u*, v*, w* of step n+1 and p after step n (distinct arrays).
 template <class R>
 void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
-                             const void *p_in, void *const *us_out, void *p_out, double *checks_out)
+                             const void *p_in, void *const *us_out, void *p_out)
 {
     const bool periodic = c->img_vel == 7 && c->img_p == 7 && c->img_phi == 7 && !c->shift_active;
     const bool zwalls = fused_z_walls(c);
@@ -593,7 +594,6 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
     a.phi = as<R>(c->d_phi);
     a.p_in = static_cast<const R *>(p_in);
     a.p_out = as<R>(p_out);
-    a.part = c->d_part;
     a.pid = 1.0 / c->delta;
     a.pdt = dt_p;
     a.prhof = 0.5 * (c->density + c->density);
@@ -611,7 +611,6 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
     }
     nbz = (c->box.nz + a.kb - 1) / a.kb;
     const long nblk = (long)a.m.nbx * a.m.nby * nbz;
-    if ((size_t)(3 * nblk) > c->n_part) throw std::runtime_error("partial buffer too small");
     if (general) {
         // the state at the wall positions, read by the march below
         const long nw = pm_wall_layout(c->box).per_field;
@@ -633,10 +632,6 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
 #undef FEN_PM
     // reads u*, v*, w*, p, phi, dv_o (+ S); writes p, u*, v*, w*, dv_o
     prof_end(c, K_PROJ_MOM, (c->has_source ? 18.0 : 15.0) * sizeof(R) * c->n_int);
-    prof_begin(c, K_REDUCE);
-    hipLaunchKernelGGL(checks_final_kernel, dim3(1), dim3(kFinalThreads), 0, c->stream, (const double *)c->d_part,
-                       nblk, checks_out);
-    prof_end(c, K_REDUCE, 24.0 * nblk);
 }
 
 // ---------------------------------------------------------------------------
@@ -711,9 +706,9 @@ template void launch_momentum<float>(fen_ctx *, double, double, double);
 template void launch_project<double>(fen_ctx *, double, double *, bool);
 template void launch_project<float>(fen_ctx *, double, double *, bool);
 template void launch_project_momentum<double>(fen_ctx *, double, double, double, double, void *const *, const void *,
-                                              void *const *, void *, double *);
+                                              void *const *, void *);
 template void launch_project_momentum<float>(fen_ctx *, double, double, double, double, void *const *, const void *,
-                                             void *const *, void *, double *);
+                                             void *const *, void *);
 template void launch_checks<double>(fen_ctx *, const double *const *, double *);
 template void launch_checks<float>(fen_ctx *, const float *const *, double *);
 

@@ -147,8 +147,8 @@ int fen_ns_step(fen_ctx *ctx, int step, double *dt, double *maxdiv, double *maxc
  * synchronisation; for fixed-dt runs and benchmarks.  On a 3-D grid periodic
  * in every direction, periodic in x and y with no-slip walls at z, or (FP64)
  * a grid of the npn or nnn solver whose faces are periodic pairs or walls
- * (Dirichlet/Neumann), at fixed dt, each step's projection, pressure update
- * and checks are fused with the next step's momentum (the state between the
+ * (Dirichlet/Neumann), at fixed dt, each step's projection and pressure
+ * update are fused with the next step's momentum (the state between the
  * steps of the run is not stored); the result equals nsteps fen_ns_step
  * calls bit for bit.  fen_ns_step on those grids leaves its projection
  * pending in the same way until the state is read or changed.
