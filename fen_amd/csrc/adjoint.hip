@@ -308,19 +308,9 @@ void adjoint_backward(fen_ctx *c, const double *seed[3], double *grad[3], double
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int first = sg * t->every, len = std::min(t->every, t->nsteps - first);
         load_state(c, t->ckpt[sg]);
-        for (int d = 0; d < 3; d++)
-            FEN_HIP_CHECK(hipMemcpyAsync(t->seg[0] + d * c->n_ghost, c->d_u[d], c->n_ghost * sizeof(double),
-                                         hipMemcpyDeviceToDevice, c->stream));
-        // step s reads its velocity from segment slot s and projects the
-        // next one straight into slot s + 1 (the last state is not needed)
-        for (int s = 0; s + 1 < len; s++) {
-            double *in[3], *out[3];
-            for (int d = 0; d < 3; d++) {
-                in[d] = t->seg[s] + d * c->n_ghost;
-                out[d] = t->seg[s + 1] + d * c->n_ghost;
-            }
-            advance_fp64_into(c, t->dt, in, out);
-        }
+        // the segment's states, each formed straight into its slot (the
+        // state after the segment's last step is not needed)
+        states_fp64(c, t->dt, len, t->seg.data());
         for (int s = len - 1; s >= 0; s--) adjoint_step(c, t, t->seg[s], ub, pb, dvo);
     }
     for (int d = 0; d < 3; d++)

@@ -515,7 +515,7 @@ bool deferrable(const fen_ctx *c)
 // kernel computes the same values as the separate projection and momentum,
 // so a sequence of pending steps equals the same number of advance() calls
 // bit for bit.
-template <class R> void step_pending(fen_ctx *c, double dt)
+template <class R> void step_pending(fen_ctx *c, double dt, void *const *save = nullptr)
 {
     using namespace fen;
     if (!c->pending) {
@@ -526,7 +526,7 @@ template <class R> void step_pending(fen_ctx *c, double dt)
         c->last_dt = dt;
         c->step_count++;
         void *out[3] = {c->d_u[0], c->d_u[1], c->d_u[2]};
-        launch_project_momentum<R>(c, c->pend_dt, dt, A, B, c->d_us, c->d_p, out, c->d_p2);
+        launch_project_momentum<R>(c, c->pend_dt, dt, A, B, c->d_us, c->d_p, out, c->d_p2, save);
         for (int d = 0; d < 3; d++) std::swap(c->d_u[d], c->d_us[d]);
         std::swap(c->d_p, c->d_p2);
         c->config_version++;   // a captured step graph holds the old array roles
@@ -576,6 +576,39 @@ void fen::advance_fp64_into(fen_ctx *c, double dt, double *const *in, double *co
     solve_phi<double>(c, dt, c->d_us);
     for (int d = 0; d < 3; d++) c->d_u[d] = out[d];   // the projection writes the new state here
     finish_step<double>(c, dt);
+    for (int d = 0; d < 3; d++) c->d_u[d] = own[d];
+}
+
+void fen::states_fp64(fen_ctx *c, double dt, int n, double *const *seg)
+{
+    if (c->pending) throw std::runtime_error("states_fp64 needs a completed state");
+    const size_t ng = c->n_ghost;
+    for (int d = 0; d < 3; d++)
+        FEN_HIP_CHECK(hipMemcpyAsync(seg[0] + d * ng, c->d_u[d], ng * sizeof(double), hipMemcpyDeviceToDevice,
+                                     c->stream));
+    if (n < 2) return;
+    if (!deferrable(c) || c->has_source) {
+        for (int s = 0; s + 1 < n; s++) {
+            double *in[3], *out[3];
+            for (int d = 0; d < 3; d++) {
+                in[d] = seg[s] + d * ng;
+                out[d] = seg[s + 1] + d * ng;
+            }
+            advance_fp64_into(c, dt, in, out);
+        }
+        return;
+    }
+    // fused: each step's projection, run with the next step's momentum,
+    // saves the state it forms into its slot; the last state is the
+    // completed projection of the last step, written into the last slot
+    step_pending<double>(c, dt);
+    for (int s = 1; s + 1 < n; s++) {
+        void *save[3] = {seg[s], seg[s] + ng, seg[s] + 2 * ng};
+        step_pending<double>(c, dt, save);
+    }
+    void *own[3] = {c->d_u[0], c->d_u[1], c->d_u[2]};
+    for (int d = 0; d < 3; d++) c->d_u[d] = seg[n - 1] + d * ng;
+    flush_pending(c);
     for (int d = 0; d < 3; d++) c->d_u[d] = own[d];
 }
 

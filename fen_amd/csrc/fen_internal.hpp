@@ -301,7 +301,7 @@ void mf_ghost_field(fen_ctx *c, double *f);
 bool fused_general_walls(const fen_ctx *c);
 template <class R>
 void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
-                             const void *p_in, void *const *us_out, void *p_out);
+                             const void *p_in, void *const *us_out, void *p_out, void *const *save = nullptr);
 template <class R> void launch_checks(fen_ctx *c, const R *const *vel, double *out2);
 long stencil_blocks(const Box &b);   // workgroups of the momentum / projection grids
 // Neumann-direction solver passes (neumann_kernels.hip, FP64).
@@ -324,6 +324,11 @@ void advance_fp64(fen_ctx *c, double dt);
 // writing the new velocity to `out` instead of the context's arrays, which
 // keep their contents (p and dv_o are the context's, updated in place).
 void advance_fp64_into(fen_ctx *c, double dt, double *const *in, double *const *out);
+// The velocities of n consecutive FP64 states of a fixed-dt run that starts
+// from the context's (completed) state: seg[s] receives u, v, w (ghosted,
+// one after the other) of the state before step s, after n - 1 steps, fused
+// where fen_ns_run would fuse them; p and dv_o advance in the context.
+void states_fp64(fen_ctx *c, double dt, int n, double *const *seg);
 // nsteps FP64 steps as fen_ns_run takes them (fused when the case allows;
 // bitwise equal to nsteps calls of advance_fp64).
 void run_fp64(fen_ctx *c, int nsteps, double dt);

@@ -555,10 +555,12 @@ bool fused_general_walls(const fen_ctx *c)
 // fused_general_walls).  Step n's checks are not formed: step n+1 ends
 // pending, so its own checks are what a reader asks for.
 // us_in/p_in: u*, v*, w* and p of step n;
-// us_out/p_out: u*, v*, w* of step n+1 and p after step n (distinct arrays).
+// us_out/p_out: u*, v*, w* of step n+1 and p after step n (distinct arrays);
+// save (FP64, periodic grid, no source), if given: u, v, w of step n with
+// their periodic images.
 template <class R>
 void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, double B, void *const *us_in,
-                             const void *p_in, void *const *us_out, void *p_out)
+                             const void *p_in, void *const *us_out, void *p_out, void *const *save)
 {
     const bool periodic = c->img_vel == 7 && c->img_p == 7 && c->img_phi == 7 && !c->shift_active;
     const bool zwalls = fused_z_walls(c);
@@ -620,7 +622,16 @@ void launch_project_momentum(fen_ctx *c, double dt_p, double dt_m, double A, dou
     const dim3 grid((unsigned)nblk), block(SH::FTX * SH::FTY);
 #define FEN_PM(SRC, ZW) \
     hipLaunchKernelGGL((project_momentum_kernel<R, SRC, ZW, SH::FTY, SH::FOCC, SH::FTX>), grid, block, 0, c->stream, a)
-    if (c->has_source) {
+    if (save) {
+        if constexpr (std::is_same<R, double>::value) {
+            if (!periodic || c->has_source) throw std::runtime_error("state saving needs a periodic grid and no source");
+            for (int d = 0; d < 3; d++) a.save[d] = as<R>(save[d]);
+            hipLaunchKernelGGL((project_momentum_kernel<R, 0, 0, SH::FTY, SH::FOCC, SH::FTX, 1>), grid, block, 0,
+                               c->stream, a);
+        } else {
+            throw std::runtime_error("state saving is FP64");
+        }
+    } else if (c->has_source) {
         if (zwalls) FEN_PM(1, 1);
         else if (general) FEN_PM(1, 2);
         else FEN_PM(1, 0);
@@ -706,9 +717,9 @@ template void launch_momentum<float>(fen_ctx *, double, double, double);
 template void launch_project<double>(fen_ctx *, double, double *, bool);
 template void launch_project<float>(fen_ctx *, double, double *, bool);
 template void launch_project_momentum<double>(fen_ctx *, double, double, double, double, void *const *, const void *,
-                                              void *const *, void *);
+                                              void *const *, void *, void *const *);
 template void launch_project_momentum<float>(fen_ctx *, double, double, double, double, void *const *, const void *,
-                                             void *const *, void *);
+                                             void *const *, void *, void *const *);
 template void launch_checks<double>(fen_ctx *, const double *const *, double *);
 template void launch_checks<float>(fen_ctx *, const float *const *, double *);
 

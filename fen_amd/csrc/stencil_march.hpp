@@ -354,6 +354,7 @@ template <class R> struct PMArgs {
     int gtype[4];
     const R *const *gplane;
     R *wbuf;   // ZW == 2: the state at the wall positions (pm_wall_layout)
+    R *save[3];   // SAVE: u, v, w of step n (ghosted, with periodic images)
 };
 
 // The general-wall form (ZW == 2).  Along one wall axis a ghosted position
@@ -446,7 +447,10 @@ __device__ __forceinline__ int pm_src(int g, int n, bool wall)
     return g;
 }
 
-template <class R, int SRC, int TY, int ZW, int TX = 64>
+// SAVE (periodic grids): also store u, v, w of step n and their periodic
+// images into a.save, for callers that keep every state (the adjoint's
+// recomputed segments).
+template <class R, int SRC, int TY, int ZW, int TX = 64, int SAVE = 0>
 struct ProjectMomentumMarch {
     using S = TileShape<TY, TX>;
     static constexpr int NT = S::NT, NRAW = 8;
@@ -648,6 +652,14 @@ struct ProjectMomentumMarch {
             // p of step n; its checks are not formed here: the step ends
             // pending, and a read of the scalars runs the checks-only pass
             store_stream(&a.p_out[m.b.g(i, j, k)], lds[(3 * 3 + (int)sc) * S::N + c]);
+            if constexpr (SAVE) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    const R st = lds[(f * 3 + (int)sc) * S::N + c];
+                    a.save[f][m.b.g(i, j, k)] = st;
+                    march_images(a.save[f], m.b, 7, i, j, k, st, xedge);
+                }
+            }
             // momentum of step n+1
             auto at = [&](int f) {
                 return [=](int di, int dj, int dk) {
@@ -700,11 +712,11 @@ struct ProjectMomentumMarch {
     }
 };
 
-template <class R, int SRC, int ZW, int TY, int OCC = 1, int TX = 64>
+template <class R, int SRC, int ZW, int TY, int OCC = 1, int TX = 64, int SAVE = 0>
 __global__ __launch_bounds__(TX * TY, OCC) void project_momentum_kernel(PMArgs<R> a)
 {
     __shared__ R lds[ring_size<TileShape<TY, TX>, 4>()];
-    ProjectMomentumMarch<R, SRC, TY, ZW, TX>::run(a, lds);
+    ProjectMomentumMarch<R, SRC, TY, ZW, TX, SAVE>::run(a, lds);
 }
 
 // The general-wall form's face buffer: u, v, w, p of step n at every wall
