@@ -75,6 +75,25 @@ def test_gpu_adjoint_checkpoint_interval_is_exact():
     s.close()
 
 
+def test_gpu_adjoint_fused_recompute_is_exact(monkeypatch):
+    """The backward sweep recomputes each segment as a fused run whose
+    kernel saves every state it forms; with FEN_FUSED_RUN=0 it steps one
+    step at a time into the segment buffer instead.  Both give the same
+    gradient bit for bit."""
+    state = ref._initial(seed=5)
+    grads = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FEN_FUSED_RUN", fused)
+        s, dt = _solver(ref.N, ref.L, 1.0 / 400)
+        load_state(s, state, dt)
+        s.adjoint_record(0, 9, 4, dt)
+        seed = [np.asfortranarray(a) for a in final_velocity(s)]
+        grads.append(s.adjoint_backward(seed))
+        s.close()
+    for a, b in zip(grads[0], grads[1]):
+        assert np.array_equal(a, b)
+
+
 def test_gpu_adjoint_tgv128_re400_matches_central_differences():
     """Discrete adjoint of the 3-D periodic 128^3 case, Re = 400, 200 steps,
     checkpoints every 20: d(KE_final)/d(u0) along 16 random directions
