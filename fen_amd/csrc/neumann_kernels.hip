@@ -47,6 +47,29 @@ struct DctArgs {
     int gl, d3;
 };
 
+// The DCT-II's last step: y_k = 2 Re(exp(-i pi k/(2N)) V_k) of the block's
+// 2L rows from their packed spectra in LDS (row 2l in Re, 2l + 1 in Im of
+// line l), times post.  Adjacent threads write adjacent addresses: rows
+// (x pass, orr = 1) or k (y pass, ok = 1).
+template <int N>
+__device__ __forceinline__ void dct2_out(const DctArgs &a, const double2 *buf, const double2 *mks, long row0)
+{
+    constexpr int L = nb_lines<N>(), TJ = 2 * L, LL = lds_line<N>();
+    const bool k_fast = a.ok == 1;
+    for (int e = threadIdx.x; e < TJ * N; e += kNB) {
+        const int row = k_fast ? e / N : e % TJ, k = k_fast ? e % N : e / TJ;
+        const long r = row0 + row;
+        if (r >= a.nrows) continue;
+        const double2 *line = buf + (row >> 1) * LL;
+        const double2 zk = line[lds_pad(k)], zn = line[lds_pad((N - k) & (N - 1))];
+        const double c = mks[k].x, s = mks[k].y;
+        double y;
+        if ((row & 1) == 0) y = c * (zk.x + zn.x) + s * (zk.y - zn.y);
+        else y = c * (zk.y + zn.y) + s * (zn.x - zk.x);
+        a.out[k * a.ok + r * a.orr] = y * a.post;
+    }
+}
+
 template <int N, int SRC>
 __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
 {
@@ -54,6 +77,7 @@ __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
     __shared__ double2 buf[L * LL];
     __shared__ double2 tw[N];
     load_twiddles<N>(tw, a.tw);
+    const double2 *mks = a.mk;
     const long row0 = (long)blockIdx.x * TJ;
     // staging: the loads of BATCH elements are issued before their LDS
     // writes; adjacent threads read adjacent x, so every load is coalesced
@@ -99,21 +123,101 @@ __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
     }
     __syncthreads();
     fft_lds<N, -1>(buf + (threadIdx.x / T) * LL, threadIdx.x % T, tw);
-    // adjacent threads write adjacent addresses: rows (x pass, orr = 1) or
-    // k (y pass, ok = 1)
-    const bool k_fast = a.ok == 1;
-    for (int e = threadIdx.x; e < TJ * N; e += kNB) {
-        const int row = k_fast ? e / N : e % TJ, k = k_fast ? e % N : e / TJ;
-        const long r = row0 + row;
-        if (r >= a.nrows) continue;
-        const double2 *line = buf + (row >> 1) * LL;
-        const double2 zk = line[lds_pad(k)], zn = line[lds_pad((N - k) & (N - 1))];
-        const double c = a.mk[k].x, s = a.mk[k].y;
-        double y;
-        if ((row & 1) == 0) y = c * (zk.x + zn.x) + s * (zk.y - zn.y);
-        else y = c * (zk.y + zn.y) + s * (zn.x - zk.x);
-        a.out[k * a.ok + r * a.orr] = y * a.post;
+    dct2_out<N>(a, buf, mks, row0);
+}
+
+// The same DCT-II with the first Stockham stage's inputs loaded straight
+// into registers (N <= 512, so a line's T = N/8 threads lie in one wave):
+// SRC 0 (div(u*) rho/dt, x pass) or 2 (contiguous lines, y pass).  Thread j
+// of a line needs Makhoul's v_p at p = j + rT, r = 0..7: the even x_{2j+2rT}
+// (r < 4) and the odd x_{2rT+2T-1-2j} (v[7-r]).  Each thread loads aligned
+// pairs of adjacent x with 16-byte loads, keeps the even element and passes
+// the odd one through a cross-lane read (ds_bpermute) to the thread whose
+// v[7-r] it is.  The values and the transform's arithmetic are
+// dct2_rows_kernel's, so the results are bitwise equal.
+//   SRC 2: pair (2rT+2j, +1); the odd x_{2rT+2j+1} goes to thread T-1-j.
+//   SRC 0: ghosted pair (q, q+1), q = 2rT + 2j, holds elements q-1 (odd)
+//   and q (even) (ghosted x index = element + 1); the odd one goes to thread
+//   T-j, and thread 0 takes its own from chunk r+1 (lane 0 loads a fifth
+//   pair, q = N, for x_{N-1}).  Needs 16-byte aligned fields and even gx.
+template <int N, int SRC, int MKL>
+__global__ __launch_bounds__(kNB) void dct2_pairs_kernel(DctArgs a)
+{
+    constexpr int T = nb_tpl<N>(), L = nb_lines<N>(), TJ = 2 * L, LL = lds_line<N>();
+    static_assert(T <= 64 && (SRC == 0 || SRC == 2), "one wave per line; x pass or y pass");
+    __shared__ double2 buf[L * LL];
+    __shared__ double2 tw[N];
+    load_twiddles<N>(tw, a.tw);   // first read after the first stage's barrier
+    __shared__ double2 mkl[MKL ? N : 1];
+    const double2 *mks = a.mk;
+    if constexpr (MKL) {
+        load_twiddles<N>(mkl, a.mk);
+        mks = mkl;
     }
+    const long row0 = (long)blockIdx.x * TJ;
+    const int l = threadIdx.x / T, j = threadIdx.x % T;
+    auto ld2 = [](const double *p) { return *reinterpret_cast<const double2 *>(p); };
+    double ev[2][4], od[2][5];   // [row][chunk]: the even element kept, the odd one passed on
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const long r0 = row0 + 2 * l + h;
+        const bool ok = r0 < a.nrows;
+        const long r = ok ? r0 : row0;   // a valid row: the loads are kept, the values dropped
+        if constexpr (SRC == 2) {
+            const double *in = a.u + r * N;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const double2 p = ld2(in + 2 * c * T + 2 * j);
+                ev[h][c] = ok ? p.x : 0.0;
+                od[h][c] = ok ? p.y : 0.0;
+            }
+        } else {
+            const int ri = (int)r, jy = ri % a.b.ny + 1, kz = ri / a.b.ny + 1;
+            const long g0 = a.b.g(0, jy, kz);
+            // elements q - 1 and q from the ghosted pair (q, q + 1)
+            auto pair = [&](long gq, double &d_odd, double &d_even) {
+                const double um = a.u[gq - 1];
+                const double2 uu = ld2(a.u + gq), vv = ld2(a.v + gq), vs = ld2(a.v + gq - a.b.gx);
+                double d0 = (uu.x - um) * a.id + (vv.x - vs.x) * a.id;
+                double d1 = (uu.y - uu.x) * a.id + (vv.y - vs.y) * a.id;
+                if (a.d3) {
+                    const double2 ww = ld2(a.w + gq), ws = ld2(a.w + gq - a.b.gxy);
+                    d0 = d0 + (ww.x - ws.x) * a.id;
+                    d1 = d1 + (ww.y - ws.y) * a.id;
+                }
+                d_odd = ok ? d0 * a.rho / a.dt : 0.0;
+                d_even = ok ? d1 * a.rho / a.dt : 0.0;
+            };
+#pragma unroll
+            for (int c = 0; c < 4; c++) pair(g0 + 2 * c * T + 2 * j, od[h][c], ev[h][c]);
+            od[h][4] = 0.0;
+            if (j == 0) {
+                double unused;
+                pair(g0 + N, od[h][4], unused);
+            }
+        }
+    }
+    double2 v[8];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        v[c] = double2{ev[0][c], ev[1][c]};
+        double o0, o1;
+        if constexpr (SRC == 2) {
+            o0 = __shfl(od[0][c], T - 1 - j, T);
+            o1 = __shfl(od[1][c], T - 1 - j, T);
+        } else {
+            const double s0 = __shfl(od[0][c], T - j, T), s1 = __shfl(od[1][c], T - j, T);
+            o0 = j == 0 ? od[0][c + 1] : s0;
+            o1 = j == 0 ? od[1][c + 1] : s1;
+        }
+        v[7 - c] = double2{o0, o1};
+    }
+    double2 *line = buf + l * LL;
+    fft_regs<N, -1>(v, line, j, tw);
+#pragma unroll
+    for (int q = 0; q < 8; q++) line[lds_pad(j + q * T)] = v[q];
+    __syncthreads();
+    dct2_out<N>(a, buf, mks, row0);
 }
 
 // ---------------------------------------------------------------------------
@@ -376,7 +480,7 @@ template <class E> __device__ __forceinline__ E zero_();
 template <> __device__ __forceinline__ double zero_<double>() { return 0.0; }
 template <> __device__ __forceinline__ double2 zero_<double2>() { return double2{0.0, 0.0}; }
 
-constexpr int kGU = 8;   // rows loaded ahead per batch in the strided sweeps
+constexpr int kGU = 8;   // rows loaded ahead per batch in the strided sweeps (16 and 32 measured slower)
 
 // SPLIT (E = double on complex columns): the real and imaginary parts of a
 // column are swept by two adjacent threads.  The coefficients are real, so
@@ -565,6 +669,14 @@ template <class E> __global__ __launch_bounds__(64) void thomas_column_kernel(Th
     default: throw std::runtime_error("unsupported transform length");   \
     }
 
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+template <int N, int SRC, int MKL> static void launch_dct2_pairs(unsigned grid, hipStream_t s, const DctArgs &a)
+{
+    if constexpr (N <= 512) hipLaunchKernelGGL((dct2_pairs_kernel<N, SRC, MKL>), dim3(grid), dim3(kNB), 0, s, a);
+    else throw std::logic_error("dct2_pairs_kernel takes N <= 512");
+}
+
 static unsigned row_blocks(long nrows, int N) { return (unsigned)((nrows + 2 * (kNB / (N / 8)) - 1) / (2 * (kNB / (N / 8)))); }
 
 // x DCT-II of the rows of div(u*)*rho/dt (vel != nullptr) or of phi (direct)
@@ -589,8 +701,12 @@ void launch_dct2_x(fen_ctx *c, const double *const *vel, const double *direct, l
     a.d3 = c->dim == 3;
     const unsigned grid = row_blocks(a.nrows, c->nx);
     prof_begin(c, K_DIV_R2C);
-#define L_D2(NN)                                                                          \
-    if (vel) hipLaunchKernelGGL((dct2_rows_kernel<NN, 0>), dim3(grid), dim3(kNB), 0, c->stream, a); \
+    // the register-loading form: N <= 512, 16-byte aligned pairs (even gx, aligned fields)
+    const bool pairs = vel && c->nx <= 512 && c->box.gx % 2 == 0 && aligned16(a.u) && aligned16(a.v) &&
+                       (!a.d3 || aligned16(a.w));
+#define L_D2(NN)                                                                              \
+    if (pairs) launch_dct2_pairs<NN, 0, 0>(grid, c->stream, a);                               \
+    else if (vel) hipLaunchKernelGGL((dct2_rows_kernel<NN, 0>), dim3(grid), dim3(kNB), 0, c->stream, a); \
     else hipLaunchKernelGGL((dct2_rows_kernel<NN, 1>), dim3(grid), dim3(kNB), 0, c->stream, a);
     FEN_NB_DISPATCH(c->nx, L_D2)
 #undef L_D2
@@ -655,7 +771,10 @@ void launch_dct_y(fen_ctx *c, int inverse, double post)
         a.nrows = nrows;
         a.ok = 1;
         a.orr = c->ny;
-#define L_DY(NN) hipLaunchKernelGGL((dct2_rows_kernel<NN, 2>), dim3(grid), dim3(kNB), 0, c->stream, a);
+        // the register-loading form for N <= 512, with the DCT's (cos, sin) table in LDS
+#define L_DY(NN)                                                                              \
+    if (c->ny <= 512) launch_dct2_pairs<NN, 2, 1>(grid, c->stream, a);                        \
+    else hipLaunchKernelGGL((dct2_rows_kernel<NN, 2>), dim3(grid), dim3(kNB), 0, c->stream, a);
         FEN_NB_DISPATCH(c->ny, L_DY)
 #undef L_DY
     } else {
