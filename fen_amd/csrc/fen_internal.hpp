@@ -205,10 +205,10 @@ struct fen_ctx {
     // Poisson solver state
     int hx = 0;                  // nx/2 + 1
     int num_cu = 256;            // compute units of the device (queried at create)
-    void *d_spec = nullptr;      // complex R, [kx][k][j]
+    void *d_spec = nullptr;      // complex R, [kx][k][j] (npn, nnn: real, [k][kx][j])
     double *d_c1 = nullptr;      // Thomas c1, computed at create (layout of the sweep's columns)
     double2 *d_dd = nullptr;     // FP64 Thomas right-hand side (ppn, FP32 form), [kx][k][j]
-    double2 *d_spec2 = nullptr;  // npn: y half spectrum of the real x-DCT lines, [kx][k][jy]
+    double2 *d_spec2 = nullptr;  // npn: y half spectrum of the real x-DCT lines, [k][kx][jy]
     double2 *d_mkx = nullptr, *d_mky = nullptr;   // (cos, sin)(pi k/(2n)), k < n: DCTs along x, y
     double *d_mwx = nullptr, *d_mwy = nullptr, *d_mwz = nullptr;
     double *d_ta = nullptr, *d_tb = nullptr, *d_tc = nullptr;

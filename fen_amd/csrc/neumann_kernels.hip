@@ -11,7 +11,9 @@
 //   inverse: U_k = (x_k - i x_{n-k}) exp(i pi k/(2n)) (x_n = 0), u = IFFT(U)
 //            (unnormalised, real), y_{2m} = u_m, y_{2m+1} = u_{n-1-m}.
 // The transforms run in LDS with the Stockham kernels of fft_device.hpp.
-// Real spectra are laid out [kx][k][j] (j fastest), as the complex one.
+// Spectra are laid out [k][kx][j] (j fastest; npn's y half spectrum
+// [k][kx][jy]): one k row of every z column is contiguous, so the Thomas
+// sweeps along z read and write whole contiguous rows at each step.
 #include <stdexcept>
 
 #include "fen_internal.hpp"
@@ -43,6 +45,7 @@ struct DctArgs {
     Box b;
     double id, rho, dt, post;
     long nrows, ok, orr;          // rows, output strides
+    long okz;                     // x pass: extra output stride per z (row r has k = r / ny)
     long sx, sy;                  // SRC 1: strides of the phi array
     int gl, d3;
 };
@@ -56,6 +59,26 @@ __device__ __forceinline__ void dct2_out(const DctArgs &a, const double2 *buf, c
 {
     constexpr int L = nb_lines<N>(), TJ = 2 * L, LL = lds_line<N>();
     const bool k_fast = a.ok == 1;
+    if constexpr (kNB % TJ == 0) {
+        if (!k_fast) {
+            // x pass: a thread keeps one row and walks k
+            const int row = threadIdx.x % TJ;
+            const long r = row0 + row;
+            if (r >= a.nrows) return;
+            const long zo = a.okz ? (long)((int)r / a.b.ny) * a.okz : 0;   // rows < 2^31: 32-bit division
+            double *o = a.out + r * a.orr + zo;
+            const double2 *line = buf + (row >> 1) * LL;
+            for (int k = threadIdx.x / TJ; k < N; k += kNB / TJ) {
+                const double2 zk = line[lds_pad(k)], zn = line[lds_pad((N - k) & (N - 1))];
+                const double c = mks[k].x, s = mks[k].y;
+                double y;
+                if ((row & 1) == 0) y = c * (zk.x + zn.x) + s * (zk.y - zn.y);
+                else y = c * (zk.y + zn.y) + s * (zn.x - zk.x);
+                o[k * a.ok] = y * a.post;
+            }
+            return;
+        }
+    }
     for (int e = threadIdx.x; e < TJ * N; e += kNB) {
         const int row = k_fast ? e / N : e % TJ, k = k_fast ? e % N : e / TJ;
         const long r = row0 + row;
@@ -66,7 +89,8 @@ __device__ __forceinline__ void dct2_out(const DctArgs &a, const double2 *buf, c
         double y;
         if ((row & 1) == 0) y = c * (zk.x + zn.x) + s * (zk.y - zn.y);
         else y = c * (zk.y + zn.y) + s * (zn.x - zk.x);
-        a.out[k * a.ok + r * a.orr] = y * a.post;
+        const long zo = a.okz ? (long)((int)r / a.b.ny) * a.okz : 0;   // rows < 2^31: 32-bit division
+        a.out[k * a.ok + r * a.orr + zo] = y * a.post;
     }
 }
 
@@ -230,6 +254,7 @@ struct Dct3Args {
     double *out;
     const double2 *tw, *mk;
     long nrows, ik, ir;
+    long ikz;   // x pass: extra input stride per z (row r has k = r / ny)
     long sx, sy;
     int gl, ny, nz;
     // x pass with gl == 1: also write the ghosts, when every ghost of phi
@@ -295,13 +320,15 @@ __global__ __launch_bounds__(kNB) void dct3_rows_kernel(Dct3Args a)
             const int ls = e % L, k = e / L;
             const long r = row0 + 2 * ls;
             a1[m] = 0.0; b1[m] = 0.0; a2[m] = 0.0; b2[m] = 0.0;
+            // rows r and r + 1 share k (3-D: r and ny even; 2-D: k = 0), so one offset serves both
+            const long zo = a.ikz ? (long)((int)r / a.ny) * a.ikz : 0;
             if (r < a.nrows) {
-                a1[m] = a.in[k * a.ik + r * a.ir];
-                if (k > 0) b1[m] = a.in[(N - k) * a.ik + r * a.ir];
+                a1[m] = a.in[k * a.ik + r * a.ir + zo];
+                if (k > 0) b1[m] = a.in[(N - k) * a.ik + r * a.ir + zo];
             }
             if (r + 1 < a.nrows) {
-                a2[m] = a.in[k * a.ik + (r + 1) * a.ir];
-                if (k > 0) b2[m] = a.in[(N - k) * a.ik + (r + 1) * a.ir];
+                a2[m] = a.in[k * a.ik + (r + 1) * a.ir + zo];
+                if (k > 0) b2[m] = a.in[(N - k) * a.ik + (r + 1) * a.ir + zo];
             }
         }
 #pragma unroll
@@ -696,8 +723,10 @@ void launch_dct2_x(fen_ctx *c, const double *const *vel, const double *direct, l
     a.dt = dt;
     a.post = post;
     a.nrows = (long)c->ny * c->nz;
-    a.ok = a.nrows;
+    // spectrum [k][kx][j]: element kx of row r = j + ny k at kx ny + j + k nx ny
+    a.ok = c->ny;
     a.orr = 1;
+    a.okz = (long)(c->nx - 1) * c->ny;
     a.d3 = c->dim == 3;
     const unsigned grid = row_blocks(a.nrows, c->nx);
     prof_begin(c, K_DIV_R2C);
@@ -734,8 +763,9 @@ void launch_dct3_x(fen_ctx *c, double *phi, int gl, long sx, long sy, double pos
     a.tw = as<double2>(c->d_twx);
     a.mk = c->d_mkx;
     a.nrows = (long)c->ny * c->nz;
-    a.ik = a.nrows;
+    a.ik = c->ny;   // spectrum [k][kx][j], as the x DCT-II writes it
     a.ir = 1;
+    a.ikz = (long)(c->nx - 1) * c->ny;
     a.sx = sx;
     a.sy = sy;
     a.gl = gl;
@@ -836,13 +866,13 @@ static ThomasGArgs thomas_layout(fen_ctx *c, bool *complex_out, bool *assoc2d_ou
         a.x = c->d_spec; assoc2d = true;
         a.ncol = c->nx; a.ni = 1; a.os = c->ny; a.s = 1; a.n = c->ny;
         break;
-    case FEN_SOLVER_NPN:  // complex [kx][k][jy], columns along k
+    case FEN_SOLVER_NPN:  // complex [k][kx][jy], columns along k: column kx hy + jy, one k row contiguous
         a.x = c->d_spec2; complex_e = true;
-        a.ncol = (long)c->nx * hy; a.ni = hy; a.os = (long)c->nz * hy; a.s = hy; a.n = c->nz;
+        a.ncol = (long)c->nx * hy; a.ni = hy; a.os = hy; a.s = (long)c->nx * hy; a.n = c->nz;
         break;
-    case FEN_SOLVER_NNN:  // real [kx][k][j]
+    case FEN_SOLVER_NNN:  // real [k][kx][j]
         a.x = c->d_spec;
-        a.ncol = (long)c->nx * c->ny; a.ni = c->ny; a.os = (long)c->nz * c->ny; a.s = c->ny; a.n = c->nz;
+        a.ncol = (long)c->nx * c->ny; a.ni = c->ny; a.os = c->ny; a.s = (long)c->nx * c->ny; a.n = c->nz;
         break;
     case FEN_SOLVER_PPN:  // complex [kx][k][j] (thomas_kernel)
         a.x = c->d_spec; complex_e = true;
