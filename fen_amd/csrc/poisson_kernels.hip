@@ -37,11 +37,11 @@ template <class R> struct DivR2CArgs {
     int ny_shift;   // log2(ny)
 };
 
-template <class R, int D3, int N, int DIRECT>
-__global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs<R> a)
+template <class R, int D3, int N, int DIRECT, int NT = kBlock>
+__global__ __launch_bounds__(NT) void div_r2c_kernel(DivR2CArgs<R> a)
 {
     using C = typename Cplx<R>::type;
-    constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), TJ = 2 * L, LL = lds_line<N>();
+    constexpr int T = threads_per_line<N>(), L = NT / T, TJ = 2 * L, LL = lds_line<N>();
     __shared__ C buf[L * LL];
     __shared__ C tw[N];
     load_twiddles<N>(tw, a.tw);   // first read after the first stage's barrier
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void div_r2c_kernel(DivR2CArgs<R> a)
 #pragma unroll
     for (int q = 0; q < 8; q++) line[lds_pad(t + q * (N / 8))] = v[q];
     __syncthreads();
-    for (int e = threadIdx.x; e < TJ * (N / 2 + 1); e += kBlock) {
+    for (int e = threadIdx.x; e < TJ * (N / 2 + 1); e += NT) {
         const int row = e % TJ, kx = e / TJ;
         const long r = row0 + row;
         if (r >= a.nrows) continue;
@@ -112,22 +112,22 @@ template <class R> struct C2RArgs {
     int img;   // periodic image axes to write (phi in the time step)
 };
 
-template <class R, int N>
-__global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs<R> a)
+template <class R, int N, int NT = kBlock>
+__global__ __launch_bounds__(NT) void c2r_kernel(C2RArgs<R> a)
 {
     using C = typename Cplx<R>::type;
-    constexpr int T = threads_per_line<N>(), L = lines_per_block<N>(), TJ = 2 * L, LL = lds_line<N>();
-    static_assert(L * LL * sizeof(C) >= kBlock * sizeof(double), "the reduction reuses buf");
+    constexpr int T = threads_per_line<N>(), L = NT / T, TJ = 2 * L, LL = lds_line<N>();
+    static_assert(L * LL * sizeof(C) >= NT * sizeof(double), "the reduction reuses buf");
     __shared__ C buf[L * LL];
     __shared__ C tw[N];
     load_twiddles<N>(tw, a.tw);
     const long row0 = (long)blockIdx.x * TJ;
     // Staging: all of a thread's spectrum loads are issued before its LDS writes.
-    constexpr int NE = L * (N / 2 + 1), PER = (NE + kBlock - 1) / kBlock;
+    constexpr int NE = L * (N / 2 + 1), PER = (NE + NT - 1) / NT;
     C X1[PER], X2[PER];
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kBlock;
+        const int e = threadIdx.x + m * NT;
         const int ee = e < NE ? e : 0;
         const int l = ee % L, kx = ee / L;
         const long r = row0 + 2 * l;
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs<R> a)
     }
 #pragma unroll
     for (int m = 0; m < PER; m++) {
-        const int e = threadIdx.x + m * kBlock;
+        const int e = threadIdx.x + m * NT;
         if (e >= NE) continue;
         const int l = e % L, kx = e / L;
         C *line = buf + l * LL;
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void c2r_kernel(C2RArgs<R> a)
         double *red = reinterpret_cast<double *>(buf);
         red[threadIdx.x] = acc;
         __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1) {
+        for (int s = NT / 2; s > 0; s >>= 1) {
             if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
             __syncthreads();
         }
@@ -512,6 +512,18 @@ template <class R> static double spec_bytes(const fen_ctx *c)
 
 template <class R> using CplxT = typename Cplx<R>::type;
 
+// The x passes (div_r2c, c2r) at N = 512 run 512-thread workgroups: 16 rows
+// per workgroup, so each access to a kx plane of the spectrum covers 256
+// contiguous bytes instead of 128.
+static int x_pass_threads(const fen_ctx *c) { return c->nx == 512 ? 512 : kBlock; }
+
+template <class R, int D3, int N, int DIRECT>
+static void launch_div_r2c_n(const fen_ctx *c, dim3 grid, const DivR2CArgs<R> &a)
+{
+    if constexpr (N == 512) hipLaunchKernelGGL((div_r2c_kernel<R, D3, N, DIRECT, 512>), grid, dim3(512), 0, c->stream, a);
+    else hipLaunchKernelGGL((div_r2c_kernel<R, D3, N, DIRECT>), grid, dim3(kBlock), 0, c->stream, a);
+}
+
 // vel == nullptr: direct mode, the RHS is already in `direct_src` (phi array).
 template <class R>
 void launch_div_r2c_impl(fen_ctx *c, const R *const *vel, const R *direct_src, long sx, long sy, int gl, double rho,
@@ -530,14 +542,14 @@ void launch_div_r2c_impl(fen_ctx *c, const R *const *vel, const R *direct_src, l
     a.nrows = (long)c->ny * c->nz;
     a.kxs = (long)c->ny * c->nz;
     a.ny_shift = ilog2(c->ny);
-    const int tj = 2 * (kBlock / (c->nx / 8));
+    const int tj = 2 * (x_pass_threads(c) / (c->nx / 8));
     const dim3 grid((unsigned)((a.nrows + tj - 1) / tj));
     const double bytes = (vel ? (c->dim == 3 ? 3.0 : 2.0) : 1.0) * sizeof(R) * c->n_int + spec_bytes<R>(c);
     prof_begin(c, K_DIV_R2C);
 #define L_DIV(NN)                                                                                        \
-    if (a.direct) hipLaunchKernelGGL((div_r2c_kernel<R, 0, NN, 1>), grid, dim3(kBlock), 0, c->stream, a);    \
-    else if (c->dim == 3) hipLaunchKernelGGL((div_r2c_kernel<R, 1, NN, 0>), grid, dim3(kBlock), 0, c->stream, a); \
-    else hipLaunchKernelGGL((div_r2c_kernel<R, 0, NN, 0>), grid, dim3(kBlock), 0, c->stream, a);
+    if (a.direct) launch_div_r2c_n<R, 0, NN, 1>(c, grid, a);                                             \
+    else if (c->dim == 3) launch_div_r2c_n<R, 1, NN, 0>(c, grid, a);                                     \
+    else launch_div_r2c_n<R, 0, NN, 0>(c, grid, a);
     FEN_DISPATCH_N(c->nx, L_DIV)
 #undef L_DIV
     prof_end(c, K_DIV_R2C, bytes);
@@ -649,9 +661,15 @@ template <class R> void launch_thomas(fen_ctx *c)
 }
 
 // Workgroups of the x c2r (each writes one block sum when the mean is taken).
+template <class R, int N> static void launch_c2r_n(const fen_ctx *c, long nblk, const C2RArgs<R> &a)
+{
+    if constexpr (N == 512) hipLaunchKernelGGL((c2r_kernel<R, N, 512>), dim3((unsigned)nblk), dim3(512), 0, c->stream, a);
+    else hipLaunchKernelGGL((c2r_kernel<R, N>), dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, a);
+}
+
 long c2r_blocks(const fen_ctx *c)
 {
-    const int tj = 2 * (kBlock / (c->nx / 8));
+    const int tj = 2 * (x_pass_threads(c) / (c->nx / 8));
     return ((long)c->ny * c->nz + tj - 1) / tj;
 }
 
@@ -681,7 +699,7 @@ template <class R> void launch_c2r(fen_ctx *c, R *phi, int gl_out, long sx, long
     const long nblk = c2r_blocks(c);
     if (want_sum && (size_t)nblk > n_part) throw std::runtime_error("partial-sum buffer too small");
     prof_begin(c, K_C2R);
-#define L_C(NN) hipLaunchKernelGGL((c2r_kernel<R, NN>), dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, a);
+#define L_C(NN) launch_c2r_n<R, NN>(c, nblk, a);
     FEN_DISPATCH_N(c->nx, L_C)
 #undef L_C
     prof_end(c, K_C2R, spec_bytes<R>(c) + sizeof(R) * c->n_int);
