@@ -43,7 +43,7 @@ _PATTERNS = [
     (r"adj_momentum_kernel", "adj_momentum"),
     (r"adj_project_kernel", "adj_project"),
     (r"adj_rhs_kernel", "adj_rhs"),
-    (r"dct2_rows_kernel", "dct2"),
+    (r"dct2_(rows|pairs)_kernel", "dct2"),
     (r"dct3_rows_kernel", "dct3"),
     (r"rfft_rows_kernel|irfft_rows_kernel", "rfft_y"),
     (r"c2r_kernel", "c2r"),
