@@ -62,6 +62,13 @@ STEP_CASES = {
     "duct3d_npn": lambda: cases.duct3d(32, 16, 24, steps=20),
     "duct3d_npn_inflow": lambda: cases.duct3d(16, 32, 20, z_io=True, steps=20),
     "cavity3d_nnn": lambda: cases.cavity3d(16, 16, 20, steps=20),
+    # the DCT-II's register-loading form at the ends of its range of
+    # transform lengths (x: 8, 128, 512; y: 8, 128) and with an odd row count
+    "cavity3d_nnn_x8": lambda: cases.cavity3d(8, 8, 6, steps=10),
+    "cavity3d_nnn_x128": lambda: cases.cavity3d(128, 16, 8, steps=6),
+    "cavity3d_nnn_y128": lambda: cases.cavity3d(16, 128, 6, steps=6),
+    "duct3d_npn_x512": lambda: cases.duct3d(512, 8, 6, steps=4),
+    "cavity2d_nn_odd_ny": lambda: cases.cavity2d(64, 25, steps=20),
 }
 
 
