@@ -203,8 +203,6 @@ template <class R> struct YArgs {
 
 // Scales X by s1 and divides by s2 in FP64, rounding once to R (for R =
 // double these are the reference's two roundings, in its order).
-// Scales X by s1 and divides by s2 in FP64, rounding once to R (for R =
-// double these are the reference's two roundings, in its order).
 template <class C> __device__ __forceinline__ C scale_div(C X, double s1, double s2)
 {
     using R = decltype(X.x);
