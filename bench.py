@@ -509,6 +509,9 @@ def main():
         "step_roofline": {"words_per_cell": WORDS_3D, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
         "kernel_us_per_launch": r64["breakdown"],
+        "kernel_us_note": "kernel_us_per_launch: untimed calibration steps with an event pair around every "
+                          "launch (each pair drains the stream, so these read high); roofline.avg_launch_ms "
+                          "samples the dominant kernel alone inside the timed region",
         "maxdiv": r64["maxdiv"],
         "dropin": dropin,
         "fp32_refine": fp32,
