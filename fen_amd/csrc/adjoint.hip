@@ -102,56 +102,120 @@ struct AdjMomArgs {
     double id, id2, mu_rho;
 };
 
-// ub = ub* + J_adv^T dvb + (mu/rho) lap(dvb), gathered per face.
-__global__ __launch_bounds__(kAX * kAY) void adj_momentum_kernel(AdjDims d, AdjMomArgs a)
+// ub = ub* + J_adv^T dvb + (mu/rho) lap(dvb), gathered per face, as a
+// k-march: a 64x4 tile of cells walks kMK planes of z with planes k-1, k,
+// k+1 of u, v, w (ghosted, images in place) and of the three dvb arrays
+// (periodic wrap) in an LDS ring, so each value is read from memory once per
+// tile instead of once per use (up to 13 uses per cell; a one-thread-per-cell
+// form reading every operand from memory took 87 us against 60 us at 128^3).
+// Plane k+2 is loaded into registers while plane k is computed.
+constexpr int kMX = 64, kMY = 4, kMK = 16;
+constexpr int kMLX = kMX + 2, kMN = kMLX * (kMY + 2);      // haloed tile
+constexpr int kMPER = (kMN + kMX * kMY - 1) / (kMX * kMY);   // tile elements per thread
+
+__global__ __launch_bounds__(kMX * kMY) void adj_momentum_march_kernel(AdjDims d, AdjMomArgs a)
 {
-    const int i = blockIdx.x * kAX + threadIdx.x % kAX, j = blockIdx.y * kAY + threadIdx.x / kAX, k = blockIdx.z;
-    if (i >= d.nx || j >= d.ny) return;
+    __shared__ double lds[6 * 3 * kMN];   // [field u v w A B E][slot][tile]
+    const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY, k0 = blockIdx.z * kMK;
+    const int kend = min(k0 + kMK, d.nz);
+    const int tx = threadIdx.x % kMX, ty = threadIdx.x / kMX;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = i < d.nx && j < d.ny;
     const int nx = d.nx, ny = d.ny, nz = d.nz;
-    const int ip = up(i, nx), im = dn(i, nx), jp = up(j, ny), jm = dn(j, ny), kp = up(k, nz), km = dn(k, nz);
-    const double *A = a.dvb[0], *Bv = a.dvb[1], *E = a.dvb[2];
+    // tile element e at (i0 - 1 + e % kMLX, j0 - 1 + e / kMLX); past the grid
+    // clamped to the last ghost (values no active cell reads)
+    long og[kMPER], oq[kMPER];   // ghosted and interior (wrapped) plane offsets
+#pragma unroll
+    for (int m = 0; m < kMPER; m++) {
+        const int e = min((int)threadIdx.x + m * kMX * kMY, kMN - 1);
+        const int ii = min(i0 - 1 + e % kMLX, nx), jj = min(j0 - 1 + e / kMLX, ny);
+        og[m] = (ii + 1) + d.gx * (jj + 1);
+        oq[m] = (ii < 0 ? nx - 1 : (ii == nx ? 0 : ii)) + (long)nx * (jj < 0 ? ny - 1 : (jj == ny ? 0 : jj));
+    }
+    const double *Af = a.dvb[0], *Bf = a.dvb[1], *Ef = a.dvb[2];
+    double raw[6][kMPER];
+    auto fetch = [&](int kk) {   // plane kk in [-1, nz]
+        const long kg = d.gxy * (kk + 1), kq = (long)nx * ny * (kk < 0 ? nz - 1 : (kk == nz ? 0 : kk));
+#pragma unroll
+        for (int m = 0; m < kMPER; m++) {
+            raw[0][m] = a.u[og[m] + kg];
+            raw[1][m] = a.v[og[m] + kg];
+            raw[2][m] = a.w[og[m] + kg];
+            raw[3][m] = Af[oq[m] + kq];
+            raw[4][m] = Bf[oq[m] + kq];
+            raw[5][m] = Ef[oq[m] + kq];
+        }
+    };
+    auto commit = [&](int slot) {
+#pragma unroll
+        for (int m = 0; m < kMPER; m++) {
+            const int e = (int)threadIdx.x + m * kMX * kMY;
+            if (e < kMN) {
+#pragma unroll
+                for (int f = 0; f < 6; f++) lds[(f * 3 + slot) * kMN + e] = raw[f][m];
+            }
+        }
+    };
+    fetch(k0 - 1);
+    commit(0);
+    fetch(k0);
+    commit(1);
+    fetch(k0 + 1);
+    int sm = 0, sc = 1, sn = 2;
+    const int c = (ty + 1) * kMLX + tx + 1;
     const double id = a.id;
-    // state at offsets from (i,j,k); ghost layers hold the periodic images
-    auto U = [&](int di, int dj, int dk) { return a.u[d.g(i + di, j + dj, k + dk)]; };
-    auto V = [&](int di, int dj, int dk) { return a.v[d.g(i + di, j + dj, k + dk)]; };
-    auto W = [&](int di, int dj, int dk) { return a.w[d.g(i + di, j + dj, k + dk)]; };
-    // adjoint values with periodic wrap (0-based interior indices)
-    auto X = [&](const double *f, int ii, int jj, int kk) { return f[d.q(ii, jj, kk)]; };
-    // flux adjoints
-    auto Qxx = [&](int ii, int jj, int kk) { return (X(A, up(ii, nx), jj, kk) - X(A, ii, jj, kk)) * id; };
-    auto Qyy = [&](int ii, int jj, int kk) { return (X(Bv, ii, up(jj, ny), kk) - X(Bv, ii, jj, kk)) * id; };
-    auto Qzz = [&](int ii, int jj, int kk) { return (X(E, ii, jj, up(kk, nz)) - X(E, ii, jj, kk)) * id; };
-    auto Rxy = [&](int ii, int jj, int kk) {
-        return (X(A, ii, up(jj, ny), kk) - X(A, ii, jj, kk)) * id + (X(Bv, up(ii, nx), jj, kk) - X(Bv, ii, jj, kk)) * id;
-    };
-    auto Rxz = [&](int ii, int jj, int kk) {
-        return (X(A, ii, jj, up(kk, nz)) - X(A, ii, jj, kk)) * id + (X(E, up(ii, nx), jj, kk) - X(E, ii, jj, kk)) * id;
-    };
-    auto Ryz = [&](int ii, int jj, int kk) {
-        return (X(Bv, ii, jj, up(kk, nz)) - X(Bv, ii, jj, kk)) * id + (X(E, ii, up(jj, ny), kk) - X(E, ii, jj, kk)) * id;
-    };
-    auto lap = [&](const double *f) {
-        const double c = X(f, i, j, k);
-        return ((X(f, ip, j, k) - 2.0 * c + X(f, im, j, k)) + (X(f, i, jp, k) - 2.0 * c + X(f, i, jm, k)) +
-                (X(f, i, j, kp) - 2.0 * c + X(f, i, j, km))) * a.id2;
-    };
-    const long q = d.q(i, j, k);
-    const double u0 = U(0, 0, 0), v0 = V(0, 0, 0), w0 = W(0, 0, 0);
-    // u face
-    double gu = Qxx(i, j, k) * 0.5 * (u0 + U(1, 0, 0)) + Qxx(im, j, k) * 0.5 * (U(-1, 0, 0) + u0);
-    gu += Rxy(i, j, k) * 0.25 * (V(1, 0, 0) + v0) + Rxy(i, jm, k) * 0.25 * (V(1, -1, 0) + V(0, -1, 0));
-    gu += Rxz(i, j, k) * 0.25 * (W(1, 0, 0) + w0) + Rxz(i, j, km) * 0.25 * (W(1, 0, -1) + W(0, 0, -1));
-    // v face
-    double gv = Qyy(i, j, k) * 0.5 * (v0 + V(0, 1, 0)) + Qyy(i, jm, k) * 0.5 * (V(0, -1, 0) + v0);
-    gv += Rxy(i, j, k) * 0.25 * (U(0, 1, 0) + u0) + Rxy(im, j, k) * 0.25 * (U(-1, 1, 0) + U(-1, 0, 0));
-    gv += Ryz(i, j, k) * 0.25 * (W(0, 1, 0) + w0) + Ryz(i, j, km) * 0.25 * (W(0, 1, -1) + W(0, 0, -1));
-    // w face
-    double gw = Qzz(i, j, k) * 0.5 * (w0 + W(0, 0, 1)) + Qzz(i, j, km) * 0.5 * (W(0, 0, -1) + w0);
-    gw += Rxz(i, j, k) * 0.25 * (U(0, 0, 1) + u0) + Rxz(im, j, k) * 0.25 * (U(-1, 0, 1) + U(-1, 0, 0));
-    gw += Ryz(i, j, k) * 0.25 * (V(0, 0, 1) + v0) + Ryz(i, jm, k) * 0.25 * (V(0, -1, 1) + V(0, -1, 0));
-    a.ub[0][q] = a.us[0][q] + gu + a.mu_rho * lap(A);
-    a.ub[1][q] = a.us[1][q] + gv + a.mu_rho * lap(Bv);
-    a.ub[2][q] = a.us[2][q] + gw + a.mu_rho * lap(E);
+    for (int k = k0; k < kend; k++) {
+        commit(sn);   // plane k + 1
+        __syncthreads();
+        if (k + 2 <= nz && k + 2 <= kend) fetch(k + 2);
+        if (active) {
+            auto at = [&](int f, int di, int dj, int dk) {
+                const int sl = dk < 0 ? sm : (dk > 0 ? sn : sc);
+                return lds[(f * 3 + sl) * kMN + c + dj * kMLX + di];
+            };
+            auto U = [&](int di, int dj, int dk) { return at(0, di, dj, dk); };
+            auto V = [&](int di, int dj, int dk) { return at(1, di, dj, dk); };
+            auto W = [&](int di, int dj, int dk) { return at(2, di, dj, dk); };
+            auto A = [&](int di, int dj, int dk) { return at(3, di, dj, dk); };
+            auto B = [&](int di, int dj, int dk) { return at(4, di, dj, dk); };
+            auto E = [&](int di, int dj, int dk) { return at(5, di, dj, dk); };
+            // flux adjoints at offsets from (i, j, k)
+            auto Qxx = [&](int di, int dj, int dk) { return (A(di + 1, dj, dk) - A(di, dj, dk)) * id; };
+            auto Qyy = [&](int di, int dj, int dk) { return (B(di, dj + 1, dk) - B(di, dj, dk)) * id; };
+            auto Qzz = [&](int di, int dj, int dk) { return (E(di, dj, dk + 1) - E(di, dj, dk)) * id; };
+            auto Rxy = [&](int di, int dj, int dk) {
+                return (A(di, dj + 1, dk) - A(di, dj, dk)) * id + (B(di + 1, dj, dk) - B(di, dj, dk)) * id;
+            };
+            auto Rxz = [&](int di, int dj, int dk) {
+                return (A(di, dj, dk + 1) - A(di, dj, dk)) * id + (E(di + 1, dj, dk) - E(di, dj, dk)) * id;
+            };
+            auto Ryz = [&](int di, int dj, int dk) {
+                return (B(di, dj, dk + 1) - B(di, dj, dk)) * id + (E(di, dj + 1, dk) - E(di, dj, dk)) * id;
+            };
+            auto lap = [&](int f) {
+                const double c0 = at(f, 0, 0, 0);
+                return ((at(f, 1, 0, 0) - 2.0 * c0 + at(f, -1, 0, 0)) + (at(f, 0, 1, 0) - 2.0 * c0 + at(f, 0, -1, 0)) +
+                        (at(f, 0, 0, 1) - 2.0 * c0 + at(f, 0, 0, -1))) * a.id2;
+            };
+            const long q = d.q(i, j, k);
+            const double u0 = U(0, 0, 0), v0 = V(0, 0, 0), w0 = W(0, 0, 0);
+            double gu = Qxx(0, 0, 0) * 0.5 * (u0 + U(1, 0, 0)) + Qxx(-1, 0, 0) * 0.5 * (U(-1, 0, 0) + u0);
+            gu += Rxy(0, 0, 0) * 0.25 * (V(1, 0, 0) + v0) + Rxy(0, -1, 0) * 0.25 * (V(1, -1, 0) + V(0, -1, 0));
+            gu += Rxz(0, 0, 0) * 0.25 * (W(1, 0, 0) + w0) + Rxz(0, 0, -1) * 0.25 * (W(1, 0, -1) + W(0, 0, -1));
+            double gv = Qyy(0, 0, 0) * 0.5 * (v0 + V(0, 1, 0)) + Qyy(0, -1, 0) * 0.5 * (V(0, -1, 0) + v0);
+            gv += Rxy(0, 0, 0) * 0.25 * (U(0, 1, 0) + u0) + Rxy(-1, 0, 0) * 0.25 * (U(-1, 1, 0) + U(-1, 0, 0));
+            gv += Ryz(0, 0, 0) * 0.25 * (W(0, 1, 0) + w0) + Ryz(0, 0, -1) * 0.25 * (W(0, 1, -1) + W(0, 0, -1));
+            double gw = Qzz(0, 0, 0) * 0.5 * (w0 + W(0, 0, 1)) + Qzz(0, 0, -1) * 0.5 * (W(0, 0, -1) + w0);
+            gw += Rxz(0, 0, 0) * 0.25 * (U(0, 0, 1) + u0) + Rxz(-1, 0, 0) * 0.25 * (U(-1, 0, 1) + U(-1, 0, 0));
+            gw += Ryz(0, 0, 0) * 0.25 * (V(0, 0, 1) + v0) + Ryz(0, -1, 0) * 0.25 * (V(0, -1, 1) + V(0, -1, 0));
+            a.ub[0][q] = a.us[0][q] + gu + a.mu_rho * lap(3);
+            a.ub[1][q] = a.us[1][q] + gv + a.mu_rho * lap(4);
+            a.ub[2][q] = a.us[2][q] + gw + a.mu_rho * lap(5);
+        }
+        __syncthreads();   // every read of slot sm precedes the next commit into it
+        const int t = sm;
+        sm = sc; sc = sn; sn = t;
+    }
 }
 
 }  // namespace
@@ -243,7 +307,9 @@ void adjoint_step(fen_ctx *c, AdjointTape *t, const double *st, double **ub, dou
     ma.id = id;
     ma.id2 = 1.0 / (c->delta * c->delta);
     ma.mu_rho = c->viscosity / rho;
-    hipLaunchKernelGGL(adj_momentum_kernel, grid, blk, 0, c->stream, d, ma);
+    const dim3 mgrid((unsigned)((c->nx + kMX - 1) / kMX), (unsigned)((c->ny + kMY - 1) / kMY),
+                     (unsigned)((c->nz + kMK - 1) / kMK));
+    hipLaunchKernelGGL(adj_momentum_march_kernel, mgrid, dim3(kMX * kMY), 0, c->stream, d, ma);
 }
 
 }  // namespace

@@ -26,7 +26,7 @@ import sys
 _PATTERNS = [
     (r"project_momentum_kernel", "project_momentum"),
     (r"wall_state_kernel", "wall_state"),
-    (r"momentum_march_kernel|momentum_kernel_2d", "momentum"),
+    (r"(?<!adj_)momentum_march_kernel|momentum_kernel_2d", "momentum"),
     (r"project_march_kernel|project_kernel_2d", "project"),
     (r"div_r2c_kernel", "div_r2c"),
     (r"y_kernel<(?:double|float), \d+, -1, 0>", "y_fwd"),
@@ -40,7 +40,7 @@ _PATTERNS = [
     (r"vof_finish_kernel", "vof_finish"),
     (r"mf_fields_kernel", "mf_fields"),
     (r"mf_momentum_kernel", "mf_momentum"),
-    (r"adj_momentum_kernel", "adj_momentum"),
+    (r"adj_momentum_(march_)?kernel", "adj_momentum"),
     (r"adj_project_kernel", "adj_project"),
     (r"adj_rhs_kernel", "adj_rhs"),
     (r"dct2_(rows|pairs)_kernel", "dct2"),
