@@ -164,6 +164,8 @@ __global__ __launch_bounds__(kNB) void dct2_rows_kernel(DctArgs a)
 //   and q (even) (ghosted x index = element + 1); the odd one goes to thread
 //   T-j, and thread 0 takes its own from chunk r+1 (lane 0 loads a fifth
 //   pair, q = N, for x_{N-1}).  Needs 16-byte aligned fields and even gx.
+// MKL: the (cos, sin) table of the last step in LDS (it costs the x pass an
+// LDS-bound workgroup per CU, so only the y pass takes it).
 template <int N, int SRC, int MKL>
 __global__ __launch_bounds__(kNB) void dct2_pairs_kernel(DctArgs a)
 {
