@@ -34,7 +34,12 @@ def _ke_grad(run, u0):
     return [t.grad.numpy() for t in x]
 
 
-def test_gpu_adjoint_matches_oracle_adjoint():
+@pytest.mark.parametrize("shape", [None, (16, 8, 32)])
+def test_gpu_adjoint_matches_oracle_adjoint(shape, monkeypatch):
+    # (16, 8, 32): the transposed momentum marches z in two chunks of planes
+    if shape is not None:
+        monkeypatch.setattr(ref, "N", shape)
+        monkeypatch.setattr(ref, "L", (2 * np.pi, np.pi, 4 * np.pi))
     o, dt = ref._oracle()
     state = ref._initial()
     nsteps = 6
